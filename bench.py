@@ -1,0 +1,161 @@
+"""RK3 steps/s of the shear-periodic box on one MI355X.
+
+Workload: 384x385x384 (mgalx x my x mgalz), Lx:Ly:Lz = 2:1:1, S = 1,
+Re_z = 8000, CFL 0.5 adaptive dt, explicit viscous terms, FP64, seeded
+k^4 exp(-2 (k/k0)^2) solenoidal field (synthetic).  A step is one full RK3
+step (3 substeps) of the whole box with the state resident in HBM.
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" members for the
+dominant kernel's HBM roofline and the CPU oracle timed on a bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=str, default="384,385,384")
+    ap.add_argument("--Re", type=float, default=8000.0)
+    ap.add_argument("--explicit", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-grid", type=str, default="",
+                    help="grid of the CPU baseline sample (default: the workload grid)")
+    return ap.parse_args()
+
+
+def algorithmic_bytes(kxn, mz, my, mgalz):
+    """compulsory HBM bytes per launch of each kernel class, FP64 complex = 16 B.
+    Nc = kxn*mz*my spectral coefficients, Nh = kxn*mgalz*my hybrid values."""
+    nc = kxn * mz * my
+    nh = kxn * mgalz * my
+    return {
+        # A: reads vor, phi, v, dv/dy, d(omega_y)/dy; writes 6 hybrid fields
+        "xz_A": 16.0 * (5 * nc + 6 * nh),
+        # B: reads 6 hybrid fields, writes 3
+        "xz_B": 16.0 * (9 * nh),
+        # C: reads 3 hybrid fields and v, writes 3 spectral fields
+        "xz_C": 16.0 * (3 * nh + 4 * nc),
+        # fused y stage, averaged over the 3 substeps: reads dvordy, hv, hg,
+        # vor, phi (+ vorwk, phiwk at rk 2,3); writes vor, phi, vorwk,
+        # phiwk (not at rk 3), hg, dvordy, hv
+        "y_lines": 16.0 * nc * (12 + 14 + 12) / 3.0,
+    }
+
+
+def step_bytes(nc):
+    """B64 = 992 Nc: compulsory bytes of one RK3 step with planes on chip"""
+    return 992.0 * nc
+
+
+def cpu_baseline(grid, Re, explicit, seed):
+    """Times the numpy oracle (one RK3 step) on the host; returns (steps/s, sample)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.grid import Grid
+    from oracle.cross import Cross
+    from shear_nnf_amd.ic import make_ic
+    mgalx, my, mgalz = grid
+    g = Grid(mgalx, my, mgalz, Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0, s=1.0, CFL=0.5)
+    vor, phi, u00, w00 = make_ic(mgalx, my, mgalz, g.alp, g.gam, g.Ly, seed=seed)
+    c = Cross(g, explicit=explicit, nhist=10 ** 9)
+    c.set_state(vor, phi, u00, w00)
+    c.run(1)          # warm-up, includes the first-step v solve
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        c.run(1)
+        n += 1
+        if time.perf_counter() - t0 > 10.0 or n >= 3:
+            break
+    dt = time.perf_counter() - t0
+    return n / dt, f"{n} RK3 step(s) of the numpy oracle at {mgalx}x{my}x{mgalz}, after 1 warm-up step"
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU (sharded) path "
+              "is design-only (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import shear_nnf_amd as snf
+    from shear_nnf_amd.ic import make_ic
+    mgalx, my, mgalz = (int(x) for x in a.grid.split(","))
+    seed = 20260001
+    vor, phi, u00, w00 = make_ic(mgalx, my, mgalz, math.pi, 2 * math.pi, 1.0, seed=seed)
+    box = snf.ShearBox(mgalx, my, mgalz, a.Re, explicit=a.explicit, nhist=10 ** 9)
+    box.set_state(vor, phi, u00, w00)
+    del vor, phi
+    st = box.advance_async_status(a.warmup)          # warm-up (synchronises at the end)
+    box.set_timing(True)
+    t0 = time.perf_counter()
+    st = box.advance_async_status(a.steps)           # one call = K steps, synchronised
+    t1 = time.perf_counter()
+    timing = box.get_timing()
+    box.set_timing(False)
+    # an untimed-instrumentation pass for the headline number
+    t2 = time.perf_counter()
+    st2 = box.advance_async_status(a.steps)
+    t3 = time.perf_counter()
+    steps_done = st2.steps_done
+    wall = t3 - t2
+    sps = steps_done / wall
+    nc = box.nc
+    ab = algorithmic_bytes(box.kxn, box.mz, my, mgalz)
+    per_cls = {}
+    for k, (ms, cnt) in timing.items():
+        if cnt:
+            per_cls[k] = ms / cnt
+    dom = max((k for k in per_cls if k in ab), key=lambda k: timing[k][0])
+    achieved = ab[dom] / (per_cls[dom] * 1e-3) / 1e9
+    out = {
+        "metric": "RK3 steps/s at 384^3 (FP64), % HBM peak",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / max(steps_done, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded k^4 exp(-2(k/k0)^2) solenoidal field, k0=4)",
+        "config": {"workload": f"shear-periodic box DNS {mgalx}x{my}x{mgalz}, Lx:Ly:Lz=2:1:1, "
+                               f"S=1, Re_z={a.Re:g}, CFL 0.5 adaptive dt, "
+                               f"{'explicit' if a.explicit else 'implicit'} viscous, FP64",
+                   "grid": [mgalx, my, mgalz], "Nc": nc},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": ab[dom],
+                     "avg_launch_ms": per_cls[dom]},
+        "step_roofline": {"bytes_per_step_B64": step_bytes(nc),
+                          "achieved_GBs": step_bytes(nc) * sps / 1e9,
+                          "frac": step_bytes(nc) * sps / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms_per_launch": per_cls,
+        "kernel_ms_per_step": {k: timing[k][0] / max(st.steps_done, 1) for k in timing},
+        "timed_pass_steps_per_s": st.steps_done / (t1 - t0),
+        "sim_time_advanced": st2.time,
+        "Deltat": st2.Deltat,
+    }
+    if not a.no_cpu:
+        cg = tuple(int(x) for x in a.cpu_grid.split(",")) if a.cpu_grid else (mgalx, my, mgalz)
+        v, sample = cpu_baseline(cg, a.Re, a.explicit, seed)
+        ncores = 1
+        out["cpu_baseline"] = {"value": v, "unit": "steps/s", "cores": ncores, "kind": "port",
+                               "sample": sample + " (numpy FFT and line solves, single thread)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,163 @@
+/*
+ * shear_rk3.h -- C ABI of the MI355X (gfx950) RK3 time advance for the
+ * shear-periodic box (drop-in for `cross` of hiiragiyua/shear_nnf).
+ *
+ * The reference entry point is the Fortran subroutine
+ *   cross(vor,phi,u00,w00,hv,hg,phiwk,vorwk,dvordy,rhvc1,rhvc2,rhgc1,rhgc2,chwk)
+ *   (cross.f90:3-4), whose grid, run-control and boundary state live in the
+ *   modules ctes / running / bcs (mod.f90:1-130).  Here that state is a
+ *   POD parameter block given once at creation, a device-resident handle,
+ *   and explicit upload / advance / download calls.  Every function returns
+ *   an snf_status code and never exits the process.
+ *
+ * Host arrays in the reference layout ("jik", cross.f90:32-33):
+ *   vor, phi : complex(0:my-1, 0:mx/2-1, 0:mz-1) stored as interleaved
+ *              real/imag doubles, y fastest, then kx, then kz
+ *   u00, w00 : real(0:my-1)
+ * Device storage is [y][kx][kz] (kz fastest); the handle transposes on
+ * upload/download.  All arithmetic is IEEE binary64 unless the handle was
+ * created with SNF_PREC_MIXED.
+ */
+#ifndef SHEAR_RK3_H
+#define SHEAR_RK3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct snf_handle snf_handle;
+
+enum snf_status_code {
+  SNF_OK = 0,
+  SNF_BAD_PARAM = 1,     /* sizes or options the path does not support     */
+  SNF_DEVICE_ERROR = 2,  /* a HIP call failed; the message says which       */
+  SNF_NO_DEVICE = 3,     /* no gfx950 device visible                        */
+  SNF_NOT_READY = 4      /* advance before upload                           */
+};
+
+enum snf_stop_code {     /* why an advance returned (cross.f90 istop/idump) */
+  SNF_RUNNING = 0,       /* all requested steps done                         */
+  SNF_STOP = 1,          /* istop: etime reached, NaN, laminar, dt < 1e-10   */
+  SNF_DUMP = 2           /* idump: dumptime reached, caller should write     */
+};
+
+/* Parameters of modules ctes/running/bcs (mod.f90) and hre2.dat/hre3.dat
+ * (initcr.f90:54-135, 1950-1971) that the time advance reads.            */
+typedef struct snf_params {
+  int32_t mgalx, my, mgalz;   /* physical collocation points (hre2.dat)   */
+  double Re;                  /* 1/nu                                     */
+  double alp, gam;            /* fundamental wavenumbers, Lx=2pi/alp ...  */
+  double Ly;                  /* box height                               */
+  double s;                   /* mean shear rate                          */
+  double chi;                 /* rotation (enters u x omega as 2 chi)     */
+  double CFL;                 /* CFL number for the adaptive dt           */
+  int32_t explicit_visc;      /* 1: explicit viscous terms (default);
+                                 0: implicit Helmholtz stages             */
+  int32_t nhist;              /* statistics every nhist steps             */
+  int32_t readflag;           /* 0: generated IC (first dt uses umax>=1)  */
+  double uprim;               /* IC velocity scale (isotropic first dt)   */
+  double vbulk;               /* v of the (0,0) mode (cross.f90:1276)     */
+  double etime, dumptime, dtimag;  /* stop / dump times (cross.f90:1680-1712) */
+  int32_t precision;          /* SNF_PREC_FP64 (0) or SNF_PREC_MIXED (1)  */
+  int32_t device;             /* HIP device ordinal                       */
+} snf_params;
+
+#define SNF_PREC_FP64 0
+#define SNF_PREC_MIXED 1
+
+/* Status returned by an advance: the scalars of module `running`/`bcs`. */
+typedef struct snf_run_status {
+  int32_t steps_done;
+  int32_t stop;               /* snf_stop_code                             */
+  double time, Deltat, xoffb, xofft;
+  double umax[4];             /* umax(0:3) of the last CFL evaluation      */
+  double CFLv;
+  int32_t nhist_records;      /* UNER records produced by this call        */
+} snf_run_status;
+
+/* Number of doubles in one UNER record (.cfbin row, cross.f90:2087):
+ * time, Deltat, uner(1:10), CFL, CFLv, umax(1:3), xoffb, xofft, umax(0),
+ * uner2(1:4). */
+#define SNF_UNER_LEN 24
+
+/* ---- lifetime ---------------------------------------------------------
+ * Replaces initcr(0) grid setup + derivadas/ffti (initcr.f90:186-306,
+ * diffy_shear_hepta.f90:535-573, fou3D_thread.f:1-19).                  */
+int snf_create(const snf_params* p, snf_handle** out);
+int snf_destroy(snf_handle* h);
+const char* snf_last_error(void);
+
+/* sizes: kxn = mx/2 complex kx modes, mz kz modes, nc = my*kxn*mz */
+int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc);
+
+/* ---- state transfer (host jik layout, FP64) ---------------------------
+ * Replaces the argument passing of cross (vor, phi, u00, w00) and the
+ * module scalars time (running) and xoffb/xofft (bcs).                   */
+int snf_state_upload(snf_handle* h, const double* vor, const double* phi,
+                     const double* u00, const double* w00,
+                     double time, double xoffb, double xofft);
+int snf_state_download(snf_handle* h, double* vor, double* phi,
+                       double* u00, double* w00,
+                       double* time, double* xoffb, double* xofft);
+/* same, in the device layout [y][kx][kz] (interleaved complex)           */
+int snf_state_upload_ykxkz(snf_handle* h, const double* vor, const double* phi,
+                           const double* u00, const double* w00,
+                           double time, double xoffb, double xofft);
+int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi,
+                             double* u00, double* w00);
+
+/* ---- the time advance -------------------------------------------------
+ * Advances up to nstep RK3 steps (cross.f90:336-1029); returns early when
+ * istop or idump is raised.  UNER records are kept on the device; fetch
+ * them with snf_history.                                                 */
+int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st);
+/* copies up to max_records records (SNF_UNER_LEN doubles each) produced
+ * since the last call; *n receives the count                             */
+int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n);
+/* per-plane statistics stats(1:23, 0:my-1) and stats2 (cross.f90:1798-2045),
+ * column-major like the reference: out[s + 23*j]                          */
+int snf_statistics(snf_handle* h, double* stats, double* stats2,
+                   int32_t* nacum, double* total_dt);
+
+/* ---- operator entry points for parity tests (host arrays, [y][line]) --
+ * derivyc (diffy_shear_hepta.f90:333-450): order 1 or 2 on nl lines of
+ * length my; line l uses twist shp[l], shm[l].                           */
+int snf_op_derivy(snf_handle* h, const double* u, double* du, int32_t nl,
+                  const double* shp, const double* shm, int32_t order);
+/* lapcdy (diffy_shear_hepta.f90:466-525): (d_yy - k2[l]) p = u           */
+int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl,
+                     const double* k2, const double* shp, const double* shm);
+/* fourxz (fou3D_thread.f:22-73) on nplanes planes: dir=+1 spectral
+ * (kxn x mz complex, [kx][kz]) -> physical (mgalx x mgalz, [x][z]);
+ * dir=-1 the reverse with the 1/(mgalx mgalz) scaling.                   */
+int snf_op_fourxz(snf_handle* h, const double* in, double* out,
+                  int32_t nplanes, int32_t dir);
+/* one hvhg evaluation (cross.f90:1127-1765) on the uploaded state with the
+ * given v, dv/dy and d(omega_y)/dy fields (device layout); returns the
+ * three outputs (ome1c -> dvordy, rhgc -> hg, rhvc -> hv) and rf0u, rf0w. */
+int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv,
+                const double* dvordy, double* out_dvordy, double* out_hg,
+                double* out_hv, double* rf0u, double* rf0w);
+
+/* ---- kernel timing (HIP events on the handle's stream) -----------------
+ * No counterpart in the reference (its TIMING buckets, cross.f90:1041-1047,
+ * are host wall-clock).  When enabled, snf_advance brackets every kernel
+ * launch with events; snf_get_timing returns, per kernel class, the summed
+ * milliseconds and the launch count since the last reset.               */
+enum snf_kernel_class {
+  SNF_K_XZ_A = 0,      /* spectral -> hybrid, inverse z DFT              */
+  SNF_K_XZ_B = 1,      /* x c2r, u x omega, x r2c                        */
+  SNF_K_XZ_C = 2,      /* hybrid -> spectral, forward z DFT, combination */
+  SNF_K_Y = 3,         /* fused y-line substep (explicit or implicit)    */
+  SNF_K_SMALL = 4,     /* scalar/CFL, twist tails, hist, step control    */
+  SNF_K_NCLASS = 5
+};
+int snf_set_timing(snf_handle* h, int32_t on);
+int snf_get_timing(snf_handle* h, double* ms, int32_t* counts, int32_t nclass);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

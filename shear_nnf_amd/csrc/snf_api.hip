@@ -1,0 +1,791 @@
+// C ABI of libshear_rk3 (include/shear_rk3.h): handle lifetime, state
+// transfer, the RK3 advance orchestration and operator entry points.
+//
+// One translation unit: the kernels of snf_xz.hip and snf_ykern.hip are
+// included here so the library is a single hipcc invocation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/shear_rk3.h"
+#include "snf_internal.h"
+#include "snf_setup.h"
+#include "snf_xz.hip"
+#include "snf_ykern.hip"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(SNF_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool plan_of(int n, FFTPlan& p) {
+  p.n = n;
+  p.nr = 0;
+  int m = n;
+  while (m % 4 == 0 && m > 1) { p.r[p.nr++] = 4; m /= 4; }
+  while (m % 2 == 0 && m > 1) { p.r[p.nr++] = 2; m /= 2; }
+  while (m % 3 == 0 && m > 1) { p.r[p.nr++] = 3; m /= 3; }
+  return m == 1 && p.nr <= 12;
+}
+
+std::vector<cplx> twiddles(int n) {
+  std::vector<cplx> t(n);
+  const long double pi2 = 2.0L * 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < n; ++k) {
+    const long double a = pi2 * (long double)k / (long double)n;
+    t[k] = mkc((double)cosl(a), (double)(-sinl(a)));
+  }
+  return t;
+}
+
+}  // namespace
+
+struct snf_handle {
+  snf_params p{};
+  Geom g{};
+  YCoef c{};
+  int dev = 0;
+  long P = 0, PS = 0, NC = 0;
+  hipStream_t stream = nullptr;
+  FFTPlan px{}, pz{};
+  // device memory
+  cplx *vor = nullptr, *phi = nullptr, *hg = nullptr, *hv = nullptr, *dvordy = nullptr;
+  cplx *vorwk = nullptr, *phiwk = nullptr, *s1 = nullptr, *s2 = nullptr;
+  cplx *hyb_in = nullptr, *hyb_out = nullptr;
+  double* hr = nullptr;
+  cplx* hc = nullptr;
+  cplx *u00c = nullptr, *w00c = nullptr, *u00wk = nullptr, *w00wk = nullptr;
+  double *u00r = nullptr, *w00r = nullptr;  // real copies read by the x-z stage
+  double *rf0u = nullptr, *rf0w = nullptr, *dudy = nullptr, *dwdy = nullptr;
+  double* base = nullptr;   // 2 operators x 8 arrays x N
+  TwistTail* tw[2] = {nullptr, nullptr};
+  cplx* shp[2] = {nullptr, nullptr};
+  cplx* shm[2] = {nullptr, nullptr};
+  int cur = 0;
+  cplx *twx = nullptr, *twz = nullptr;
+  DevScalars* sc = nullptr;
+  unsigned long long* umax_acc = nullptr;
+  double *stat_plane = nullptr, *phist = nullptr, *stats = nullptr, *stats2 = nullptr;
+  double* ring = nullptr;
+  int cap = 4096;
+  int nrec_read = 0;
+  bool uploaded = false;
+  BaseFactor b1{}, b2{};
+  TwistTail id1{}, id2{};
+  size_t ldsA = 0, ldsB = 0, ldsC = 0;
+  double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
+  // kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_cls;
+  int ev_used = 0;
+  double t_ms[SNF_K_NCLASS] = {0, 0, 0, 0, 0};
+  int t_cnt[SNF_K_NCLASS] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T) + 256);
+  if (e != hipSuccess) return fail(SNF_DEVICE_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemset(*p, 0, n * sizeof(T) + 256);
+  if (e != hipSuccess) return fail(SNF_DEVICE_ERROR, std::string("hipMemset: ") + hipGetErrorString(e));
+  return SNF_OK;
+}
+
+#define ALLOC(ptr, n)                          \
+  do {                                         \
+    int r_ = dalloc(&(ptr), (size_t)(n));      \
+    if (r_ != SNF_OK) return r_;               \
+  } while (0)
+
+TwistTail identity_tail(const snf_setup::Base& b, int N) {
+  TwistTail t;
+  for (int q = 0; q < 3; ++q) {
+    t.Le[q] = mkc(b.Le[N - 6 + q], 0.0);
+    t.Lf[q] = mkc(b.Lf[N - 5 + q], 0.0);
+    t.Lg[q] = mkc(b.Lg[N - 4 + q], 0.0);
+    t.Dm1[q] = b.Dm1[N - 3 + q];
+  }
+  return t;
+}
+
+YArgs yargs(snf_handle* h) {
+  YArgs a;
+  a.g = h->g;
+  a.c = h->c;
+  a.b1 = h->b1;
+  a.b2 = h->b2;
+  a.id1 = h->id1;
+  a.id2 = h->id2;
+  a.tw_old = h->tw[h->cur];
+  a.tw_new = h->tw[1 - h->cur];
+  a.shp_old = h->shp[h->cur];
+  a.shm_old = h->shm[h->cur];
+  a.shp_new = h->shp[1 - h->cur];
+  a.shm_new = h->shm[1 - h->cur];
+  a.vor = h->vor;
+  a.phi = h->phi;
+  a.hg = h->hg;
+  a.hv = h->hv;
+  a.dvordy = h->dvordy;
+  a.vorwk = h->vorwk;
+  a.phiwk = h->phiwk;
+  a.s1 = h->s1;
+  a.s2 = h->s2;
+  a.hs = HScratch{h->hr, h->hc};
+  a.u00c = h->u00c;
+  a.w00c = h->w00c;
+  a.u00wk = h->u00wk;
+  a.w00wk = h->w00wk;
+  a.rf0u = h->rf0u;
+  a.rf0w = h->rf0w;
+  a.dudy = h->dudy;
+  a.dwdy = h->dwdy;
+  a.sc = h->sc;
+  a.PS = h->PS;
+  return a;
+}
+
+__global__ void k_cplx_to_real(const cplx* a, const cplx* b, double* ra, double* rb, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) {
+    ra[j] = a[j].x;
+    rb[j] = b[j].x;
+  }
+}
+
+// bracket a launch with events when timing is on (pairs: start, stop)
+struct TimedLaunch {
+  snf_handle* h;
+  int cls;
+  int slot = -1;
+  TimedLaunch(snf_handle* hh, int c) : h(hh), cls(c) {
+    if (h->timing && h->ev_used + 2 <= (int)h->ev.size()) {
+      slot = h->ev_used;
+      h->ev_used += 2;
+      h->ev_cls[slot / 2] = cls;
+      (void)hipEventRecord(h->ev[slot], h->stream);
+    }
+  }
+  ~TimedLaunch() {
+    if (slot >= 0) (void)hipEventRecord(h->ev[slot + 1], h->stream);
+  }
+};
+
+void collect_timing(snf_handle* h) {
+  for (int q = 0; q + 1 < h->ev_used; q += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev[q], h->ev[q + 1]) == hipSuccess) {
+      h->t_ms[h->ev_cls[q / 2]] += ms;
+      h->t_cnt[h->ev_cls[q / 2]] += 1;
+    }
+  }
+  h->ev_used = 0;
+}
+
+int launch_xz(snf_handle* h, int rk) {
+  const Geom& g = h->g;
+  // the x-z stage reads U, W as real arrays
+  k_cplx_to_real<<<(g.my + 255) / 256, 256, 0, h->stream>>>(h->u00c, h->w00c, h->u00r, h->w00r, g.my);
+  dim3 gA(g.kxn, g.my);
+  { TimedLaunch tl(h, SNF_K_XZ_A);
+  k_xz_A<<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->hg, h->hv, h->dvordy, h->u00r,
+                                          h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->pz,
+                                          h->twz, h->sc); }
+  if (rk == 1)
+    k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->hg, h->hv, h->dvordy, h->u00r,
+                                          h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->stat_plane,
+                                          h->sc, h->p.nhist);
+  dim3 gB(g.mgalz / 2, g.my);
+  { TimedLaunch tl(h, SNF_K_XZ_B);
+  k_xz_B<<<gB, 256, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk, h->umax_acc,
+                                          h->phist, h->sc, h->p.nhist); }
+  { TimedLaunch tl(h, SNF_K_XZ_C);
+  k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->dvordy, h->hg, h->hv, h->rf0u, h->rf0w,
+                                          h->pz, h->twz, h->sc); }
+  HIPCHK(hipGetLastError());
+  return SNF_OK;
+}
+
+ScalarArgs sargs(snf_handle* h) {
+  ScalarArgs a;
+  a.g = h->g;
+  a.sc = h->sc;
+  a.nhist = h->p.nhist;
+  a.readflag = h->p.readflag;
+  a.explicit_visc = h->p.explicit_visc;
+  a.cflx = h->cflx;
+  a.cfly = h->cfly;
+  a.cflz = h->cflz;
+  a.cflvv = h->cflvv;
+  a.CFL = h->p.CFL;
+  a.uprim = h->p.uprim;
+  a.umax_acc = h->umax_acc;
+  a.stat_plane = h->stat_plane;
+  a.phist = h->phist;
+  a.stats = h->stats;
+  a.stats2 = h->stats2;
+  a.ring = h->ring;
+  a.cap = h->cap;
+  a.shp_new = h->shp[1 - h->cur];
+  a.shm_new = h->shm[1 - h->cur];
+  return a;
+}
+
+int launch_twist(snf_handle* h, int which) {
+  dim3 gt(h->g.kxn, 2);
+  k_twist<<<gt, 64, 0, h->stream>>>(h->g.my, h->b1, h->b2, h->c, h->shp[which], h->tw[which],
+                                    h->g.kxn, h->sc);
+  HIPCHK(hipGetLastError());
+  return SNF_OK;
+}
+
+unsigned ylines_blocks(const snf_handle* h) { return (unsigned)((h->P + 2 + 127) / 128); }
+
+
+}  // namespace
+
+extern "C" {
+
+const char* snf_last_error(void) { return g_err.c_str(); }
+
+int snf_create(const snf_params* p, snf_handle** out) {
+  if (!p || !out) return fail(SNF_BAD_PARAM, "null argument");
+  *out = nullptr;
+  const int mgalx = p->mgalx, my = p->my, mgalz = p->mgalz;
+  if (my < 16 || my > SNF_MAXN) return fail(SNF_BAD_PARAM, "my must be in [16, 1032]");
+  if (mgalx < 12 || mgalz < 12 || mgalx > SNF_MAXFFT || mgalz > SNF_MAXFFT || (mgalz % 2) != 0 ||
+      (mgalx % 2) != 0)
+    return fail(SNF_BAD_PARAM, "mgalx, mgalz must be even and in [12, 1024]");
+  if (p->nhist < 1) return fail(SNF_BAD_PARAM, "nhist must be >= 1");
+  if (!(p->Re > 0) || !(p->alp > 0) || !(p->gam > 0) || !(p->Ly > 0) || !(p->CFL > 0))
+    return fail(SNF_BAD_PARAM, "Re, alp, gam, Ly, CFL must be positive");
+  if (p->precision != SNF_PREC_FP64) return fail(SNF_BAD_PARAM, "only SNF_PREC_FP64 is built");
+  snf_handle* h = new snf_handle();
+  h->p = *p;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    delete h;
+    return fail(SNF_NO_DEVICE, "no HIP device");
+  }
+  h->dev = p->device;
+  if (hipSetDevice(h->dev) != hipSuccess) {
+    delete h;
+    return fail(SNF_NO_DEVICE, "hipSetDevice failed");
+  }
+  Geom& g = h->g;
+  g.my = my;
+  g.mgalx = mgalx;
+  g.mgalz = mgalz;
+  const int mx = 2 * (mgalx / 3);
+  g.mz = 2 * (mgalz / 3) - 1;
+  g.kxn = mx / 2;
+  g.nz = (g.mz - 1) / 2;
+  g.P = (long)g.kxn * g.mz;
+  g.alp = p->alp;
+  g.gam = p->gam;
+  g.Ly = p->Ly;
+  g.dy = p->Ly / my;
+  g.s = p->s;
+  g.chi = p->chi;
+  g.Re = p->Re;
+  h->P = g.P;
+  h->PS = g.P + 2;
+  h->NC = g.P * my;
+  if (!plan_of(mgalx, h->px) || !plan_of(mgalz, h->pz)) {
+    delete h;
+    return fail(SNF_BAD_PARAM, "mgalx and mgalz must be of the form 2^a 3^b");
+  }
+  // CFL constants (initcr.f90:474-495)
+  const double pi = 4.0 * std::atan(1.0);
+  const double dx = 2 * pi / g.alp / mx, dz = 2 * pi / g.gam / g.mz, dy = g.dy;
+  h->cflx = pi / p->CFL / dx;
+  h->cflz = pi / p->CFL / dz;
+  h->cfly = 2.5 / p->CFL / dy;
+  h->cflvv = 1.0 / p->Re / p->CFL * std::fmax(std::fmax(pi * pi / dx / dx, 6.25 / dy / dy), pi * pi / dz / dz);
+  // compact coefficients (diffy_shear_hepta.f90:68-296, update_dy 569-571)
+  double pr1[7], rh1h[7], pr2[7], rh2h[7];
+  snf_setup::design_first(pr1, rh1h);
+  snf_setup::design_second(pr2, rh2h);
+  for (int m = 0; m < 7; ++m) {
+    h->c.pr1[m] = pr1[m];
+    h->c.pr2[m] = pr2[m];
+    h->c.rh1[m] = rh1h[m] / dy;
+    h->c.rh2[m] = rh2h[m] / (dy * dy);
+  }
+  const snf_setup::Base f1 = snf_setup::base_factor(pr1, my);
+  const snf_setup::Base f2 = snf_setup::base_factor(pr2, my);
+  h->id1 = identity_tail(f1, my);
+  h->id2 = identity_tail(f2, my);
+  int rc;
+  auto A = [&](auto** ptr, size_t n) { return dalloc(ptr, n); };
+#define TRY(x)                   \
+  do {                           \
+    rc = (x);                    \
+    if (rc != SNF_OK) {          \
+      snf_destroy(h);            \
+      return rc;                 \
+    }                            \
+  } while (0)
+  TRY(hipStreamCreate(&h->stream) == hipSuccess ? SNF_OK : fail(SNF_DEVICE_ERROR, "stream"));
+  const size_t NC = h->NC, NS = (size_t)my * h->PS;
+  TRY(A(&h->vor, NC));
+  TRY(A(&h->phi, NC));
+  TRY(A(&h->hg, NC));
+  TRY(A(&h->hv, NC));
+  TRY(A(&h->dvordy, NC));
+  TRY(A(&h->vorwk, NC));
+  TRY(A(&h->phiwk, NC));
+  TRY(A(&h->s1, NS));
+  TRY(A(&h->s2, NS));
+  const size_t NH = (size_t)my * g.kxn * mgalz;
+  TRY(A(&h->hyb_in, 6 * NH));
+  TRY(A(&h->hyb_out, 3 * NH));
+  TRY(A(&h->hr, 4 * NS));
+  TRY(A(&h->hc, 3 * NS));
+  TRY(A(&h->u00c, my));
+  TRY(A(&h->w00c, my));
+  TRY(A(&h->u00wk, my));
+  TRY(A(&h->w00wk, my));
+  TRY(A(&h->u00r, my));
+  TRY(A(&h->w00r, my));
+  TRY(A(&h->rf0u, my));
+  TRY(A(&h->rf0w, my));
+  TRY(A(&h->dudy, my));
+  TRY(A(&h->dwdy, my));
+  TRY(A(&h->base, 16 * (size_t)my));
+  for (int q = 0; q < 2; ++q) {
+    TRY(A(&h->tw[q], 2 * (size_t)g.kxn));
+    TRY(A(&h->shp[q], (size_t)g.kxn));
+    TRY(A(&h->shm[q], (size_t)g.kxn));
+  }
+  TRY(A(&h->twx, (size_t)mgalx));
+  TRY(A(&h->twz, (size_t)mgalz));
+  TRY(A(&h->sc, 1));
+  TRY(A(&h->umax_acc, 4));
+  TRY(A(&h->stat_plane, (size_t)my * STAT_W));
+  TRY(A(&h->phist, (size_t)my * (mgalz / 2) * 3));
+  TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
+  TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
+  TRY(A(&h->ring, (size_t)h->cap * 24));
+  // base factors -> device
+  {
+    std::vector<double> hb(16 * (size_t)my);
+    const std::vector<double>* src1[8] = {&f1.D, &f1.Dm1, &f1.La, &f1.Lb, &f1.Lc, &f1.Le, &f1.Lf, &f1.Lg};
+    const std::vector<double>* src2[8] = {&f2.D, &f2.Dm1, &f2.La, &f2.Lb, &f2.Lc, &f2.Le, &f2.Lf, &f2.Lg};
+    for (int q = 0; q < 8; ++q) {
+      std::memcpy(&hb[(size_t)q * my], src1[q]->data(), my * sizeof(double));
+      std::memcpy(&hb[(size_t)(8 + q) * my], src2[q]->data(), my * sizeof(double));
+    }
+    TRY(hipMemcpy(h->base, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+            ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy base"));
+    double* d = h->base;
+    h->b1 = BaseFactor{d, d + my, d + 2 * my, d + 3 * my, d + 4 * my, d + 5 * my, d + 6 * my, d + 7 * my};
+    d += 8 * my;
+    h->b2 = BaseFactor{d, d + my, d + 2 * my, d + 3 * my, d + 4 * my, d + 5 * my, d + 6 * my, d + 7 * my};
+  }
+  {
+    std::vector<cplx> tx = twiddles(mgalx), tz = twiddles(mgalz);
+    TRY(hipMemcpy(h->twx, tx.data(), tx.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
+            ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy twiddles"));
+    TRY(hipMemcpy(h->twz, tz.data(), tz.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
+            ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy twiddles"));
+  }
+  {
+    DevScalars s{};
+    s.etime = p->etime;
+    s.dumptime = p->dumptime;
+    s.dtimag = p->dtimag;
+    TRY(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess
+            ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy scalars"));
+  }
+  h->ldsA = 2 * 3 * (size_t)mgalz * sizeof(cplx);
+  h->ldsB = 2 * 6 * (size_t)mgalx * sizeof(cplx);
+  h->ldsC = h->ldsA;
+  hipFuncSetAttribute((const void*)k_xz_A, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
+  hipFuncSetAttribute((const void*)k_xz_B, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
+  hipFuncSetAttribute((const void*)k_xz_C, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
+#undef TRY
+  *out = h;
+  return SNF_OK;
+}
+
+int snf_destroy(snf_handle* h) {
+  if (!h) return SNF_OK;
+  hipSetDevice(h->dev);
+  void* ptrs[] = {h->vor, h->phi, h->hg, h->hv, h->dvordy, h->vorwk, h->phiwk, h->s1, h->s2,
+                  h->hyb_in, h->hyb_out, h->hr, h->hc, h->u00c, h->w00c, h->u00wk, h->w00wk,
+                  h->u00r, h->w00r, h->rf0u, h->rf0w, h->dudy, h->dwdy, h->base, h->tw[0],
+                  h->tw[1], h->shp[0], h->shp[1], h->shm[0], h->shm[1], h->twx, h->twz, h->sc,
+                  h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return SNF_OK;
+}
+
+int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  if (kxn) *kxn = h->g.kxn;
+  if (mz) *mz = h->g.mz;
+  if (nc) *nc = h->NC;
+  return SNF_OK;
+}
+
+static int upload_common(snf_handle* h, const std::vector<cplx>& hv, const std::vector<cplx>& hp,
+                         const double* u00, const double* w00, double time, double xoffb,
+                         double xofft) {
+  hipSetDevice(h->dev);
+  const int my = h->g.my;
+  HIPCHK(hipMemcpy(h->vor, hv.data(), h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->phi, hp.data(), h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+  std::vector<cplx> uc(my), wc(my);
+  for (int j = 0; j < my; ++j) {
+    uc[j] = mkc(u00[j], 0.0);
+    wc[j] = mkc(w00[j], 0.0);
+  }
+  HIPCHK(hipMemcpy(h->u00c, uc.data(), my * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->w00c, wc.data(), my * sizeof(cplx), hipMemcpyHostToDevice));
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  s.time = time;
+  s.xoffb = xoffb;
+  s.xofft = xofft;
+  s.xwkb = xoffb;
+  s.xwkt = xofft;
+  HIPCHK(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+  h->uploaded = true;
+  return SNF_OK;
+}
+
+int snf_state_upload_ykxkz(snf_handle* h, const double* vor, const double* phi, const double* u00,
+                           const double* w00, double time, double xoffb, double xofft) {
+  if (!h || !vor || !phi || !u00 || !w00) return fail(SNF_BAD_PARAM, "null argument");
+  std::vector<cplx> a(h->NC), b(h->NC);
+  std::memcpy(a.data(), vor, h->NC * sizeof(cplx));
+  std::memcpy(b.data(), phi, h->NC * sizeof(cplx));
+  return upload_common(h, a, b, u00, w00, time, xoffb, xofft);
+}
+
+int snf_state_upload(snf_handle* h, const double* vor, const double* phi, const double* u00,
+                     const double* w00, double time, double xoffb, double xofft) {
+  if (!h || !vor || !phi || !u00 || !w00) return fail(SNF_BAD_PARAM, "null argument");
+  const int my = h->g.my, kxn = h->g.kxn, mz = h->g.mz;
+  std::vector<cplx> a(h->NC), b(h->NC);
+  // reference jik: ((k*kxn + i)*my + j) complex  ->  device (j*kxn + i)*mz + k
+  for (int k = 0; k < mz; ++k)
+    for (int i = 0; i < kxn; ++i)
+      for (int j = 0; j < my; ++j) {
+        const size_t src = ((size_t)k * kxn + i) * my + j;
+        const size_t dst = ((size_t)j * kxn + i) * mz + k;
+        a[dst] = mkc(vor[2 * src], vor[2 * src + 1]);
+        b[dst] = mkc(phi[2 * src], phi[2 * src + 1]);
+      }
+  return upload_common(h, a, b, u00, w00, time, xoffb, xofft);
+}
+
+static int download_common(snf_handle* h, std::vector<cplx>& a, std::vector<cplx>& b, double* u00,
+                           double* w00) {
+  hipSetDevice(h->dev);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const int my = h->g.my;
+  a.resize(h->NC);
+  b.resize(h->NC);
+  HIPCHK(hipMemcpy(a.data(), h->vor, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(b.data(), h->phi, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  std::vector<cplx> uc(my), wc(my);
+  HIPCHK(hipMemcpy(uc.data(), h->u00c, my * sizeof(cplx), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(wc.data(), h->w00c, my * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (u00)
+    for (int j = 0; j < my; ++j) u00[j] = uc[j].x;
+  if (w00)
+    for (int j = 0; j < my; ++j) w00[j] = wc[j].x;
+  return SNF_OK;
+}
+
+int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi, double* u00, double* w00) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  std::vector<cplx> a, b;
+  int rc = download_common(h, a, b, u00, w00);
+  if (rc) return rc;
+  if (vor) std::memcpy(vor, a.data(), h->NC * sizeof(cplx));
+  if (phi) std::memcpy(phi, b.data(), h->NC * sizeof(cplx));
+  return SNF_OK;
+}
+
+int snf_state_download(snf_handle* h, double* vor, double* phi, double* u00, double* w00,
+                       double* time, double* xoffb, double* xofft) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  std::vector<cplx> a, b;
+  int rc = download_common(h, a, b, u00, w00);
+  if (rc) return rc;
+  const int my = h->g.my, kxn = h->g.kxn, mz = h->g.mz;
+  for (int k = 0; k < mz; ++k)
+    for (int i = 0; i < kxn; ++i)
+      for (int j = 0; j < my; ++j) {
+        const size_t dst = ((size_t)k * kxn + i) * my + j;
+        const size_t src = ((size_t)j * kxn + i) * mz + k;
+        if (vor) { vor[2 * dst] = a[src].x; vor[2 * dst + 1] = a[src].y; }
+        if (phi) { phi[2 * dst] = b[src].x; phi[2 * dst + 1] = b[src].y; }
+      }
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  if (time) *time = s.time;
+  if (xoffb) *xoffb = s.xoffb;
+  if (xofft) *xofft = s.xofft;
+  return SNF_OK;
+}
+
+int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  if (!h->uploaded) return fail(SNF_NOT_READY, "snf_advance before snf_state_upload");
+  if (nstep < 0) return fail(SNF_BAD_PARAM, "nstep < 0");
+  HIPCHK(hipSetDevice(h->dev));
+  const Geom& g = h->g;
+  int rc;
+  DevScalars s0{};
+  HIPCHK(hipMemcpy(&s0, h->sc, sizeof(s0), hipMemcpyDeviceToHost));
+  const int done0 = s0.steps_done, nrec0 = s0.nrec;
+  // start of cross: kx=0 Hermitian fix, twist of the current offsets,
+  // v = lapcdy(phi) and the first D1(vor), D1(v)
+  k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur]);
+  k_set_conj<<<g.my, 128, 0, h->stream>>>(g, h->vor, h->phi);
+  if ((rc = launch_twist(h, h->cur))) return rc;
+  k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
+  HIPCHK(hipGetLastError());
+  for (int step = 0; step < nstep; ++step) {
+    k_step_begin<<<1, 1, 0, h->stream>>>(h->sc);
+    for (int rk = 1; rk <= 3; ++rk) {
+      if ((rc = launch_xz(h, rk))) return rc;
+      {
+        TimedLaunch tl(h, SNF_K_SMALL);
+        k_scalar<<<1, 256, 0, h->stream>>>(sargs(h), rk);
+        if ((rc = launch_twist(h, 1 - h->cur))) return rc;
+      }
+      {
+        TimedLaunch tl(h, SNF_K_Y);
+        if (h->p.explicit_visc == 1)
+          k_y_explicit<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h), rk);
+        else
+          k_y_implicit<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h), rk);
+      }
+      HIPCHK(hipGetLastError());
+      h->cur = 1 - h->cur;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->timing) collect_timing(h);
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  if (st) {
+    st->steps_done = s.steps_done - done0;
+    st->stop = s.istop ? SNF_STOP : (s.idump ? SNF_DUMP : SNF_RUNNING);
+    st->time = s.time;
+    st->Deltat = s.Deltat;
+    st->xoffb = s.xoffb;
+    st->xofft = s.xofft;
+    for (int q = 0; q < 4; ++q) st->umax[q] = s.umax[q];
+    st->CFLv = s.CFLv;
+    st->nhist_records = s.nrec - nrec0;
+  }
+  return SNF_OK;
+}
+
+int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n) {
+  if (!h || !n) return fail(SNF_BAD_PARAM, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  int avail = s.nrec - h->nrec_read;
+  if (avail > h->cap) {
+    h->nrec_read = s.nrec - h->cap;
+    avail = h->cap;
+  }
+  const int cnt = avail < max_records ? avail : max_records;
+  std::vector<double> ring((size_t)h->cap * 24);
+  HIPCHK(hipMemcpy(ring.data(), h->ring, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int r = 0; r < cnt; ++r) {
+    const int slot = (h->nrec_read + r) % h->cap;
+    std::memcpy(out + (size_t)r * 24, &ring[(size_t)slot * 24], 24 * sizeof(double));
+  }
+  h->nrec_read += cnt;
+  *n = cnt;
+  return SNF_OK;
+}
+
+int snf_statistics(snf_handle* h, double* stats, double* stats2, int32_t* nacum, double* total_dt) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t n = (size_t)h->g.my * SNF_NSTAT;
+  if (stats) HIPCHK(hipMemcpy(stats, h->stats, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (stats2) HIPCHK(hipMemcpy(stats2, h->stats2, n * sizeof(double), hipMemcpyDeviceToHost));
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  if (nacum) *nacum = s.nacum;
+  if (total_dt) *total_dt = s.total_dt;
+  return SNF_OK;
+}
+
+// ---------------------------------------------------------------- operators
+int snf_op_derivy(snf_handle* h, const double* u, double* du, int32_t nl, const double* shp,
+                  const double* shm, int32_t order) {
+  if (!h || !u || !du || !shp || !shm || nl <= 0 || (order != 1 && order != 2))
+    return fail(SNF_BAD_PARAM, "bad argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const int N = h->g.my;
+  const size_t n = (size_t)N * nl;
+  cplx *du_, *dd, *ys, *sp, *sm;
+  TwistTail* tw;
+  HIPCHK(hipMalloc(&du_, n * sizeof(cplx)));
+  HIPCHK(hipMalloc(&dd, n * sizeof(cplx)));
+  HIPCHK(hipMalloc(&ys, n * sizeof(cplx)));
+  HIPCHK(hipMalloc(&sp, nl * sizeof(cplx)));
+  HIPCHK(hipMalloc(&sm, nl * sizeof(cplx)));
+  HIPCHK(hipMalloc(&tw, 2 * (size_t)nl * sizeof(TwistTail)));
+  HIPCHK(hipMemcpy(du_, u, n * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sp, shp, nl * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sm, shm, nl * sizeof(cplx), hipMemcpyHostToDevice));
+  k_twist<<<dim3(nl, 2), 64, 0, h->stream>>>(N, h->b1, h->b2, h->c, sp, tw, nl, nullptr);
+  k_op_deriv<<<(nl + 127) / 128, 128, 0, h->stream>>>(N, nl, h->c, order == 1 ? h->b1 : h->b2,
+                                                     tw + (order == 1 ? 0 : nl), sp, sm, du_, dd,
+                                                     ys, order);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(du, dd, n * sizeof(cplx), hipMemcpyDeviceToHost));
+  hipFree(du_); hipFree(dd); hipFree(ys); hipFree(sp); hipFree(sm); hipFree(tw);
+  return SNF_OK;
+}
+
+int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl, const double* k2,
+                     const double* shp, const double* shm) {
+  if (!h || !u || !p || !k2 || !shp || !shm || nl <= 0) return fail(SNF_BAD_PARAM, "bad argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const int N = h->g.my;
+  const size_t n = (size_t)N * nl;
+  cplx *ud, *pd, *ys, *sp, *sm, *hc;
+  double *k2d, *hr;
+  HIPCHK(hipMalloc(&ud, n * sizeof(cplx)));
+  HIPCHK(hipMalloc(&pd, n * sizeof(cplx)));
+  HIPCHK(hipMalloc(&ys, n * sizeof(cplx)));
+  HIPCHK(hipMalloc(&sp, nl * sizeof(cplx)));
+  HIPCHK(hipMalloc(&sm, nl * sizeof(cplx)));
+  HIPCHK(hipMalloc(&k2d, nl * sizeof(double)));
+  HIPCHK(hipMalloc(&hr, 4 * n * sizeof(double)));
+  HIPCHK(hipMalloc(&hc, 3 * n * sizeof(cplx)));
+  HIPCHK(hipMemcpy(ud, u, n * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sp, shp, nl * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(sm, shm, nl * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(k2d, k2, nl * sizeof(double), hipMemcpyHostToDevice));
+  k_op_helm<<<(nl + 127) / 128, 128, 0, h->stream>>>(N, nl, h->c, k2d, sp, sm, ud, pd, ys,
+                                                     HScratch{hr, hc});
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(p, pd, n * sizeof(cplx), hipMemcpyDeviceToHost));
+  hipFree(ud); hipFree(pd); hipFree(ys); hipFree(sp); hipFree(sm); hipFree(k2d); hipFree(hr); hipFree(hc);
+  return SNF_OK;
+}
+
+int snf_op_fourxz(snf_handle* h, const double* in, double* out, int32_t nplanes, int32_t dir) {
+  if (!h || !in || !out || nplanes <= 0 || (dir != 1 && dir != -1)) return fail(SNF_BAD_PARAM, "bad argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const Geom& g = h->g;
+  const size_t ns = (size_t)nplanes * g.kxn * g.mz, nh = (size_t)nplanes * g.kxn * g.mgalz,
+               np = (size_t)nplanes * g.mgalx * g.mgalz;
+  cplx *spec, *hyb;
+  double* phys;
+  HIPCHK(hipMalloc(&spec, ns * sizeof(cplx)));
+  HIPCHK(hipMalloc(&hyb, nh * sizeof(cplx)));
+  HIPCHK(hipMalloc(&phys, np * sizeof(double)));
+  const size_t lz = 2 * (size_t)g.mgalz * sizeof(cplx), lx = 2 * (size_t)g.mgalx * sizeof(cplx);
+  if (dir == 1) {
+    HIPCHK(hipMemcpy(spec, in, ns * sizeof(cplx), hipMemcpyHostToDevice));
+    k_fourxz_fwd_z<<<dim3(g.kxn, nplanes), 128, lz, h->stream>>>(g, spec, hyb, h->pz, h->twz);
+    k_fourxz_x_c2r<<<dim3(g.mgalz, nplanes), 128, lx, h->stream>>>(g, hyb, phys, h->px, h->twx);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, phys, np * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    HIPCHK(hipMemcpy(phys, in, np * sizeof(double), hipMemcpyHostToDevice));
+    k_fourxz_x_r2c<<<dim3(g.mgalz, nplanes), 128, lx, h->stream>>>(g, phys, hyb, h->px, h->twx);
+    k_fourxz_inv_z<<<dim3(g.kxn, nplanes), 128, lz, h->stream>>>(g, hyb, spec, h->pz, h->twz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, spec, ns * sizeof(cplx), hipMemcpyDeviceToHost));
+  }
+  hipFree(spec); hipFree(hyb); hipFree(phys);
+  return SNF_OK;
+}
+
+int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const double* dvordy,
+                double* out_dvordy, double* out_hg, double* out_hv, double* rf0u, double* rf0w) {
+  if (!h || !hg_v || !hv_dv || !dvordy) return fail(SNF_BAD_PARAM, "null argument");
+  if (!h->uploaded) return fail(SNF_NOT_READY, "snf_op_hvhg before snf_state_upload");
+  HIPCHK(hipSetDevice(h->dev));
+  const Geom& g = h->g;
+  const size_t NC = h->NC;
+  HIPCHK(hipMemcpy(h->hg, hg_v, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->hv, hv_dv, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dvordy, dvordy, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  // U, W derivatives as at the top of hvhg (cross.f90:1198-1205)
+  k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur]);
+  k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
+  int rc;
+  if ((rc = launch_xz(h, 2))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (out_dvordy) HIPCHK(hipMemcpy(out_dvordy, h->dvordy, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (out_hg) HIPCHK(hipMemcpy(out_hg, h->hg, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (out_hv) HIPCHK(hipMemcpy(out_hv, h->hv, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (rf0u) HIPCHK(hipMemcpy(rf0u, h->rf0u, g.my * sizeof(double), hipMemcpyDeviceToHost));
+  if (rf0w) HIPCHK(hipMemcpy(rf0w, h->rf0w, g.my * sizeof(double), hipMemcpyDeviceToHost));
+  return SNF_OK;
+}
+
+int snf_set_timing(snf_handle* h, int32_t on) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  HIPCHK(hipSetDevice(h->dev));
+  if (on && h->ev.empty()) {
+    const int n = 8192;
+    h->ev.resize(n);
+    h->ev_cls.resize(n / 2);
+    for (int q = 0; q < n; ++q) HIPCHK(hipEventCreate(&h->ev[q]));
+  }
+  h->timing = on != 0;
+  h->ev_used = 0;
+  for (int q = 0; q < SNF_K_NCLASS; ++q) {
+    h->t_ms[q] = 0.0;
+    h->t_cnt[q] = 0;
+  }
+  return SNF_OK;
+}
+
+int snf_get_timing(snf_handle* h, double* ms, int32_t* counts, int32_t nclass) {
+  if (!h || !ms || !counts) return fail(SNF_BAD_PARAM, "null argument");
+  for (int q = 0; q < nclass && q < SNF_K_NCLASS; ++q) {
+    ms[q] = h->t_ms[q];
+    counts[q] = h->t_cnt[q];
+  }
+  return SNF_OK;
+}
+
+}  // extern "C"

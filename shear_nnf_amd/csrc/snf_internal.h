@@ -1,0 +1,64 @@
+// Internal definitions shared by the HIP translation units of libshear_rk3.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double2 cplx;
+
+#define SNF_MAXN 1032      // largest my supported (line length)
+#define SNF_MAXFFT 1024    // largest mgalx / mgalz
+#define SNF_NSTAT 23
+
+__host__ __device__ __forceinline__ cplx mkc(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return mkc(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return mkc(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+  return mkc(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {  // a * conj(b)
+  return mkc(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ cplx crmul(double r, cplx a) { return mkc(r * a.x, r * a.y); }
+__device__ __forceinline__ cplx cconj(cplx a) { return mkc(a.x, -a.y); }
+__device__ __forceinline__ double cabs2(cplx a) { return a.x * a.x + a.y * a.y; }
+
+// Grid description passed by value to every kernel.
+struct Geom {
+  int my, kxn, mz, nz, mgalx, mgalz;
+  long P;            // kxn*mz: complex coefficients per y-plane (= number of y-lines)
+  double alp, gam, Ly, dy, s, chi, Re;
+};
+
+// Compact-scheme stencils (offsets -3..3) and untwisted base factors.
+struct YCoef {
+  double rh1[7], rh2[7], pr1[7], pr2[7];
+};
+
+// Untwisted (twist = 1) L D L^H factors of the first/second-derivative LHS;
+// border rows kept as real numbers (their imaginary part is zero).
+struct BaseFactor {
+  const double *D, *Dm1, *La, *Lb, *Lc, *Ler, *Lfr, *Lgr;  // N entries each
+};
+
+// Per-kx re-factored tail of a twisted derivative factor (modChol_addshear):
+// Le rows N-6..N-4, Lf rows N-5..N-3, Lg rows N-4..N-2, Dm1 rows N-3..N-1.
+struct TwistTail {
+  cplx Le[3], Lf[3], Lg[3];
+  double Dm1[3];
+};
+
+// Device-resident run scalars (modules running / bcs).
+struct DevScalars {
+  double time, Deltat, dtr, xoffb, xofft, xwkb, xwkt;
+  double etime, dumptime, dtimag;
+  double umax[4], CFLv;
+  int istop, idump, halted, steps_done;
+  int steps_local;    // istep of the current advance call (1-based)
+  int nrec;           // UNER records written (ring position)
+  int nacum;
+  double total_dt;
+};
+
+// Derived per-plane statistics produced by the hist kernels:
+// per plane j, STAT_W doubles (see snf_xz.hip k_hist_f for the layout).
+#define STAT_W 24

@@ -1,0 +1,434 @@
+// Nonlinear stage of one RK3 substep (hvhg, cross.f90:1127-1765):
+// velocity/vorticity reconstruction, inverse x-z transforms, u x omega,
+// forward transforms and the three output combinations.
+//
+// Data movement per substep (Nz = mgalz, Mx = mgalx):
+//   A  spectral [y][kx][kz] (5 inputs)  -> hybrid [f][y][kx][z], 6 fields,
+//      z-inverse DFT per kx column                 (grid kxn x my)
+//   B  hybrid rows (all kx at one z)   -> physical x rows in LDS,
+//      u x omega, x-forward DFT -> hybrid [f][y][kx][z], 3 fields
+//                                                  (grid Nz/2 x my)
+//   C  hybrid columns -> z-forward DFT -> 3 spectral outputs
+//                                                  (grid kxn x my)
+// The kx=0 real-to-complex convention of a c2r transform (imaginary part
+// of the kx=0 coefficient ignored) and the 3/2-rule mode map of
+// fill00/pack00 (fou3D_thread.f:75-161) are reproduced exactly.
+#include "snf_internal.h"
+#include "snf_fft.h"
+
+__device__ __forceinline__ int kz_pos(int k, int nz, int mz, int Nz) {
+  return k <= nz ? k : k - mz + Nz;
+}
+__device__ __forceinline__ double kz_of(int k, int nz, int mz, double gam) {
+  return k <= nz ? gam * (double)k : -gam * (double)(mz - k);
+}
+// (a + ib) / (i g) = (b/g, -a/g)
+__device__ __forceinline__ cplx div_i(cplx a, double g) { return mkc(a.y / g, -a.x / g); }
+// (0 + i c) * a
+__device__ __forceinline__ cplx mul_i(double c, cplx a) { return mkc(-c * a.y, c * a.x); }
+
+// Spectral inputs of plane j at (i,k) with the kx=0 Hermitian fix of
+// set_conj applied to phi and omega_y (operators.f90:1382-1429).
+struct PlaneIn {
+  cplx phic, ome2c, v, dv, dome;
+};
+
+__device__ __forceinline__ PlaneIn load_in(const Geom& g, const cplx* vor, const cplx* phi,
+                                           const cplx* hg, const cplx* hv, const cplx* dvordy,
+                                           int j, int i, int k) {
+  PlaneIn r;
+  const long base = (long)j * g.P + (long)i * g.mz;
+  const long idx = base + k;
+  if (i == 0 && k > g.nz) {
+    r.phic = cconj(phi[base + (g.mz - k)]);
+    r.ome2c = cconj(vor[base + (g.mz - k)]);
+  } else {
+    r.phic = phi[idx];
+    r.ome2c = vor[idx];
+  }
+  r.v = hg[idx];
+  r.dv = hv[idx];
+  r.dome = dvordy[idx];
+  return r;
+}
+
+// u, v, w, omega_x, omega_y, omega_z at (i,k) (cross.f90:1221-1276).
+__device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i, int k,
+                                         double u00j, double w00j, double dudyj,
+                                         double dwdyj, double vbulk, cplx* out) {
+  const double kx = g.alp * (double)i;
+  const double kz = kz_of(k, g.nz, g.mz, g.gam);
+  cplx u, w, o1, o3;
+  if (i == 0 && k == 0) {
+    u = mkc(u00j, 0.0);
+    w = mkc(w00j, 0.0);
+    o3 = mkc(-dudyj, 0.0);
+    o1 = mkc(dwdyj, 0.0);
+  } else if (i == 0) {
+    o3 = div_i(mkc(-q.dome.x, -q.dome.y), kz);
+    o1 = div_i(mkc(-q.phic.x, -q.phic.y), kz);
+    w = div_i(mkc(-q.dv.x, -q.dv.y), kz);
+    u = div_i(q.ome2c, kz);
+  } else {
+    const double kk = kx * kx + kz * kz;
+    const cplx a1 = mul_i(kz, q.dome), a2 = mul_i(kx, q.phic);
+    o3 = mkc((a1.x - a2.x) / kk, (a1.y - a2.y) / kk);
+    const cplx b1 = mul_i(kx, q.dome), b2 = mul_i(kz, q.phic);
+    o1 = mkc((b1.x + b2.x) / kk, (b1.y + b2.y) / kk);
+    const cplx c1 = mul_i(kz, q.dv), c2 = mul_i(kx, q.ome2c);
+    w = mkc((c1.x + c2.x) / kk, (c1.y + c2.y) / kk);
+    const cplx d1 = mul_i(kx, q.dv), d2 = mul_i(kz, q.ome2c);
+    u = mkc((d1.x - d2.x) / kk, (d1.y - d2.y) / kk);
+  }
+  out[0] = u;
+  out[1] = (i == 0 && k == 0) ? mkc(vbulk, 0.0) : q.v;
+  out[2] = w;
+  out[3] = o1;
+  out[4] = q.ome2c;
+  out[5] = o3;
+}
+
+// ---------------------------------------------------------------------------
+// A: spectral -> hybrid (inverse z DFT); fields u,v,w,o1,o2,o3
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+       const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+       const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+       const double* __restrict__ w00, const double* __restrict__ dudy,
+       const double* __restrict__ dwdy, double vbulk, cplx* __restrict__ hyb, FFTPlan pz,
+       const cplx* __restrict__ twz, const DevScalars* sc) {
+  if (sc->halted) return;
+  extern __shared__ cplx lds[];
+  const int Nz = g.mgalz;
+  cplx* a = lds;
+  cplx* b = lds + 3 * Nz;
+  const int i = blockIdx.x, j = blockIdx.y;
+  const double u00j = u00[j], w00j = w00[j], dudyj = dudy[j], dwdyj = dwdy[j];
+  const long fstride = (long)g.my * g.kxn * Nz;
+  for (int grp = 0; grp < 2; ++grp) {
+    for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) a[t] = mkc(0.0, 0.0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
+      PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
+      cplx d[6];
+      derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, d);
+      const int pos = kz_pos(k, g.nz, g.mz, Nz);
+#pragma unroll
+      for (int f = 0; f < 3; ++f) a[f * Nz + pos] = d[grp * 3 + f];
+    }
+    __syncthreads();
+    cplx* r = fft_lds(a, b, 3, pz, twz, +1);
+    for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
+      const int f = t / Nz, z = t - f * Nz;
+      hyb[(grp * 3 + f) * fstride + ((long)j * g.kxn + i) * Nz + z] = r[t];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// B: two z-rows per block: c2r in x, u x omega, r2c in x
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long dbits(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+
+__global__ void __launch_bounds__(256)
+k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px,
+       const cplx* __restrict__ twx, int rk, unsigned long long* umax_acc,
+       double* __restrict__ phist, const DevScalars* sc, int nhist) {
+  if (sc->halted) return;
+  extern __shared__ cplx lds[];
+  const int Mx = g.mgalx, Nz = g.mgalz, kxn = g.kxn;
+  cplx* a = lds;              // 6 sequences (2 rows x 3 pairs)
+  cplx* b = lds + 6 * Mx;
+  const int zb = blockIdx.x, j = blockIdx.y, z0 = 2 * zb;
+  const long fstride = (long)g.my * kxn * Nz;
+  const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
+  for (int t = threadIdx.x; t < 6 * Mx; t += blockDim.x) a[t] = mkc(0.0, 0.0);
+  __syncthreads();
+  // build Z_pair = G_a + i G_b with Hermitian extension in kx
+  for (int t = threadIdx.x; t < 6 * kxn; t += blockDim.x) {
+    const int seq = t / kxn, i = t - seq * kxn;
+    const int r = seq / 3, p = seq - r * 3;
+    const long off = ((long)j * kxn + i) * Nz + z0 + r;
+    const cplx Ga = hin[(2 * p) * fstride + off];
+    const cplx Gb = hin[(2 * p + 1) * fstride + off];
+    cplx* s = a + seq * Mx;
+    if (i == 0) {
+      s[0] = mkc(Ga.x, Gb.x);
+    } else {
+      s[i] = mkc(Ga.x - Gb.y, Ga.y + Gb.x);
+      s[Mx - i] = mkc(Ga.x + Gb.y, Gb.x - Ga.y);
+    }
+  }
+  __syncthreads();
+  cplx* X = fft_lds(a, b, 6, px, twx, +1);
+  cplx* Y = (X == a) ? b : a;
+  const double c2 = 2.0 * g.chi;
+  const double sy = g.s * ((double)j * g.dy - 0.5 * g.Ly);
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  for (int x = threadIdx.x; x < Mx; x += blockDim.x) {
+    double h3[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const cplx z0v = X[(r * 3 + 0) * Mx + x];
+      const cplx z1v = X[(r * 3 + 1) * Mx + x];
+      const cplx z2v = X[(r * 3 + 2) * Mx + x];
+      const double u = z0v.x, v = z0v.y, w = z1v.x, o1 = z1v.y, o2 = z2v.x, o3 = z2v.y;
+      if (rk == 1) {
+        m0 = fmax(m0, fabs(u));
+        m1 = fmax(m1, fabs(u + sy));
+        m2 = fmax(m2, fabs(v));
+        m3 = fmax(m3, fabs(w));
+      }
+      if (ihist) {
+        p0 += v * (u * u);
+        p1 += v * (w * w);
+        p2 += v * v * v;
+      }
+      const double H1 = v * (o3 + c2) - w * o2;
+      const double H2 = w * o1 - u * (o3 + c2);
+      h3[r] = u * o2 - v * o1;
+      Y[r * Mx + x] = mkc(H1, H2);
+    }
+    Y[2 * Mx + x] = mkc(h3[0], h3[1]);
+  }
+  // block reductions (deterministic tree)
+  __shared__ double red[256 * 4];
+  if (rk == 1) {
+    red[threadIdx.x] = m0;
+    red[256 + threadIdx.x] = m1;
+    red[512 + threadIdx.x] = m2;
+    red[768 + threadIdx.x] = m3;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          red[q * 256 + threadIdx.x] = fmax(red[q * 256 + threadIdx.x], red[q * 256 + threadIdx.x + st]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x < 4) atomicMax(&umax_acc[threadIdx.x], dbits(red[threadIdx.x * 256]));
+    __syncthreads();
+  }
+  if (ihist) {
+    red[threadIdx.x] = p0;
+    red[256 + threadIdx.x] = p1;
+    red[512 + threadIdx.x] = p2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) red[q * 256 + threadIdx.x] += red[q * 256 + threadIdx.x + st];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x < 3) phist[((long)j * (Nz / 2) + zb) * 3 + threadIdx.x] = red[threadIdx.x * 256];
+  }
+  __syncthreads();
+  cplx* Z = fft_lds(Y, X, 3, px, twx, -1);
+  for (int t = threadIdx.x; t < 2 * kxn; t += blockDim.x) {
+    const int r = t / kxn, k = t - r * kxn;
+    const int km = (k == 0) ? 0 : Mx - k;
+    const cplx A = Z[r * Mx + k], Am = Z[r * Mx + km];
+    const cplx B = Z[2 * Mx + k], Bm = Z[2 * Mx + km];
+    const cplx H1 = mkc(0.5 * (A.x + Am.x), 0.5 * (A.y - Am.y));
+    const cplx H2 = mkc(0.5 * (A.y + Am.y), -0.5 * (A.x - Am.x));
+    const cplx H3 = (r == 0) ? mkc(0.5 * (B.x + Bm.x), 0.5 * (B.y - Bm.y))
+                             : mkc(0.5 * (B.y + Bm.y), -0.5 * (B.x - Bm.x));
+    const long off = ((long)j * kxn + k) * Nz + z0 + r;
+    hout[off] = H1;
+    hout[fstride + off] = H2;
+    hout[2 * fstride + off] = H3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C: hybrid -> spectral (forward z DFT), output combinations (iadd_mode=5,
+// cross.f90:1533-1544) and the 00-mode terms rf0u, rf0w (cross.f90:1489-1490)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_xz_C(Geom g, const cplx* __restrict__ hout, cplx* __restrict__ dvordy,
+       cplx* __restrict__ hg, cplx* __restrict__ hv, double* __restrict__ rf0u,
+       double* __restrict__ rf0w, FFTPlan pz, const cplx* __restrict__ twz,
+       const DevScalars* sc) {
+  if (sc->halted) return;
+  extern __shared__ cplx lds[];
+  const int Nz = g.mgalz;
+  cplx* a = lds;
+  cplx* b = lds + 3 * Nz;
+  const int i = blockIdx.x, j = blockIdx.y;
+  const long fstride = (long)g.my * g.kxn * Nz;
+  for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
+    const int f = t / Nz, z = t - f * Nz;
+    a[t] = hout[f * fstride + ((long)j * g.kxn + i) * Nz + z];
+  }
+  __syncthreads();
+  cplx* r = fft_lds(a, b, 3, pz, twz, -1);
+  const double norm = 1.0 / ((double)g.mgalx * (double)g.mgalz);
+  const double kx = g.alp * (double)i;
+  const double kx2 = kx * kx;
+  const long base = (long)j * g.P + (long)i * g.mz;
+  for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
+    const int pos = kz_pos(k, g.nz, g.mz, Nz);
+    const cplx h1 = crmul(norm, r[pos]);
+    const cplx h2 = crmul(norm, r[Nz + pos]);
+    const cplx h3 = crmul(norm, r[2 * Nz + pos]);
+    const double kz = kz_of(k, g.nz, g.mz, g.gam);
+    const long idx = base + k;
+    const cplx v = hg[idx];
+    dvordy[idx] = cadd(mul_i(kx, h1), mul_i(kz, h3));
+    const cplx d = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);
+    hg[idx] = cadd(mul_i(-kx, h3), mul_i(kz, d));
+    const double kk = kx2 + kz * kz;
+    hv[idx] = mkc(-h2.x * kk, -h2.y * kk);
+    if (i == 0 && k == 0) {
+      rf0u[j] = h1.x;
+      rf0w[j] = h3.x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// hist 'f' (cross.f90:1798-1966): per-plane Fourier-space statistics.
+// Output per plane j (STAT_W doubles):
+//  0 up 1 vp 2 wp 3 uvr 4 uwr 5 vwr 6 o1p 7 o2p 8 o3p 9 o12p 10 eps
+//  11 um 12 vm 13 wm 14 o1m 15 o2m 16 o3m
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_hist_f(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+         const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+         const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+         const double* __restrict__ w00, const double* __restrict__ dudy,
+         const double* __restrict__ dwdy, double vbulk, double* __restrict__ out,
+         const DevScalars* sc, int nhist) {
+  if (sc->halted) return;
+  if ((sc->steps_local - 1) % nhist != 0) return;
+  const int j = blockIdx.x;
+  double acc[11];
+#pragma unroll
+  for (int q = 0; q < 11; ++q) acc[q] = 0.0;
+  for (long L = threadIdx.x; L < g.P; L += blockDim.x) {
+    const int i = (int)(L / g.mz), k = (int)(L - (long)i * g.mz);
+    PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
+    cplx d[6];
+    derived6(g, q, i, k, u00[j], w00[j], dudy[j], dwdy[j], vbulk, d);
+    const double wt = (i == 0) ? 1.0 : 2.0;
+    const cplx u = d[0], v = d[1], w = d[2], o1 = d[3], o2 = d[4];
+    cplx o3 = d[5];
+    acc[0] += wt * cabs2(u);
+    acc[1] += wt * cabs2(v);
+    acc[2] += wt * cabs2(w);
+    acc[3] += wt * cmulc(u, v).x;
+    acc[4] += wt * cmulc(u, w).x;
+    acc[5] += wt * cmulc(v, w).x;
+    acc[6] += wt * cabs2(o1);
+    acc[7] += wt * cabs2(o2);
+    acc[8] += wt * cabs2(o3);
+    acc[9] += wt * cmulc(o1, o2).x;
+    // dissipation, with the mean shear removed from o3(0,0)
+    const double kx = g.alp * (double)i;
+    const double kz = kz_of(k, g.nz, g.mz, g.gam);
+    if (i == 0 && k == 0) o3.x -= g.s;
+    const double uu = cabs2(u) + cabs2(v) + cabs2(w);
+    const cplx c1 = cadd(o1, mul_i(kz, v));
+    if (i == 0) {
+      acc[10] += kz * kz * uu + cabs2(q.dv) + cabs2(o3) + cabs2(c1);
+    } else {
+      const cplx c3 = csub(o3, mul_i(kx, v));
+      acc[10] += 2.0 * ((kx * kx + kz * kz) * uu + cabs2(q.dv) + cabs2(c1) + cabs2(c3));
+    }
+    if (L == 0) {
+      out[(long)j * STAT_W + 11] = u.x;
+      out[(long)j * STAT_W + 12] = v.x;
+      out[(long)j * STAT_W + 13] = w.x;
+      out[(long)j * STAT_W + 14] = o1.x;
+      out[(long)j * STAT_W + 15] = o2.x;
+      out[(long)j * STAT_W + 16] = d[5].x;
+    }
+  }
+  __shared__ double red[11][256];
+#pragma unroll
+  for (int q = 0; q < 11; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+#pragma unroll
+      for (int q = 0; q < 11; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 11) out[(long)j * STAT_W + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// standalone fourxz for the operator ABI: planes [p][kx][kz] <-> [p][x][z]
+// ---------------------------------------------------------------------------
+__global__ void k_fourxz_fwd_z(Geom g, const cplx* in, cplx* hyb, FFTPlan pz, const cplx* twz) {
+  // spectral -> hybrid [p][kx][z] (inverse z DFT)
+  extern __shared__ cplx lds[];
+  const int Nz = g.mgalz;
+  cplx* a = lds;
+  cplx* b = lds + Nz;
+  const int i = blockIdx.x, p = blockIdx.y;
+  for (int t = threadIdx.x; t < Nz; t += blockDim.x) a[t] = mkc(0.0, 0.0);
+  __syncthreads();
+  for (int k = threadIdx.x; k < g.mz; k += blockDim.x)
+    a[kz_pos(k, g.nz, g.mz, Nz)] = in[((long)p * g.kxn + i) * g.mz + k];
+  __syncthreads();
+  cplx* r = fft_lds(a, b, 1, pz, twz, +1);
+  for (int z = threadIdx.x; z < Nz; z += blockDim.x) hyb[((long)p * g.kxn + i) * Nz + z] = r[z];
+}
+
+__global__ void k_fourxz_x_c2r(Geom g, const cplx* hyb, double* out, FFTPlan px, const cplx* twx) {
+  // one z row per block: out[p][x][z]
+  extern __shared__ cplx lds[];
+  const int Mx = g.mgalx, Nz = g.mgalz;
+  cplx* a = lds;
+  cplx* b = lds + Mx;
+  const int z = blockIdx.x, p = blockIdx.y;
+  for (int t = threadIdx.x; t < Mx; t += blockDim.x) a[t] = mkc(0.0, 0.0);
+  __syncthreads();
+  for (int i = threadIdx.x; i < g.kxn; i += blockDim.x) {
+    const cplx G = hyb[((long)p * g.kxn + i) * Nz + z];
+    if (i == 0) {
+      a[0] = mkc(G.x, 0.0);
+    } else {
+      a[i] = G;
+      a[Mx - i] = cconj(G);
+    }
+  }
+  __syncthreads();
+  cplx* r = fft_lds(a, b, 1, px, twx, +1);
+  for (int x = threadIdx.x; x < Mx; x += blockDim.x) out[((long)p * Mx + x) * Nz + z] = r[x].x;
+}
+
+__global__ void k_fourxz_x_r2c(Geom g, const double* in, cplx* hyb, FFTPlan px, const cplx* twx) {
+  extern __shared__ cplx lds[];
+  const int Mx = g.mgalx, Nz = g.mgalz;
+  cplx* a = lds;
+  cplx* b = lds + Mx;
+  const int z = blockIdx.x, p = blockIdx.y;
+  for (int x = threadIdx.x; x < Mx; x += blockDim.x) a[x] = mkc(in[((long)p * Mx + x) * Nz + z], 0.0);
+  __syncthreads();
+  cplx* r = fft_lds(a, b, 1, px, twx, -1);
+  for (int i = threadIdx.x; i < g.kxn; i += blockDim.x) hyb[((long)p * g.kxn + i) * Nz + z] = r[i];
+}
+
+__global__ void k_fourxz_inv_z(Geom g, const cplx* hyb, cplx* out, FFTPlan pz, const cplx* twz) {
+  extern __shared__ cplx lds[];
+  const int Nz = g.mgalz;
+  cplx* a = lds;
+  cplx* b = lds + Nz;
+  const int i = blockIdx.x, p = blockIdx.y;
+  for (int z = threadIdx.x; z < Nz; z += blockDim.x) a[z] = hyb[((long)p * g.kxn + i) * Nz + z];
+  __syncthreads();
+  cplx* r = fft_lds(a, b, 1, pz, twz, -1);
+  const double norm = 1.0 / ((double)g.mgalx * (double)g.mgalz);
+  for (int k = threadIdx.x; k < g.mz; k += blockDim.x)
+    out[((long)p * g.kxn + i) * g.mz + k] = crmul(norm, r[kz_pos(k, g.nz, g.mz, Nz)]);
+}

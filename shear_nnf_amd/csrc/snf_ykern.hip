@@ -1,0 +1,712 @@
+// y-line stages of one RK3 substep (cross.f90:428-1000), the per-kx
+// twisted border factors, and the scalar/CFL kernel.
+//
+// Thread-per-line kernels: line L = i*mz + k (kx index i, kz index k) of the
+// [y][kx][kz] fields; two extra "virtual" lines L = P and P+1 carry the
+// real x-z mean profiles U(y) = u00 and W(y) = w00, which the reference
+// advances with the same compact operators without a twist (m=1 calls of
+// derivyc / visc3d / lapcdy).
+#include "snf_internal.h"
+#include "snf_lines.h"
+
+// SMR RK3 constants (cross.f90:18-24)
+__constant__ double c_gama[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+__constant__ double c_alpha[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
+__constant__ double c_ibeta[3] = {160.0 / 37.0, 24.0 / 5.0, 6.0};
+__constant__ double c_zeta[3] = {-17.0 / 60.0, -5.0 / 12.0, 0.0};
+__constant__ double c_ca[3] = {8.0 / 15.0, 2.0 / 3.0, 1.0};
+
+__device__ __forceinline__ double kz_val(int k, int nz, int mz, double gam) {
+  return k <= nz ? gam * (double)k : -gam * (double)(mz - k);
+}
+
+// exp(i kx y_j m) of moving_frame (operators.f90:86-145)
+__device__ __forceinline__ cplx mframe(double kx, double yj, double m) {
+  double sn, cs;
+  sincos((kx * yj) * m, &sn, &cs);
+  return mkc(cs, sn);
+}
+
+// ---------------------------------------------------------------------------
+// per-kx twisted border tail of a derivative factor (modChol_addshear with
+// the border pre-multiplication of derivyc); one wave per (kx, operator)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64)
+k_twist(int N, BaseFactor b1, BaseFactor b2, YCoef c, const cplx* __restrict__ shp_arr,
+        TwistTail* __restrict__ out, int nk, const DevScalars* sc) {
+  if (sc && sc->halted) return;
+  const int kx = blockIdx.x, op = blockIdx.y, lane = threadIdx.x;
+  const BaseFactor b = op == 0 ? b1 : b2;
+  const double* a = op == 0 ? c.pr1 : c.pr2;
+  const double a0 = a[3], a1 = a[4], a2 = a[5], a3 = a[6];
+  const cplx t = shp_arr[kx];
+  __shared__ cplx sLe[3], sLf[3], sLg[3];
+  auto Ler = [&](int k) { return crmul(b.Ler[k], t); };
+  auto Lfr = [&](int k) { return crmul(b.Lfr[k], t); };
+  auto Lgr = [&](int k) { return crmul(b.Lgr[k], t); };
+  auto rec = [&](cplx x3, cplx x2, cplx x1, int i, double av) {
+    cplx r;
+    r.x = b.Dm1[i] * (av - x3.x * b.D[i - 3] * b.Lc[i] - x2.x * b.D[i - 2] * b.Lb[i] - x1.x * b.D[i - 1] * b.La[i]);
+    r.y = b.Dm1[i] * (0.0 - x3.y * b.D[i - 3] * b.Lc[i] - x2.y * b.D[i - 2] * b.Lb[i] - x1.y * b.D[i - 1] * b.La[i]);
+    return r;
+  };
+  if (lane == 0) {
+    const cplx e0 = rec(Ler(N - 9), Ler(N - 8), Ler(N - 7), N - 6, a3);
+    const cplx e1 = rec(Ler(N - 8), Ler(N - 7), e0, N - 5, a2);
+    const cplx e2 = rec(Ler(N - 7), e0, e1, N - 4, a1);
+    sLe[0] = e0; sLe[1] = e1; sLe[2] = e2;
+    const cplx f0 = rec(Lfr(N - 8), Lfr(N - 7), Lfr(N - 6), N - 5, a3);
+    const cplx f1 = rec(Lfr(N - 7), Lfr(N - 6), f0, N - 4, a2);
+    sLf[0] = f0; sLf[1] = f1;
+    sLg[0] = rec(Lgr(N - 7), Lgr(N - 6), Lgr(N - 5), N - 4, a3);
+  }
+  __syncthreads();
+  auto Le = [&](int k) { return k < N - 6 ? Ler(k) : sel3(sLe, k - (N - 6)); };
+  auto Lf = [&](int k) { return k < N - 5 ? Lfr(k) : sel3(sLf, k - (N - 5)); };
+  auto Lg = [&](int k) { return k < N - 4 ? Lgr(k) : sel3(sLg, k - (N - 4)); };
+  // D(N-3) = a0 - sum_{k<=N-4} |Le|^2 D
+  double s_ee = 0.0;
+  cplx s_fe = mkc(0, 0), s_ge = mkc(0, 0);
+  for (int k = lane; k <= N - 4; k += 64) {
+    const cplx le = Le(k), lf = Lf(k), lg = Lg(k);
+    s_ee += cabs2(le) * b.D[k];
+    s_fe = cadd(s_fe, crmul(b.D[k], cmulc(lf, le)));
+    s_ge = cadd(s_ge, crmul(b.D[k], cmulc(lg, le)));
+  }
+  s_ee = wave_sum(s_ee);
+  s_fe.x = wave_sum(s_fe.x); s_fe.y = wave_sum(s_fe.y);
+  s_ge.x = wave_sum(s_ge.x); s_ge.y = wave_sum(s_ge.y);
+  const double DN3 = a0 - s_ee;
+  const double dN3 = 1.0 / DN3;
+  const cplx lfN3 = crmul(dN3, csub(mkc(a1, 0.0), s_fe));
+  const cplx lgN3 = crmul(dN3, csub(mkc(a2, 0.0), s_ge));
+  if (lane == 0) { sLf[2] = lfN3; sLg[1] = lgN3; }
+  __syncthreads();
+  double s_ff = 0.0;
+  cplx s_gf = mkc(0, 0);
+  for (int k = lane; k <= N - 3; k += 64) {
+    const double Dk = (k == N - 3) ? DN3 : b.D[k];
+    const cplx lf = Lf(k), lg = Lg(k);
+    s_ff += cabs2(lf) * Dk;
+    s_gf = cadd(s_gf, crmul(Dk, cmulc(lg, lf)));
+  }
+  s_ff = wave_sum(s_ff);
+  s_gf.x = wave_sum(s_gf.x); s_gf.y = wave_sum(s_gf.y);
+  const double DN2 = a0 - s_ff;
+  const double dN2 = 1.0 / DN2;
+  const cplx lgN2 = crmul(dN2, csub(mkc(a1, 0.0), s_gf));
+  if (lane == 0) sLg[2] = lgN2;
+  __syncthreads();
+  double s_gg = 0.0;
+  for (int k = lane; k <= N - 2; k += 64) {
+    const double Dk = (k == N - 3) ? DN3 : ((k == N - 2) ? DN2 : b.D[k]);
+    s_gg += cabs2(Lg(k)) * Dk;
+  }
+  s_gg = wave_sum(s_gg);
+  const double DN1 = a0 - s_gg;
+  if (lane == 0) {
+    TwistTail tt;
+    for (int q = 0; q < 3; ++q) { tt.Le[q] = sLe[q]; tt.Lf[q] = sLf[q]; tt.Lg[q] = sLg[q]; }
+    tt.Dm1[0] = dN3;
+    tt.Dm1[1] = dN2;
+    tt.Dm1[2] = 1.0 / DN1;
+    out[op * nk + kx] = tt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// arguments of the y-line kernels
+// ---------------------------------------------------------------------------
+struct YArgs {
+  Geom g;
+  YCoef c;
+  BaseFactor b1, b2;
+  TwistTail id1, id2;                       // untwisted tails (00 lines)
+  const TwistTail* tw_old;                  // [2][kxn] D1, D2 at the current twist
+  const TwistTail* tw_new;                  // [2][kxn] after the shift update
+  const cplx *shp_old, *shm_old, *shp_new, *shm_new;
+  cplx *vor, *phi, *hg, *hv, *dvordy, *vorwk, *phiwk;
+  cplx *s1, *s2;                            // forward-sweep scratch, stride PS
+  HScratch hs;                              // Helmholtz factor scratch, stride PS
+  cplx *u00c, *w00c, *u00wk, *w00wk;        // 00 lines (real data, imag 0), stride 1
+  const double *rf0u, *rf0w;
+  double *dudy, *dwdy;
+  DevScalars* sc;
+  long PS;
+};
+
+struct AddSubSink {   // out[j] = base[j] - x  (hv = hv - d(dvordy)/dy)
+  cplx* out;
+  long P;
+  __device__ void operator()(int j, const cplx* x) {
+    const long o = (long)j * P;
+    out[o] = csub(out[o], x[0]);
+  }
+};
+
+struct RealOutSink {  // dd[j] = Re x
+  double* dd;
+  __device__ void operator()(int j, const cplx* x) { dd[j] = x[0].x; }
+};
+
+// lap matrix rh2 - k2 pr2 (lapcdy, diffy_shear_hepta.f90:475)
+__device__ __forceinline__ void lapmat(const YCoef& c, double k2, double* a) {
+#pragma unroll
+  for (int m = 0; m < 7; ++m) a[m] = c.rh2[m] - k2 * c.pr2[m];
+}
+
+__device__ __forceinline__ void line_geom(const Geom& g, long L, int& i, int& k, double& kx,
+                                          double& k2) {
+  i = (int)(L / g.mz);
+  k = (int)(L - (long)i * g.mz);
+  kx = g.alp * (double)i;
+  const double kz = kz_val(k, g.nz, g.mz, g.gam);
+  k2 = kx * kx + kz * kz;
+}
+
+// D1 of vor and v with the given twist -> dvordy, hv (the derivyc calls at
+// the top of each substep, cross.f90:434-445)
+__device__ __forceinline__ void d1_pair(const YArgs& a, long L, int i, const TwistTail* tw,
+                                        const cplx* shp, const cplx* shm) {
+  const long P = a.g.P;
+  DOp o{a.b1, tw[i], shp[i], a.g.my};
+  const cplx* in[2] = {a.vor + L, a.hg + L};
+  cplx* ys[2] = {a.s1 + L, a.s2 + L};
+  deriv_forward<2, true>(o, a.c.rh1, in, P, ys, a.PS, shm[i]);
+  cplx* outp[2] = {a.dvordy + L, a.hv + L};
+  StoreSink<2> sk{outp, P};
+  deriv_backward<2>(o, ys, a.PS, sk);
+}
+
+__device__ __forceinline__ void helm_line(const YArgs& a, long L, double k2, cplx shp, cplx shm,
+                                          const cplx* in, long pin, cplx* out, long pout) {
+  double lm[7];
+  lapmat(a.c, k2, lm);
+  HScratch hs{a.hs.hr + L, a.hs.hc + L};
+  HTail tail;
+  const cplx* ins[1] = {in};
+  cplx* ys[1] = {a.s1 + L};
+  helm_forward<1>(lm, a.c.pr2, a.g.my, shp, shm, ins, pin, ys, a.PS, hs, tail);
+  cplx* outs[1] = {out};
+  StoreSink<1> sk{outs, pout};
+  helm_backward<1>(a.g.my, ys, a.PS, hs, tail, sk);
+}
+
+// D1 of a 00 line (real periodic) -> dd
+__device__ __forceinline__ void d1_real(const YArgs& a, long L, const cplx* u, double* dd) {
+  DOp o{a.b1, a.id1, mkc(1.0, 0.0), a.g.my};
+  const cplx* in[1] = {u};
+  cplx* ys[1] = {a.s1 + L};
+  deriv_forward<1, true>(o, a.c.rh1, in, 1, ys, a.PS, mkc(1.0, 0.0));
+  RealOutSink sk{dd};
+  deriv_backward<1>(o, ys, a.PS, sk);
+}
+
+// ---------------------------------------------------------------------------
+// first step of an advance call: v = lapcdy(phi), then D1(vor), D1(v)
+// (cross.f90:408-426 and 434-445); 00 lines: dU/dy, dW/dy (cross.f90:1198-1205)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(128)
+k_y_first(YArgs a) {
+  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long P = a.g.P;
+  if (L >= P + 2) return;
+  if (a.sc->halted) return;
+  if (L < P) {
+    int i, k;
+    double kx, k2;
+    line_geom(a.g, L, i, k, kx, k2);
+    if (L != 0) helm_line(a, L, k2, a.shp_old[i], a.shm_old[i], a.phi + L, P, a.hg + L, P);
+    d1_pair(a, L, i, a.tw_old, a.shp_old, a.shm_old);
+  } else if (L == P) {
+    d1_real(a, L, a.u00c, a.dudy);
+  } else {
+    d1_real(a, L, a.w00c, a.dwdy);
+  }
+}
+
+// dU/dy and dW/dy only (top of hvhg, cross.f90:1198-1205)
+__global__ void k_d1_00(YArgs a) {
+  if (threadIdx.x == 0) d1_real(a, a.g.P, a.u00c, a.dudy);
+  if (threadIdx.x == 1) d1_real(a, a.g.P + 1, a.w00c, a.dwdy);
+}
+
+// ---------------------------------------------------------------------------
+// explicit substep (cross.f90:565-796), fused per line
+// ---------------------------------------------------------------------------
+struct ExplicitRKSink {
+  cplx *vor, *phi, *hg, *hv, *vorwk, *phiwk;
+  long P;
+  double fac, k2, kx, dt, gam, zet, f1, f2, dy, Ly;
+  int rk, is00;
+  __device__ void operator()(int j, const cplx* x) {
+    const long o = (long)j * P;
+    cplx hgj = hg[o], hvj = hv[o];
+    const cplx vj = vor[o], pj = phi[o];
+    if (!is00) {
+      hgj = cadd(hgj, crmul(fac, cadd(crmul(-k2, vj), x[0])));
+      hvj = cadd(hvj, crmul(fac, cadd(crmul(-k2, pj), x[1])));
+    }
+    cplx vwk, pwk;
+    if (rk == 1) {
+      vwk = crmul(gam, hgj);
+      pwk = crmul(gam, hvj);
+    } else {
+      vwk = cadd(vorwk[o], crmul(gam, hgj));
+      pwk = cadd(phiwk[o], crmul(gam, hvj));
+    }
+    cplx vn = cadd(vj, crmul(dt, vwk));
+    cplx pn = cadd(pj, crmul(dt, pwk));
+    const double yj = (double)j * dy - 0.5 * Ly;
+    if (rk != 3) {
+      const cplx m1 = mframe(kx, yj, f1);
+      vwk = cmul(crmul(zet, hgj), m1);
+      pwk = cmul(crmul(zet, hvj), m1);
+    }
+    const cplx m2 = mframe(kx, yj, f2);
+    vor[o] = cmul(vn, m2);
+    phi[o] = cmul(pn, m2);
+    vorwk[o] = vwk;
+    phiwk[o] = pwk;
+  }
+};
+
+struct Explicit00Sink {
+  cplx *u, *wk;
+  const double* rf;
+  double fac, dt, gam, zet;
+  int rk;
+  __device__ void operator()(int j, const cplx* x) {
+    const double rfj = rf[j] + fac * (-0.0 * u[j].x + x[0].x);
+    double wkj = (rk == 1) ? gam * rfj : wk[j].x + gam * rfj;
+    const double un = u[j].x + dt * wkj;
+    if (rk != 3) wkj = rfj * zet;
+    u[j] = mkc(un, 0.0);
+    wk[j] = mkc(wkj, 0.0);
+  }
+};
+
+__device__ __forceinline__ void substep_factors(const DevScalars* sc, double s, int rk,
+                                                double& dt, double& f1, double& f2) {
+  dt = sc->Deltat;
+  const int ir = rk - 1;
+  f1 = (rk == 1) ? -(c_gama[0] * dt * s) : -((c_gama[1] + c_zeta[0]) * dt * s);
+  const double r1 = c_ca[ir] * dt;
+  f2 = (rk == 1) ? -(s * r1) : -(s * (c_ca[ir] - c_ca[ir - 1]) * dt);
+}
+
+__global__ void __launch_bounds__(128)
+k_y_explicit(YArgs a, int rk) {
+  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long P = a.g.P;
+  if (L >= P + 2) return;
+  if (a.sc->halted) return;
+  const int ir = rk - 1;
+  double dt, f1, f2;
+  substep_factors(a.sc, a.g.s, rk, dt, f1, f2);
+  const double fac = 1.0 / a.g.Re;
+  if (L < P) {
+    int i, k;
+    double kx, k2;
+    line_geom(a.g, L, i, k, kx, k2);
+    const TwistTail* tw1o = a.tw_old;
+    const TwistTail* tw2o = a.tw_old + a.g.kxn;
+    // (1) dvordy <- d/dy dvordy ; hv <- hv - dvordy        (cross.f90:565-607)
+    {
+      DOp o{a.b1, tw1o[i], a.shp_old[i], a.g.my};
+      const cplx* in[1] = {a.dvordy + L};
+      cplx* ys[1] = {a.s1 + L};
+      deriv_forward<1, true>(o, a.c.rh1, in, P, ys, a.PS, a.shm_old[i]);
+      AddSubSink sk{a.hv + L, P};
+      deriv_backward<1>(o, ys, a.PS, sk);
+    }
+    // (2)-(6) viscous terms, RK update, phase maps          (cross.f90:633-781)
+    {
+      DOp o{a.b2, tw2o[i], a.shp_old[i], a.g.my};
+      const cplx* in[2] = {a.vor + L, a.phi + L};
+      cplx* ys[2] = {a.s1 + L, a.s2 + L};
+      deriv_forward<2, false>(o, a.c.rh2, in, P, ys, a.PS, a.shm_old[i]);
+      ExplicitRKSink sk{a.vor + L, a.phi + L, a.hg + L, a.hv + L, a.vorwk + L, a.phiwk + L, P,
+                        fac, k2, kx, dt, c_gama[ir], c_zeta[ir], f1, f2, a.g.dy, a.g.Ly, rk,
+                        L == 0};
+      deriv_backward<2>(o, ys, a.PS, sk);
+    }
+    // (7) v = lapcdy(phi) with the updated twist            (cross.f90:783-795)
+    if (L != 0) helm_line(a, L, k2, a.shp_new[i], a.shm_new[i], a.phi + L, P, a.hg + L, P);
+    // next substep's D1(vor), D1(v)                          (cross.f90:434-445)
+    d1_pair(a, L, i, a.tw_new, a.shp_new, a.shm_new);
+  } else {
+    cplx* u = (L == P) ? a.u00c : a.w00c;
+    cplx* wk = (L == P) ? a.u00wk : a.w00wk;
+    const double* rf = (L == P) ? a.rf0u : a.rf0w;
+    double* dd = (L == P) ? a.dudy : a.dwdy;
+    DOp o{a.b2, a.id2, mkc(1.0, 0.0), a.g.my};
+    const cplx* in[1] = {u};
+    cplx* ys[1] = {a.s1 + L};
+    deriv_forward<1, false>(o, a.c.rh2, in, 1, ys, a.PS, mkc(1.0, 0.0));
+    Explicit00Sink sk{u, wk, rf, fac, dt, c_gama[ir], c_zeta[ir], rk};
+    deriv_backward<1>(o, ys, a.PS, sk);
+    d1_real(a, L, u, dd);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// implicit substep (cross.f90:797-990)
+// ---------------------------------------------------------------------------
+struct ImplicitRKSink {
+  cplx *vor, *phi, *hg, *hv, *vorwk, *phiwk;
+  long P;
+  double fac, facb, rdt, k2, kx, gam, f2, dy, Ly;
+  int rk, is00;
+  __device__ void operator()(int j, const cplx* x) {
+    const long o = (long)j * P;
+    const cplx hgj = hg[o], hvj = hv[o], vj = vor[o], pj = phi[o];
+    cplx vwk, pwk;
+    if (rk == 1) {
+      vwk = crmul(gam, hgj);
+      pwk = crmul(gam, hvj);
+    } else {
+      vwk = cadd(vorwk[o], crmul(gam, hgj));
+      pwk = cadd(phiwk[o], crmul(gam, hvj));
+    }
+    if (!is00) {
+      vwk = cadd(vwk, crmul(fac, cadd(crmul(-k2, vj), x[0])));
+      pwk = cadd(pwk, crmul(fac, cadd(crmul(-k2, pj), x[1])));
+    }
+    vwk = cadd(vwk, crmul(rdt, vj));
+    pwk = cadd(pwk, crmul(rdt, pj));
+    vwk = crmul(facb, vwk);
+    pwk = crmul(facb, pwk);
+    const double yj = (double)j * dy - 0.5 * Ly;
+    const cplx m2 = mframe(kx, yj, f2);
+    vorwk[o] = cmul(vwk, m2);
+    phiwk[o] = cmul(pwk, m2);
+  }
+};
+
+struct Implicit00Sink {
+  cplx *u, *wk;
+  const double* rf;
+  double fac, facb, rdt, gam;
+  int rk;
+  __device__ void operator()(int j, const cplx* x) {
+    double wkj = (rk == 1) ? gam * rf[j] : wk[j].x + gam * rf[j];
+    wkj = wkj + fac * (-0.0 * u[j].x + x[0].x);
+    wkj = facb * (wkj + rdt * u[j].x);
+    wk[j] = mkc(wkj, 0.0);
+  }
+};
+
+__global__ void __launch_bounds__(128)
+k_y_implicit(YArgs a, int rk) {
+  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long P = a.g.P;
+  if (L >= P + 2) return;
+  if (a.sc->halted) return;
+  const int ir = rk - 1;
+  double dt, f1, f2;
+  substep_factors(a.sc, a.g.s, rk, dt, f1, f2);
+  const double fac = c_alpha[ir] * (1.0 / a.g.Re);
+  const double facb = -a.g.Re * c_ibeta[ir];
+  const double rdt = 1.0 / dt;
+  const double dtr1 = a.sc->dtr * c_ibeta[ir];
+  if (L < P) {
+    int i, k;
+    double kx, k2;
+    line_geom(a.g, L, i, k, kx, k2);
+    const TwistTail* tw1o = a.tw_old;
+    const TwistTail* tw2o = a.tw_old + a.g.kxn;
+    {
+      DOp o{a.b1, tw1o[i], a.shp_old[i], a.g.my};
+      const cplx* in[1] = {a.dvordy + L};
+      cplx* ys[1] = {a.s1 + L};
+      deriv_forward<1, true>(o, a.c.rh1, in, P, ys, a.PS, a.shm_old[i]);
+      AddSubSink sk{a.hv + L, P};
+      deriv_backward<1>(o, ys, a.PS, sk);
+    }
+    {
+      DOp o{a.b2, tw2o[i], a.shp_old[i], a.g.my};
+      const cplx* in[2] = {a.vor + L, a.phi + L};
+      cplx* ys[2] = {a.s1 + L, a.s2 + L};
+      deriv_forward<2, false>(o, a.c.rh2, in, P, ys, a.PS, a.shm_old[i]);
+      ImplicitRKSink sk{a.vor + L, a.phi + L, a.hg + L, a.hv + L, a.vorwk + L, a.phiwk + L,
+                        P, fac, facb, rdt, k2, kx, c_gama[ir], f2, a.g.dy, a.g.Ly, rk, L == 0};
+      deriv_backward<2>(o, ys, a.PS, sk);
+    }
+    // (8) Helmholtz solves for vor and phi with k2 + dtr1      (cross.f90:898-928)
+    if (L != 0) {
+      double lm[7];
+      lapmat(a.c, dtr1 + k2, lm);
+      HScratch hs{a.hs.hr + L, a.hs.hc + L};
+      HTail tail;
+      const cplx* in[2] = {a.vorwk + L, a.phiwk + L};
+      cplx* ys[2] = {a.s1 + L, a.s2 + L};
+      helm_forward<2>(lm, a.c.pr2, a.g.my, a.shp_new[i], a.shm_new[i], in, P, ys, a.PS, hs, tail);
+      cplx* outs[2] = {a.vor + L, a.phi + L};
+      StoreSink<2> sk{outs, P};
+      helm_backward<2>(a.g.my, ys, a.PS, hs, tail, sk);
+    }
+    // (9) zeta * [hg, hv] -> [vorwk, phiwk] with the phase map (cross.f90:931-944)
+    if (rk != 3) {
+      const double zet = c_zeta[ir];
+      for (int j = 0; j < a.g.my; ++j) {
+        const long o = (long)j * P + L;
+        const double yj = (double)j * a.g.dy - 0.5 * a.g.Ly;
+        const cplx m1 = mframe(kx, yj, f1);
+        a.vorwk[o] = cmul(crmul(zet, a.hg[o]), m1);
+        a.phiwk[o] = cmul(crmul(zet, a.hv[o]), m1);
+      }
+    }
+    if (L != 0) helm_line(a, L, k2, a.shp_new[i], a.shm_new[i], a.phi + L, P, a.hg + L, P);
+    d1_pair(a, L, i, a.tw_new, a.shp_new, a.shm_new);
+  } else {
+    cplx* u = (L == P) ? a.u00c : a.w00c;
+    cplx* wk = (L == P) ? a.u00wk : a.w00wk;
+    const double* rf = (L == P) ? a.rf0u : a.rf0w;
+    double* dd = (L == P) ? a.dudy : a.dwdy;
+    {
+      DOp o{a.b2, a.id2, mkc(1.0, 0.0), a.g.my};
+      const cplx* in[1] = {u};
+      cplx* ys[1] = {a.s1 + L};
+      deriv_forward<1, false>(o, a.c.rh2, in, 1, ys, a.PS, mkc(1.0, 0.0));
+      Implicit00Sink sk{u, wk, rf, fac, facb, rdt, c_gama[ir], rk};
+      deriv_backward<1>(o, ys, a.PS, sk);
+    }
+    helm_line(a, L, dtr1, mkc(1.0, 0.0), mkc(1.0, 0.0), wk, 1, u, 1);
+    if (rk != 3)
+      for (int j = 0; j < a.g.my; ++j) wk[j] = mkc(c_zeta[ir] * rf[j], 0.0);
+    d1_real(a, L, u, dd);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// scalar kernel: statistics accumulation, CFL time step, UNER record,
+// shear-offset update (cross.f90:1598-1712, 2067-2119, 726-746)
+// ---------------------------------------------------------------------------
+struct ScalarArgs {
+  Geom g;
+  DevScalars* sc;
+  int nhist, readflag, explicit_visc;
+  double cflx, cfly, cflz, cflvv, CFL, uprim;
+  unsigned long long* umax_acc;
+  const double* stat_plane;   // [my][STAT_W]
+  const double* phist;        // [my][Nz/2][3]
+  double *stats, *stats2;     // [my][NSTAT]
+  double* ring;               // [cap][24]
+  int cap;
+  cplx *shp_new, *shm_new;
+};
+
+__global__ void __launch_bounds__(256)
+k_scalar(ScalarArgs a, int rk) {
+  DevScalars* sc = a.sc;
+  if (sc->halted) return;
+  const Geom& g = a.g;
+  const int tid = threadIdx.x;
+  const int ihist = (rk == 1) && ((sc->steps_local - 1) % a.nhist == 0);
+  __shared__ double sh_dt;
+  const double dt_old = sc->Deltat;
+  const int nzb = g.mgalz / 2;
+  if (ihist) {
+    for (int j = tid; j < g.my; j += blockDim.x) {
+      const double* sp = a.stat_plane + (long)j * STAT_W;
+      double ph[3] = {0.0, 0.0, 0.0};
+      for (int zb = 0; zb < nzb; ++zb)
+        for (int q = 0; q < 3; ++q) ph[q] += a.phist[((long)j * nzb + zb) * 3 + q];
+      const double o3s = sp[16] - g.s;
+      const double pk = -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);
+      const double v[SNF_NSTAT] = {sp[11], sp[12], sp[13], sp[0], sp[1], sp[2], sp[3], sp[4],
+                                   sp[5], sp[14], sp[15], sp[16], sp[6], sp[7], sp[8], sp[10],
+                                   ph[0], ph[1], ph[2], pk, sp[9], 0.0, 0.0};
+      double* st = a.stats + (long)j * SNF_NSTAT;
+      double* st2 = a.stats2 + (long)j * SNF_NSTAT;
+      for (int q = 0; q < 21; ++q) {
+        st[q] += v[q];
+        st2[q] += v[q] * dt_old;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (rk == 1) {
+      double um[4];
+      for (int q = 0; q < 4; ++q) {
+        um[q] = __longlong_as_double((long long)a.umax_acc[q]);
+        a.umax_acc[q] = 0ull;
+      }
+      if (sc->steps_local == 1 && a.readflag == 0) {
+        const double init = fabs(g.s) > 0.5 ? 1.0 : a.uprim;
+        for (int q = 0; q < 4; ++q) um[q] = fmax(init, um[q]);
+      }
+      for (int q = 0; q < 4; ++q) sc->umax[q] = um[q];
+      double dt = 1.0 / fmax(fmax(um[0] * a.cflx, um[2] * a.cfly), um[3] * a.cflz);
+      sc->CFLv = a.CFL * a.cflvv * dt;
+      if (sc->CFLv > a.CFL && a.explicit_visc == 1) dt = 1.0 / a.cflvv;
+      double Deltat = dt;
+      double dtr = g.Re / Deltat;
+      if (Deltat < 1e-10) sc->istop = 1;
+      if (sc->time + Deltat > sc->dumptime) {
+        Deltat = fabs(sc->dumptime - sc->time);
+        dtr = g.Re / Deltat;
+        if (sc->dumptime > sc->etime - 1e-13) sc->istop = 1;
+        sc->dumptime = sc->dumptime + sc->dtimag;
+        sc->idump = 1;
+      } else if (sc->time + Deltat > sc->etime) {
+        Deltat = fabs(sc->etime - sc->time);
+        dtr = g.Re / Deltat;
+        sc->istop = 1;
+      }
+      sc->Deltat = Deltat;
+      sc->dtr = dtr;
+    }
+    if (ihist) {
+      double ener[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ener2[4] = {0, 0, 0, 0};
+      for (int j = 0; j < g.my; ++j) {
+        const double* sp = a.stat_plane + (long)j * STAT_W;
+        for (int q = 0; q < 10; ++q) ener[q] += (q < 9) ? sp[q] : sp[10];
+        const double o3s = sp[16] - g.s;
+        ener2[0] += -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);
+        ener2[1] += sp[9];
+      }
+      sc->nacum += 1;
+      sc->total_dt += sc->Deltat;
+      const double my = (double)g.my;
+      double uner[10];
+      for (int q = 0; q < 3; ++q) uner[q] = sqrt(fabs(ener[q] / my));
+      for (int q = 3; q < 6; ++q) uner[q] = ener[q] / my;
+      for (int q = 6; q < 9; ++q) uner[q] = sqrt(fabs(ener[q] / my));
+      uner[9] = ener[9] / my;
+      double* r = a.ring + (long)(sc->nrec % a.cap) * 24;
+      r[0] = sc->time;
+      r[1] = sc->Deltat;
+      for (int q = 0; q < 10; ++q) r[2 + q] = uner[q];
+      r[12] = a.CFL;
+      r[13] = sc->CFLv;
+      r[14] = sc->umax[1];
+      r[15] = sc->umax[2];
+      r[16] = sc->umax[3];
+      r[17] = sc->xoffb;
+      r[18] = sc->xofft;
+      r[19] = sc->umax[0];
+      for (int q = 0; q < 4; ++q) r[20 + q] = ener2[q] / my;
+      sc->nrec += 1;
+      if (isnan(uner[0])) sc->istop = 1;
+      if (uner[1] / (2.0 * M_PI / g.gam) < 1e-5) sc->istop = 1;
+    }
+    sh_dt = sc->Deltat;
+  }
+  __syncthreads();
+  // shear-periodic offsets for the next stage (cross.f90:726-746 / 866-884)
+  const double Deltat = sh_dt;
+  const double r1 = c_ca[rk - 1] * Deltat;
+  const double thr = a.explicit_visc == 1 ? 0.01 : 0.5;
+  double xwkb = sc->xwkb, xwkt = sc->xwkt;
+  if (fabs(g.s) > thr) {
+    xwkb = sc->xoffb + r1 * (g.s * (-g.Ly));
+    xwkt = sc->xofft + r1 * (g.s * g.Ly);
+  }
+  for (int i = tid; i < g.kxn; i += blockDim.x) {
+    const double kx = g.alp * (double)i;
+    double sn, cs;
+    sincos(kx * xwkt, &sn, &cs);
+    a.shp_new[i] = mkc(cs, -sn);
+    sincos(kx * xwkb, &sn, &cs);
+    a.shm_new[i] = mkc(cs, -sn);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    sc->xwkb = xwkb;
+    sc->xwkt = xwkt;
+    if (rk == 3) {
+      sc->xoffb = xwkb;
+      sc->xofft = xwkt;
+      sc->time = sc->time + Deltat;
+      sc->steps_done += 1;
+    }
+  }
+}
+
+// start of an advance call (cross.f90:81-103, 287-291, 383-406)
+__global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    sc->halted = 0;
+    sc->steps_local = 0;
+    sc->istop = 0;
+    sc->idump = 0;
+    sc->xwkb = sc->xoffb;
+    sc->xwkt = sc->xofft;
+  }
+  const double xb = sc->xoffb, xt = sc->xofft;
+  for (int i = tid; i < g.kxn; i += blockDim.x) {
+    const double kx = g.alp * (double)i;
+    double sn, cs;
+    sincos(kx * xt, &sn, &cs);
+    shp[i] = mkc(cs, -sn);
+    sincos(kx * xb, &sn, &cs);
+    shm[i] = mkc(cs, -sn);
+  }
+}
+
+// the reference checks istop/idump at the top of every step except the
+// second one (cross.f90:353-374)
+__global__ void k_step_begin(DevScalars* sc) {
+  if (sc->halted) return;
+  const int next = sc->steps_local + 1;
+  if (next - 1 != 1 && (sc->istop || sc->idump)) {
+    sc->halted = 1;
+    return;
+  }
+  sc->steps_local = next;
+}
+
+// kx = 0 Hermitian fix on the state (cross.f90:287-291, set_conj)
+__global__ void k_set_conj(Geom g, cplx* vor, cplx* phi) {
+  const int j = blockIdx.x;
+  for (int k = g.nz + 1 + threadIdx.x; k < g.mz; k += blockDim.x) {
+    const long b = (long)j * g.P;
+    vor[b + k] = cconj(vor[b + g.mz - k]);
+    phi[b + k] = cconj(phi[b + g.mz - k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// operator entry points (parity tests): nl independent lines, layout [y][l]
+// ---------------------------------------------------------------------------
+__global__ void k_op_deriv(int N, long nl, YCoef c, BaseFactor b, const TwistTail* tw,
+                           const cplx* shp, const cplx* shm, const cplx* u, cplx* du, cplx* ys,
+                           int order) {
+  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  DOp o{b, tw[l], shp[l], N};
+  const cplx* in[1] = {u + l};
+  cplx* y[1] = {ys + l};
+  if (order == 1)
+    deriv_forward<1, true>(o, c.rh1, in, nl, y, nl, shm[l]);
+  else
+    deriv_forward<1, false>(o, c.rh2, in, nl, y, nl, shm[l]);
+  cplx* out[1] = {du + l};
+  StoreSink<1> sk{out, nl};
+  deriv_backward<1>(o, y, nl, sk);
+}
+
+__global__ void k_op_helm(int N, long nl, YCoef c, const double* k2, const cplx* shp,
+                          const cplx* shm, const cplx* u, cplx* p, cplx* ys, HScratch hs) {
+  const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  double lm[7];
+  lapmat(c, k2[l], lm);
+  HScratch h{hs.hr + l, hs.hc + l};
+  HTail tail;
+  const cplx* in[1] = {u + l};
+  cplx* y[1] = {ys + l};
+  helm_forward<1>(lm, c.pr2, N, shp[l], shm[l], in, nl, y, nl, h, tail);
+  cplx* out[1] = {p + l};
+  StoreSink<1> sk{out, nl};
+  helm_backward<1>(N, y, nl, h, tail, sk);
+}

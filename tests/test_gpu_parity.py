@@ -1,0 +1,156 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Floating-point bar: relative L2 <= 1e-12 per field after 10 RK3 steps
+(FP64 path), <= 1e-13 for single operator applications.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.boxes import ALP, GAM, LY, box_ic, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+OP_TOL = 1e-13
+STEP_TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import shear_nnf_amd as snf
+    return snf.load_library()
+
+
+def _box(mgalx=48, my=49, mgalz=48, Re=1000.0, **kw):
+    import shear_nnf_amd as snf
+    return snf.ShearBox(mgalx, my, mgalz, Re, alp=ALP, gam=GAM, Ly=LY, **kw)
+
+
+def _oracle(mgalx=48, my=49, mgalz=48, Re=1000.0, explicit=1, **kw):
+    from oracle.grid import Grid
+    from oracle.cross import Cross
+    g = Grid(mgalx, my, mgalz, Re=Re, alp=ALP, gam=GAM, Ly=LY, s=1.0, CFL=0.5)
+    return Cross(g, explicit=explicit, **kw)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("my", [33, 49, 97])
+def test_derivyc(lib, order, my):
+    from oracle.compact import YOps
+    rng = np.random.default_rng(100 + my + order)
+    nl = 300
+    box = _box(48, my, 48)
+    u = rng.normal(size=(my, nl)) + 1j * rng.normal(size=(my, nl))
+    th = rng.uniform(-math.pi, math.pi, nl)
+    shp = np.exp(1j * th)
+    shm = np.conj(shp)
+    got = box.derivyc(u, shp, shm, order)
+    ops = YOps(my, LY / my)
+    ref = ops.deriv(u, shp, shm, order)
+    assert rel_l2(got, ref) < OP_TOL
+
+
+@pytest.mark.parametrize("my", [33, 49, 193])
+def test_lapcdy(lib, my):
+    from oracle.compact import YOps
+    rng = np.random.default_rng(7 + my)
+    nl = 257
+    box = _box(48, my, 48)
+    u = rng.normal(size=(my, nl)) + 1j * rng.normal(size=(my, nl))
+    shp = np.exp(1j * rng.uniform(-math.pi, math.pi, nl))
+    # the smallest k2 of the DNS is min(alp, gam)^2 = pi^2 (kx index 1, kz 0)
+    k2 = np.exp(rng.uniform(math.log(math.pi ** 2), math.log(4e5), nl))
+    got = box.lapcdy(u, k2, shp, np.conj(shp))
+    ops = YOps(my, LY / my)
+    ref = ops.helmholtz(u, k2, shp, np.conj(shp))
+    # (d_yy - k2) has cond ~ 4/(h^2 k2): up to ~1.5e4 here, so rounding-level
+    # differences reach ~1e-13 relative
+    assert rel_l2(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("grid", [(48, 49, 48), (32, 33, 32), (96, 97, 64)])
+def test_fourxz(lib, grid):
+    from oracle import fftxz
+    mgalx, my, mgalz = grid
+    box = _box(mgalx, my, mgalz)
+    rng = np.random.default_rng(3)
+    npl = 5
+    F = rng.normal(size=(npl, box.kxn, box.mz)) + 1j * rng.normal(size=(npl, box.kxn, box.mz))
+    got = box.fourxz(F, 1)
+    ref = fftxz.to_phys(F, mgalx, mgalz)
+    assert rel_l2(got, ref) < OP_TOL
+    f = rng.normal(size=(npl, mgalx, mgalz))
+    got2 = box.fourxz(f, -1)
+    ref2 = fftxz.to_spec(f, box.kxn, box.mz)
+    assert rel_l2(got2, ref2) < OP_TOL
+
+
+def test_hvhg(lib):
+    vor, phi, u00, w00 = box_ic(48, 49, 48)
+    rng = np.random.default_rng(11)
+    shape = vor.shape
+    v = 0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    dv = 0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    dom = 0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    box = _box()
+    box.set_state(vor, phi, u00, w00)
+    o_dvordy, o_hg, o_hv, rf0u, rf0w = box.hvhg(v, dv, dom)
+    c = _oracle()
+    c.set_state(vor, phi, u00, w00)
+    c.hg, c.hv, c.dvordy = v.copy(), dv.copy(), dom.copy()
+    c.istep = 5
+    c.hvhg(2, 0)
+    assert rel_l2(o_dvordy, c.dvordy) < OP_TOL
+    assert rel_l2(o_hg, c.hg) < OP_TOL
+    assert rel_l2(o_hv, c.hv) < OP_TOL
+    assert rel_l2(rf0u, c.rf0u) < OP_TOL
+    assert rel_l2(rf0w, c.rf0w) < OP_TOL
+
+
+def _run_pair(explicit, nstep, grid=(48, 49, 48), Re=1000.0, readflag=1, nhist=1):
+    vor, phi, u00, w00 = box_ic(*grid)
+    box = _box(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist)
+    box.set_state(vor, phi, u00, w00)
+    st = box.cross(nstep)
+    gv = box.get_state()
+    c = _oracle(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist)
+    c.set_state(vor, phi, u00, w00)
+    c.run(nstep)
+    return st, gv, box, c
+
+
+@pytest.mark.parametrize("explicit", [1, 0])
+def test_cross_10_steps(lib, explicit):
+    st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 10)
+    assert st.steps_done == 10
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    assert abs(st.time - c.time) <= 1e-13 * c.time
+    assert abs(st.xofft - c.xofft) <= 1e-13 * abs(c.xofft)
+    hist = np.array(box.history)
+    ref = np.array(c.history)
+    assert hist.shape == ref.shape
+    np.testing.assert_allclose(hist, ref, rtol=1e-11, atol=1e-14)
+
+
+def test_cross_generated_ic_first_dt(lib):
+    """readflag=0: the first step uses umax >= 1 (cross.f90:1211-1219)"""
+    st, (vor, phi, u00, w00), box, c = _run_pair(1, 3, readflag=0)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+def test_jik_roundtrip(lib):
+    vor, phi, u00, w00 = box_ic(48, 49, 48)
+    box = _box()
+    jv = np.ascontiguousarray(vor.transpose(2, 1, 0))
+    jp = np.ascontiguousarray(phi.transpose(2, 1, 0))
+    box.set_state_jik(jv, jp, u00, w00, time=1.5, xoffb=-0.25, xofft=0.25)
+    v2, p2, u2, w2 = box.get_state()
+    assert np.array_equal(v2, vor) and np.array_equal(p2, phi)
+    a, b, c2, d, t, xb, xt = box.get_state_jik()
+    assert np.array_equal(a, jv) and np.array_equal(b, jp)
+    assert (t, xb, xt) == (1.5, -0.25, 0.25)
