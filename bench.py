@@ -31,6 +31,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-grid", type=str, default="",
                     help="grid of the CPU baseline sample (default: the workload grid)")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run the streaming-copy probe on this many GiB and exit")
     return ap.parse_args()
 
 
@@ -58,27 +60,28 @@ def step_bytes(nc):
     return 992.0 * nc
 
 
-def cpu_baseline(grid, Re, explicit, seed):
-    """Times the numpy oracle (one RK3 step) on the host; returns (steps/s, sample)."""
+def cpu_baseline(grid, sample_grid, Re, explicit, seed):
+    """Times the numpy oracle for one RK3 step on the host at sample_grid and
+    scales to grid by the number of spectral coefficients (the oracle's cost
+    is linear in Nc up to the log factor of the FFTs).  One 384^3 step of the
+    oracle takes ~100 s, so the default sample is 192x193x192 (~15 s)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle.grid import Grid
     from oracle.cross import Cross
     from shear_nnf_amd.ic import make_ic
-    mgalx, my, mgalz = grid
+    mgalx, my, mgalz = sample_grid
     g = Grid(mgalx, my, mgalz, Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0, s=1.0, CFL=0.5)
     vor, phi, u00, w00 = make_ic(mgalx, my, mgalz, g.alp, g.gam, g.Ly, seed=seed)
     c = Cross(g, explicit=explicit, nhist=10 ** 9)
     c.set_state(vor, phi, u00, w00)
-    c.run(1)          # warm-up, includes the first-step v solve
     t0 = time.perf_counter()
-    n = 0
-    while True:
-        c.run(1)
-        n += 1
-        if time.perf_counter() - t0 > 10.0 or n >= 3:
-            break
+    c.run(1)
     dt = time.perf_counter() - t0
-    return n / dt, f"{n} RK3 step(s) of the numpy oracle at {mgalx}x{my}x{mgalz}, after 1 warm-up step"
+    G = Grid(grid[0], grid[1], grid[2], Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0)
+    scale = g.nc / G.nc
+    return scale / dt, (f"1 RK3 step (incl. the first-step v solve) of the numpy oracle at "
+                        f"{mgalx}x{my}x{mgalz} took {dt:.1f} s; scaled to "
+                        f"{grid[0]}x{grid[1]}x{grid[2]} by Nc ratio {scale:.4f}")
 
 
 def main():
@@ -90,6 +93,12 @@ def main():
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import shear_nnf_amd as snf
     from shear_nnf_amd.ic import make_ic
+    if a.copy_probe > 0:
+        nbytes = int(a.copy_probe * 2 ** 30)
+        ms = snf.copy_probe(nbytes, iters=20)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms,
+                          "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     mgalx, my, mgalz = (int(x) for x in a.grid.split(","))
     seed = 20260001
     vor, phi, u00, w00 = make_ic(mgalx, my, mgalz, math.pi, 2 * math.pi, 1.0, seed=seed)
@@ -149,8 +158,11 @@ def main():
         "Deltat": st2.Deltat,
     }
     if not a.no_cpu:
-        cg = tuple(int(x) for x in a.cpu_grid.split(",")) if a.cpu_grid else (mgalx, my, mgalz)
-        v, sample = cpu_baseline(cg, a.Re, a.explicit, seed)
+        if a.cpu_grid:
+            cg = tuple(int(x) for x in a.cpu_grid.split(","))
+        else:
+            cg = (max(mgalx // 2, 32), max((my - 1) // 2 + 1, 33), max(mgalz // 2, 32))
+        v, sample = cpu_baseline((mgalx, my, mgalz), cg, a.Re, a.explicit, seed)
         ncores = 1
         out["cpu_baseline"] = {"value": v, "unit": "steps/s", "cores": ncores, "kind": "port",
                                "sample": sample + " (numpy FFT and line solves, single thread)"}

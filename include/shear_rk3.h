@@ -157,6 +157,12 @@ enum snf_kernel_class {
 int snf_set_timing(snf_handle* h, int32_t on);
 int snf_get_timing(snf_handle* h, double* ms, int32_t* counts, int32_t nclass);
 
+/* Streaming-copy probe (no counterpart in the reference): copies nbytes
+ * between two device buffers iters times with a plain grid-stride kernel
+ * and returns the mean kernel time in *ms.  Used to measure attainable HBM
+ * bandwidth and to calibrate the FETCH_SIZE / WRITE_SIZE counters.        */
+int snf_copy_probe(int32_t device, int64_t nbytes, int32_t iters, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,7 +29,7 @@ EXPORTS = (
     "snf_state_upload", "snf_state_download", "snf_state_upload_ykxkz",
     "snf_state_download_ykxkz", "snf_advance", "snf_history", "snf_statistics",
     "snf_op_derivy", "snf_op_helmholtz", "snf_op_fourxz", "snf_op_hvhg",
-    "snf_set_timing", "snf_get_timing",
+    "snf_set_timing", "snf_get_timing", "snf_copy_probe",
 )
 KERNEL_CLASSES = ("xz_A", "xz_B", "xz_C", "y_lines", "small")
 
@@ -90,8 +90,19 @@ def load_library(path=LIB_PATH):
     lib.snf_op_hvhg.argtypes = [H, dp, dp, dp, dp, dp, dp, dp, dp]
     lib.snf_set_timing.argtypes = [H, ctypes.c_int32]
     lib.snf_get_timing.argtypes = [H, dp, P(ctypes.c_int32), ctypes.c_int32]
+    lib.snf_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, dp]
     _lib = lib
     return lib
+
+
+def copy_probe(nbytes, iters=20, device=0):
+    """mean ms of a plain device copy of nbytes (read nbytes + write nbytes)"""
+    lib = load_library()
+    ms = ctypes.c_double()
+    rc = lib.snf_copy_probe(device, int(nbytes), int(iters), ctypes.byref(ms))
+    if rc != SNF_OK:
+        raise SnfError(lib.snf_last_error().decode())
+    return ms.value
 
 
 def _ptr(a):

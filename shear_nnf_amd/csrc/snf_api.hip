@@ -163,6 +163,11 @@ YArgs yargs(snf_handle* h) {
   return a;
 }
 
+__global__ void k_copy_probe(const double2* __restrict__ a, double2* __restrict__ b, long n) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+    b[q] = a[q];
+}
+
 __global__ void k_cplx_to_real(const cplx* a, const cplx* b, double* ra, double* rb, int n) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) {
@@ -776,6 +781,33 @@ int snf_set_timing(snf_handle* h, int32_t on) {
     h->t_ms[q] = 0.0;
     h->t_cnt[q] = 0;
   }
+  return SNF_OK;
+}
+
+int snf_copy_probe(int32_t device, int64_t nbytes, int32_t iters, double* ms) {
+  if (nbytes < 4096 || iters < 1 || !ms) return fail(SNF_BAD_PARAM, "bad argument");
+  HIPCHK(hipSetDevice(device));
+  const long n = (long)(nbytes / (long)sizeof(double2));
+  double2 *a, *b;
+  HIPCHK(hipMalloc(&a, n * sizeof(double2)));
+  HIPCHK(hipMalloc(&b, n * sizeof(double2)));
+  HIPCHK(hipMemset(a, 0, n * sizeof(double2)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const unsigned blocks = 256 * 32;
+  k_copy_probe<<<blocks, 256>>>(a, b, n);  // warm-up
+  HIPCHK(hipEventRecord(e0));
+  for (int q = 0; q < iters; ++q) k_copy_probe<<<blocks, 256>>>(a, b, n);
+  HIPCHK(hipEventRecord(e1));
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   return SNF_OK;
 }
 
