@@ -15,6 +15,7 @@
 #include "snf_internal.h"
 #include "snf_setup.h"
 #include "snf_xz.hip"
+#include "snf_scalar.hip"
 #include "snf_ykern.hip"
 
 namespace {
@@ -64,8 +65,10 @@ struct snf_handle {
   hipStream_t stream = nullptr;
   FFTPlan px{}, pz{};
   // device memory
-  cplx *vor = nullptr, *phi = nullptr, *hg = nullptr, *hv = nullptr, *dvordy = nullptr;
-  cplx *vorwk = nullptr, *phiwk = nullptr, *s1 = nullptr, *s2 = nullptr;
+  cplx *vor = nullptr, *phi = nullptr, *vorwk = nullptr, *phiwk = nullptr;
+  cplx *v = nullptr, *dome = nullptr, *dv = nullptr;        // y-stage outputs
+  cplx *rdome = nullptr, *rhg = nullptr, *rhv = nullptr;    // x-z stage outputs
+  cplx *s1 = nullptr, *s2 = nullptr, *s3 = nullptr;
   cplx *hyb_in = nullptr, *hyb_out = nullptr;
   double* hr = nullptr;
   cplx* hc = nullptr;
@@ -88,6 +91,7 @@ struct snf_handle {
   BaseFactor b1{}, b2{};
   TwistTail id1{}, id2{};
   size_t ldsA = 0, ldsB = 0, ldsC = 0;
+  int zb = 8;   // z rows per block of the x-row kernel
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
   // kernel timing
   bool timing = false;
@@ -142,13 +146,17 @@ YArgs yargs(snf_handle* h) {
   a.shm_new = h->shm[1 - h->cur];
   a.vor = h->vor;
   a.phi = h->phi;
-  a.hg = h->hg;
-  a.hv = h->hv;
-  a.dvordy = h->dvordy;
   a.vorwk = h->vorwk;
   a.phiwk = h->phiwk;
+  a.v = h->v;
+  a.dome = h->dome;
+  a.dv = h->dv;
+  a.rdome = h->rdome;
+  a.rhg = h->rhg;
+  a.rhv = h->rhv;
   a.s1 = h->s1;
   a.s2 = h->s2;
+  a.s3 = h->s3;
   a.hs = HScratch{h->hr, h->hc};
   a.u00c = h->u00c;
   a.w00c = h->w00c;
@@ -211,19 +219,20 @@ int launch_xz(snf_handle* h, int rk) {
   k_cplx_to_real<<<(g.my + 255) / 256, 256, 0, h->stream>>>(h->u00c, h->w00c, h->u00r, h->w00r, g.my);
   dim3 gA(g.kxn, g.my);
   { TimedLaunch tl(h, SNF_K_XZ_A);
-  k_xz_A<<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->hg, h->hv, h->dvordy, h->u00r,
+  k_xz_A<<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                           h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->pz,
                                           h->twz, h->sc); }
   if (rk == 1)
-    k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->hg, h->hv, h->dvordy, h->u00r,
+    k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                           h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->stat_plane,
                                           h->sc, h->p.nhist);
-  dim3 gB(g.mgalz / 2, g.my);
+  dim3 gB(g.mgalz / h->zb, g.my);
   { TimedLaunch tl(h, SNF_K_XZ_B);
-  k_xz_B<<<gB, 256, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk, h->umax_acc,
-                                          h->phist, h->sc, h->p.nhist); }
+  k_xz_B<<<gB, XZ_B_THREADS, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk,
+                                                   h->umax_acc, h->phist, h->sc, h->p.nhist,
+                                                   h->zb); }
   { TimedLaunch tl(h, SNF_K_XZ_C);
-  k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->dvordy, h->hg, h->hv, h->rf0u, h->rf0w,
+  k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg, h->rhv, h->rf0u, h->rf0w,
                                           h->pz, h->twz, h->sc); }
   HIPCHK(hipGetLastError());
   return SNF_OK;
@@ -236,6 +245,7 @@ ScalarArgs sargs(snf_handle* h) {
   a.nhist = h->p.nhist;
   a.readflag = h->p.readflag;
   a.explicit_visc = h->p.explicit_visc;
+  a.zb_rows = h->zb;
   a.cflx = h->cflx;
   a.cfly = h->cfly;
   a.cflz = h->cflz;
@@ -353,9 +363,13 @@ int snf_create(const snf_params* p, snf_handle** out) {
   const size_t NC = h->NC, NS = (size_t)my * h->PS;
   TRY(A(&h->vor, NC));
   TRY(A(&h->phi, NC));
-  TRY(A(&h->hg, NC));
-  TRY(A(&h->hv, NC));
-  TRY(A(&h->dvordy, NC));
+  TRY(A(&h->v, NC));
+  TRY(A(&h->dome, NC));
+  TRY(A(&h->dv, NC));
+  TRY(A(&h->rdome, NC));
+  TRY(A(&h->rhg, NC));
+  TRY(A(&h->rhv, NC));
+  TRY(A(&h->s3, NS));
   TRY(A(&h->vorwk, NC));
   TRY(A(&h->phiwk, NC));
   TRY(A(&h->s1, NS));
@@ -386,7 +400,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->sc, 1));
   TRY(A(&h->umax_acc, 4));
   TRY(A(&h->stat_plane, (size_t)my * STAT_W));
-  TRY(A(&h->phist, (size_t)my * (mgalz / 2) * 3));
+  TRY(A(&h->phist, (size_t)my * mgalz * 3));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
   TRY(A(&h->ring, (size_t)h->cap * 24));
@@ -421,12 +435,23 @@ int snf_create(const snf_params* p, snf_handle** out) {
     TRY(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy scalars"));
   }
-  h->ldsA = 2 * 3 * (size_t)mgalz * sizeof(cplx);
-  h->ldsB = 2 * 6 * (size_t)mgalx * sizeof(cplx);
-  h->ldsC = h->ldsA;
-  hipFuncSetAttribute((const void*)k_xz_A, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
-  hipFuncSetAttribute((const void*)k_xz_B, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
-  hipFuncSetAttribute((const void*)k_xz_C, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
+  // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place)
+  h->zb = 8;
+  while (h->zb > 2 && (3 * (size_t)h->zb * mgalx * sizeof(cplx) > 144 * 1024 || mgalz % h->zb != 0 ||
+                       (h->zb / 2) * mgalx > XZ_B_MAXW * XZ_B_THREADS ||
+                       fft_max_values(h->px, 3 * h->zb, XZ_B_THREADS) > XZ_MAXV))
+    h->zb /= 2;
+  if (mgalz % h->zb != 0 || fft_max_values(h->px, 3 * h->zb, XZ_B_THREADS) > XZ_MAXV ||
+      (h->zb / 2) * mgalx > XZ_B_MAXW * XZ_B_THREADS) {
+    snf_destroy(h);
+    return fail(SNF_BAD_PARAM, "x-z transform sizes exceed the LDS plan of this build");
+  }
+  h->ldsA = 2 * 3 * (size_t)mgalz * sizeof(cplx);   // ping-pong, 3 sequences
+  h->ldsB = 3 * (size_t)h->zb * mgalx * sizeof(cplx);
+  h->ldsC = 2 * 3 * (size_t)mgalz * sizeof(cplx);
+  (void)hipFuncSetAttribute((const void*)k_xz_A, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
+  (void)hipFuncSetAttribute((const void*)k_xz_B, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
+  (void)hipFuncSetAttribute((const void*)k_xz_C, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
 #undef TRY
   *out = h;
   return SNF_OK;
@@ -435,7 +460,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
 int snf_destroy(snf_handle* h) {
   if (!h) return SNF_OK;
   hipSetDevice(h->dev);
-  void* ptrs[] = {h->vor, h->phi, h->hg, h->hv, h->dvordy, h->vorwk, h->phiwk, h->s1, h->s2,
+  void* ptrs[] = {h->vor, h->phi, h->v, h->dome, h->dv, h->rdome, h->rhg, h->rhv, h->s3,
+                  h->vorwk, h->phiwk, h->s1, h->s2,
                   h->hyb_in, h->hyb_out, h->hr, h->hc, h->u00c, h->w00c, h->u00wk, h->w00wk,
                   h->u00r, h->w00r, h->rf0u, h->rf0w, h->dudy, h->dwdy, h->base, h->tw[0],
                   h->tw[1], h->shp[0], h->shp[1], h->shm[0], h->shm[1], h->twx, h->twz, h->sc,
@@ -589,7 +615,7 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
       {
         TimedLaunch tl(h, SNF_K_Y);
         if (h->p.explicit_visc == 1)
-          k_y_explicit<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h), rk);
+          k_y_explicit<<<(unsigned)((2 * h->P + 2 + 127) / 128), 128, 0, h->stream>>>(yargs(h), rk);
         else
           k_y_implicit<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h), rk);
       }
@@ -749,18 +775,18 @@ int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const do
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
   const size_t NC = h->NC;
-  HIPCHK(hipMemcpy(h->hg, hg_v, NC * sizeof(cplx), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->hv, hv_dv, NC * sizeof(cplx), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->dvordy, dvordy, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->v, hg_v, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dv, hv_dv, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dome, dvordy, NC * sizeof(cplx), hipMemcpyHostToDevice));
   // U, W derivatives as at the top of hvhg (cross.f90:1198-1205)
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur]);
   k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
   int rc;
   if ((rc = launch_xz(h, 2))) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
-  if (out_dvordy) HIPCHK(hipMemcpy(out_dvordy, h->dvordy, NC * sizeof(cplx), hipMemcpyDeviceToHost));
-  if (out_hg) HIPCHK(hipMemcpy(out_hg, h->hg, NC * sizeof(cplx), hipMemcpyDeviceToHost));
-  if (out_hv) HIPCHK(hipMemcpy(out_hv, h->hv, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (out_dvordy) HIPCHK(hipMemcpy(out_dvordy, h->rdome, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (out_hg) HIPCHK(hipMemcpy(out_hg, h->rhg, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (out_hv) HIPCHK(hipMemcpy(out_hv, h->rhv, NC * sizeof(cplx), hipMemcpyDeviceToHost));
   if (rf0u) HIPCHK(hipMemcpy(rf0u, h->rf0u, g.my * sizeof(double), hipMemcpyDeviceToHost));
   if (rf0w) HIPCHK(hipMemcpy(rf0w, h->rf0w, g.my * sizeof(double), hipMemcpyDeviceToHost));
   return SNF_OK;

@@ -72,6 +72,80 @@ __device__ __forceinline__ void stockham_pass(const cplx* src, cplx* dst, int n,
   }
 }
 
+// In-place variant: every pass first loads all butterfly inputs of the
+// thread into registers (at most MB butterflies per thread), synchronises,
+// then writes the outputs to their Stockham positions in the same buffer.
+// Halves the LDS footprint of the ping-pong form.  The caller guarantees
+// nseq*n/R <= MB*blockDim.x for every radix R of the plan.
+template <int R, int MB>
+__device__ __forceinline__ void stockham_pass_ip(cplx* buf, int n, int nseq, int Ns,
+                                                 const cplx* tw, int sign) {
+  const int nb = n / R;
+  const int total = nseq * nb;
+  const int stw = n / (Ns * R);
+  cplx v[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      const int s = t / nb;
+      const int j = t - s * nb;
+      const int k = j % Ns;
+      const cplx* sp = buf + s * n;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cplx x = sp[j + r * nb];
+        if (r > 0) x = cmul_sign(x, tw[r * k * stw], sign);
+        v[q][r] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      const int s = t / nb;
+      const int j = t - s * nb;
+      const int k = j % Ns;
+      dft_small<R>(v[q], sign);
+      cplx* dp = buf + s * n + (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dp[r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// MAXV: complex values a thread may hold per pass (R * butterflies)
+template <int MAXV>
+__device__ __forceinline__ void fft_lds_ip(cplx* buf, int nseq, const FFTPlan& pl,
+                                           const cplx* tw, int sign) {
+  int Ns = 1;
+  for (int p = 0; p < pl.nr; ++p) {
+    const int R = pl.r[p];
+    if (R == 4)
+      stockham_pass_ip<4, MAXV / 4>(buf, pl.n, nseq, Ns, tw, sign);
+    else if (R == 2)
+      stockham_pass_ip<2, MAXV / 2>(buf, pl.n, nseq, Ns, tw, sign);
+    else
+      stockham_pass_ip<3, MAXV / 3>(buf, pl.n, nseq, Ns, tw, sign);
+    Ns *= R;
+  }
+}
+
+// largest number of butterflies per thread any pass of the plan needs
+inline int fft_max_values(const FFTPlan& pl, int nseq, int nthreads) {
+  int m = 0;
+  for (int p = 0; p < pl.nr; ++p) {
+    const int R = pl.r[p];
+    const int total = nseq * (pl.n / R);
+    const int per = (total + nthreads - 1) / nthreads;
+    if (per * R > m) m = per * R;
+  }
+  return m;
+}
+
 // Transforms nseq sequences in a (result in the returned buffer, a or b).
 // sign = -1: forward exp(-i...), sign = +1: inverse exp(+i...), unnormalised.
 __device__ __forceinline__ cplx* fft_lds(cplx* a, cplx* b, int nseq, const FFTPlan& pl,

@@ -22,6 +22,23 @@ __device__ __forceinline__ cplx crmul(double r, cplx a) { return mkc(r * a.x, r 
 __device__ __forceinline__ cplx cconj(cplx a) { return mkc(a.x, -a.y); }
 __device__ __forceinline__ double cabs2(cplx a) { return a.x * a.x + a.y * a.y; }
 
+// Loads and stores through the global address space.  Pointers that reach
+// a device function through a struct lose their address space, and the
+// generic (flat) accesses the compiler then emits are waited for with
+// vmcnt(0) lgkmcnt(0), which serialises every prefetch behind them.
+#define SNF_AS1 __attribute__((address_space(1)))
+__device__ __forceinline__ cplx gld(const cplx* p) {
+  const SNF_AS1 double* d = (const SNF_AS1 double*)p;
+  return make_double2(d[0], d[1]);
+}
+__device__ __forceinline__ void gst(cplx* p, cplx v) {
+  SNF_AS1 double* d = (SNF_AS1 double*)p;
+  d[0] = v.x;
+  d[1] = v.y;
+}
+__device__ __forceinline__ double gldd(const double* p) { return *(const SNF_AS1 double*)p; }
+__device__ __forceinline__ void gstd(double* p, double v) { *(SNF_AS1 double*)p = v; }
+
 // Grid description passed by value to every kernel.
 struct Geom {
   int my, kxn, mz, nz, mgalx, mgalz;

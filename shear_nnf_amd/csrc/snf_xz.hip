@@ -89,8 +89,13 @@ __device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i,
 }
 
 // ---------------------------------------------------------------------------
-// A: spectral -> hybrid (inverse z DFT); fields u,v,w,o1,o2,o3
+// A: spectral -> hybrid (inverse z DFT) of u,v,w,o1,o2,o3; one kx column per
+// block, two groups of three sequences (ping-pong LDS buffers)
 // ---------------------------------------------------------------------------
+#define XZ_MAXV 12       // values per thread per FFT pass in the x-row kernel
+#define XZ_B_MAXW 2      // (row pair, x) product items per thread
+
+
 __global__ void __launch_bounds__(256)
 k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
        const cplx* __restrict__ hg, const cplx* __restrict__ hv,
@@ -128,34 +133,38 @@ k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
 }
 
 // ---------------------------------------------------------------------------
-// B: two z-rows per block: c2r in x, u x omega, r2c in x
+// B: ZB z-rows per block: c2r in x, u x omega, r2c in x.  Reads and writes
+// runs of ZB consecutive z values per (field, kx) of the [f][y][kx][z]
+// hybrid layout.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long dbits(double v) {
   return (unsigned long long)__double_as_longlong(v);
 }
 
-__global__ void __launch_bounds__(256)
+#define XZ_B_THREADS 1024
+
+__global__ void __launch_bounds__(XZ_B_THREADS)
 k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px,
        const cplx* __restrict__ twx, int rk, unsigned long long* umax_acc,
-       double* __restrict__ phist, const DevScalars* sc, int nhist) {
+       double* __restrict__ phist, const DevScalars* sc, int nhist, int ZB) {
   if (sc->halted) return;
   extern __shared__ cplx lds[];
   const int Mx = g.mgalx, Nz = g.mgalz, kxn = g.kxn;
-  cplx* a = lds;              // 6 sequences (2 rows x 3 pairs)
-  cplx* b = lds + 6 * Mx;
-  const int zb = blockIdx.x, j = blockIdx.y, z0 = 2 * zb;
+  const int zb = blockIdx.x, j = blockIdx.y, z0 = ZB * zb;
   const long fstride = (long)g.my * kxn * Nz;
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
-  for (int t = threadIdx.x; t < 6 * Mx; t += blockDim.x) a[t] = mkc(0.0, 0.0);
+  const int nseq = 3 * ZB;    // sequence s = p*ZB + r (pair p, row r)
+  for (int t = threadIdx.x; t < nseq * Mx; t += blockDim.x) lds[t] = mkc(0.0, 0.0);
   __syncthreads();
-  // build Z_pair = G_a + i G_b with Hermitian extension in kx
-  for (int t = threadIdx.x; t < 6 * kxn; t += blockDim.x) {
-    const int seq = t / kxn, i = t - seq * kxn;
-    const int r = seq / 3, p = seq - r * 3;
+  // Z_pair = G_a + i G_b with the Hermitian extension in kx
+  for (int t = threadIdx.x; t < 3 * kxn * ZB; t += blockDim.x) {
+    const int r = t % ZB;
+    const int pi_ = t / ZB;
+    const int i = pi_ % kxn, p = pi_ / kxn;
     const long off = ((long)j * kxn + i) * Nz + z0 + r;
     const cplx Ga = hin[(2 * p) * fstride + off];
     const cplx Gb = hin[(2 * p + 1) * fstride + off];
-    cplx* s = a + seq * Mx;
+    cplx* s = lds + (p * ZB + r) * Mx;
     if (i == 0) {
       s[0] = mkc(Ga.x, Gb.x);
     } else {
@@ -164,82 +173,112 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
     }
   }
   __syncthreads();
-  cplx* X = fft_lds(a, b, 6, px, twx, +1);
-  cplx* Y = (X == a) ? b : a;
+  fft_lds_ip<XZ_MAXV>(lds, nseq, px, twx, +1);
+  // u x omega: each (row pair, x) handled by one thread; results go to
+  // sequences 0..ZB-1 (H1 + i H2 per row) and ZB..3ZB/2-1 (H3 of two rows)
   const double c2 = 2.0 * g.chi;
   const double sy = g.s * ((double)j * g.dy - 0.5 * g.Ly);
   double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
   double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-  for (int x = threadIdx.x; x < Mx; x += blockDim.x) {
-    double h3[2];
+  const int npair = ZB / 2;
+  const int nwork = npair * Mx;
+  constexpr int MAXW = XZ_B_MAXW;   // host-checked
+  double h1[MAXW][2], h2[MAXW][2], h3[MAXW][2];
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const cplx z0v = X[(r * 3 + 0) * Mx + x];
-      const cplx z1v = X[(r * 3 + 1) * Mx + x];
-      const cplx z2v = X[(r * 3 + 2) * Mx + x];
-      const double u = z0v.x, v = z0v.y, w = z1v.x, o1 = z1v.y, o2 = z2v.x, o3 = z2v.y;
-      if (rk == 1) {
-        m0 = fmax(m0, fabs(u));
-        m1 = fmax(m1, fabs(u + sy));
-        m2 = fmax(m2, fabs(v));
-        m3 = fmax(m3, fabs(w));
+  for (int q = 0; q < MAXW; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < nwork) {
+      const int rp = t / Mx, x = t - rp * Mx;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int r = 2 * rp + h;
+        const cplx z0v = lds[(0 * ZB + r) * Mx + x];
+        const cplx z1v = lds[(1 * ZB + r) * Mx + x];
+        const cplx z2v = lds[(2 * ZB + r) * Mx + x];
+        const double u = z0v.x, v = z0v.y, w = z1v.x, o1 = z1v.y, o2 = z2v.x, o3 = z2v.y;
+        if (rk == 1) {
+          m0 = fmax(m0, fabs(u));
+          m1 = fmax(m1, fabs(u + sy));
+          m2 = fmax(m2, fabs(v));
+          m3 = fmax(m3, fabs(w));
+        }
+        if (ihist) {
+          p0 += v * (u * u);
+          p1 += v * (w * w);
+          p2 += v * v * v;
+        }
+        h1[q][h] = v * (o3 + c2) - w * o2;
+        h2[q][h] = w * o1 - u * (o3 + c2);
+        h3[q][h] = u * o2 - v * o1;
       }
-      if (ihist) {
-        p0 += v * (u * u);
-        p1 += v * (w * w);
-        p2 += v * v * v;
-      }
-      const double H1 = v * (o3 + c2) - w * o2;
-      const double H2 = w * o1 - u * (o3 + c2);
-      h3[r] = u * o2 - v * o1;
-      Y[r * Mx + x] = mkc(H1, H2);
     }
-    Y[2 * Mx + x] = mkc(h3[0], h3[1]);
   }
-  // block reductions (deterministic tree)
-  __shared__ double red[256 * 4];
-  if (rk == 1) {
-    red[threadIdx.x] = m0;
-    red[256 + threadIdx.x] = m1;
-    red[512 + threadIdx.x] = m2;
-    red[768 + threadIdx.x] = m3;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (threadIdx.x < st) {
+  __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          red[q * 256 + threadIdx.x] = fmax(red[q * 256 + threadIdx.x], red[q * 256 + threadIdx.x + st]);
-      }
-      __syncthreads();
+  for (int q = 0; q < MAXW; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < nwork) {
+      const int rp = t / Mx, x = t - rp * Mx;
+      lds[(2 * rp) * Mx + x] = mkc(h1[q][0], h2[q][0]);
+      lds[(2 * rp + 1) * Mx + x] = mkc(h1[q][1], h2[q][1]);
+      lds[(ZB + rp) * Mx + x] = mkc(h3[q][0], h3[q][1]);
     }
-    if (threadIdx.x < 4) atomicMax(&umax_acc[threadIdx.x], dbits(red[threadIdx.x * 256]));
+  }
+  // block reductions: wave shuffles, then one value per wave (fixed order)
+  __shared__ double red[4][XZ_B_THREADS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+  if (rk == 1) {
+    for (int o = 32; o > 0; o >>= 1) {
+      m0 = fmax(m0, __shfl_xor(m0, o, 64));
+      m1 = fmax(m1, __shfl_xor(m1, o, 64));
+      m2 = fmax(m2, __shfl_xor(m2, o, 64));
+      m3 = fmax(m3, __shfl_xor(m3, o, 64));
+    }
+    if (lane == 0) {
+      red[0][wv] = m0;
+      red[1][wv] = m1;
+      red[2][wv] = m2;
+      red[3][wv] = m3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double m = 0.0;
+      for (int q = 0; q < nwv; ++q) m = fmax(m, red[threadIdx.x][q]);
+      atomicMax(&umax_acc[threadIdx.x], dbits(m));
+    }
     __syncthreads();
   }
   if (ihist) {
-    red[threadIdx.x] = p0;
-    red[256 + threadIdx.x] = p1;
-    red[512 + threadIdx.x] = p2;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (threadIdx.x < st) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) red[q * 256 + threadIdx.x] += red[q * 256 + threadIdx.x + st];
-      }
-      __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+      p0 += __shfl_xor(p0, o, 64);
+      p1 += __shfl_xor(p1, o, 64);
+      p2 += __shfl_xor(p2, o, 64);
     }
-    if (threadIdx.x < 3) phist[((long)j * (Nz / 2) + zb) * 3 + threadIdx.x] = red[threadIdx.x * 256];
+    if (lane == 0) {
+      red[0][wv] = p0;
+      red[1][wv] = p1;
+      red[2][wv] = p2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double acc = 0.0;
+      for (int q = 0; q < nwv; ++q) acc += red[threadIdx.x][q];
+      phist[((long)j * (Nz / ZB) + zb) * 3 + threadIdx.x] = acc;
+    }
   }
   __syncthreads();
-  cplx* Z = fft_lds(Y, X, 3, px, twx, -1);
-  for (int t = threadIdx.x; t < 2 * kxn; t += blockDim.x) {
-    const int r = t / kxn, k = t - r * kxn;
+  fft_lds_ip<XZ_MAXV>(lds, ZB + npair, px, twx, -1);
+  // unpack the pairs, z fastest among consecutive threads
+  for (int t = threadIdx.x; t < kxn * ZB; t += blockDim.x) {
+    const int r = t % ZB, k = t / ZB;
     const int km = (k == 0) ? 0 : Mx - k;
-    const cplx A = Z[r * Mx + k], Am = Z[r * Mx + km];
-    const cplx B = Z[2 * Mx + k], Bm = Z[2 * Mx + km];
+    const cplx A = lds[r * Mx + k], Am = lds[r * Mx + km];
+    const cplx* Bs = lds + (ZB + r / 2) * Mx;
+    const cplx B = Bs[k], Bm = Bs[km];
     const cplx H1 = mkc(0.5 * (A.x + Am.x), 0.5 * (A.y - Am.y));
     const cplx H2 = mkc(0.5 * (A.y + Am.y), -0.5 * (A.x - Am.x));
-    const cplx H3 = (r == 0) ? mkc(0.5 * (B.x + Bm.x), 0.5 * (B.y - Bm.y))
-                             : mkc(0.5 * (B.y + Bm.y), -0.5 * (B.x - Bm.x));
+    const cplx H3 = (r % 2 == 0) ? mkc(0.5 * (B.x + Bm.x), 0.5 * (B.y - Bm.y))
+                                 : mkc(0.5 * (B.y + Bm.y), -0.5 * (B.x - Bm.x));
     const long off = ((long)j * kxn + k) * Nz + z0 + r;
     hout[off] = H1;
     hout[fstride + off] = H2;
@@ -252,8 +291,9 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
 // cross.f90:1533-1544) and the 00-mode terms rf0u, rf0w (cross.f90:1489-1490)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-k_xz_C(Geom g, const cplx* __restrict__ hout, cplx* __restrict__ dvordy,
-       cplx* __restrict__ hg, cplx* __restrict__ hv, double* __restrict__ rf0u,
+k_xz_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
+       cplx* __restrict__ dvordy, cplx* __restrict__ hg, cplx* __restrict__ hv,
+       double* __restrict__ rf0u,
        double* __restrict__ rf0w, FFTPlan pz, const cplx* __restrict__ twz,
        const DevScalars* sc) {
   if (sc->halted) return;
@@ -280,7 +320,7 @@ k_xz_C(Geom g, const cplx* __restrict__ hout, cplx* __restrict__ dvordy,
     const cplx h3 = crmul(norm, r[2 * Nz + pos]);
     const double kz = kz_of(k, g.nz, g.mz, g.gam);
     const long idx = base + k;
-    const cplx v = hg[idx];
+    const cplx v = vel[idx];
     dvordy[idx] = cadd(mul_i(kx, h1), mul_i(kz, h3));
     const cplx d = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);
     hg[idx] = cadd(mul_i(-kx, h3), mul_i(kz, d));

@@ -1,121 +1,151 @@
-// y-line stages of one RK3 substep (cross.f90:428-1000), the per-kx
-// twisted border factors, and the scalar/CFL kernel.
+// y-line stages of one RK3 substep (cross.f90:428-1000).
 //
-// Thread-per-line kernels: line L = i*mz + k (kx index i, kz index k) of the
-// [y][kx][kz] fields; two extra "virtual" lines L = P and P+1 carry the
-// real x-z mean profiles U(y) = u00 and W(y) = w00, which the reference
-// advances with the same compact operators without a twist (m=1 calls of
-// derivyc / visc3d / lapcdy).
+// Explicit substeps use two threads per (kx,kz) line, which the reference
+// equations allow because the omega_y and phi updates are independent:
+//   vor chain:  d2(vor) -> viscous + RK update of vor, vorwk -> d(vor)/dy
+//   phi chain:  d(rdome)/dy and d2(phi) -> RK update of phi, phiwk
+//               -> v = lapcdy(phi) -> dv/dy
+// The x-z stage outputs (rdome, rhg, rhv) and the y-stage outputs consumed by
+// the next x-z stage (v, dome, dv) live in separate fields, so the chains
+// never read a line the other chain writes.  Two extra threads advance the
+// real x-z means U(y) = u00 and W(y) = w00 with the untwisted operators
+// (m=1 calls of derivyc / visc3d / lapcdy).
 #include "snf_internal.h"
 #include "snf_lines.h"
 
-// SMR RK3 constants (cross.f90:18-24)
-__constant__ double c_gama[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
-__constant__ double c_alpha[3] = {29.0 / 96.0, -3.0 / 40.0, 1.0 / 6.0};
-__constant__ double c_ibeta[3] = {160.0 / 37.0, 24.0 / 5.0, 6.0};
-__constant__ double c_zeta[3] = {-17.0 / 60.0, -5.0 / 12.0, 0.0};
-__constant__ double c_ca[3] = {8.0 / 15.0, 2.0 / 3.0, 1.0};
-
-__device__ __forceinline__ double kz_val(int k, int nz, int mz, double gam) {
-  return k <= nz ? gam * (double)k : -gam * (double)(mz - k);
-}
-
-// exp(i kx y_j m) of moving_frame (operators.f90:86-145)
-__device__ __forceinline__ cplx mframe(double kx, double yj, double m) {
-  double sn, cs;
-  sincos((kx * yj) * m, &sn, &cs);
-  return mkc(cs, sn);
-}
-
 // ---------------------------------------------------------------------------
-// per-kx twisted border tail of a derivative factor (modChol_addshear with
-// the border pre-multiplication of derivyc); one wave per (kx, operator)
+// two twisted derivative operators of the same line advanced in lock step
+// (d/dy of one field and d2/dy2 of another, as in cross.f90:565-664); the
+// sweeps are independent, and interleaving them keeps both streams in flight
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+template <bool FA, bool FB, int D>
+__device__ __forceinline__ void deriv_forward_dual(const DOp& oa, const double* ca, const DOp& ob,
+                                                   const double* cb, const cplx* ina,
+                                                   const cplx* inb, long pin, cplx* ysa,
+                                                   cplx* ysb, long py, cplx shm) {
+  const int N = oa.N;
+  const int mlast = N - 7;
+  FwdState<1, FA> sa;
+  FwdState<1, FB> sb;
+  window_init(ina, pin, N, oa.shp, shm, sa.w[0], sa.bot[0], sa.top[0]);
+  window_init(inb, pin, N, ob.shp, shm, sb.w[0], sb.bot[0], sb.top[0]);
+  sa.y1[0] = sa.y2[0] = sa.y3[0] = sa.E[0] = sa.F[0] = sa.G[0] = mkc(0, 0);
+  sb.y1[0] = sb.y2[0] = sb.y3[0] = sb.E[0] = sb.F[0] = sb.G[0] = mkc(0, 0);
+  cplx pa[D], pb[D];
+  cplx* ya[1] = {ysa};
+  cplx* yb[1] = {ysb};
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    pa[q] = gld(ina + (long)(3 + q) * pin);
+    pb[q] = gld(inb + (long)(3 + q) * pin);
+  }
+  for (int jb = 0; jb <= mlast; jb += D) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const int j = jb + q;
+      if (j <= mlast) {
+        const int rn = j + 3 + D;
+        const cplx wa = pa[q], wb = pb[q];
+        if (rn <= N - 4) {
+          pa[q] = gld(ina + (long)rn * pin);
+          pb[q] = gld(inb + (long)rn * pin);
+        }
+        deriv_fwd_row<1, FA, 0>(oa, ca, j, sa, &wa, ya, py);
+        deriv_fwd_row<1, FB, 0>(ob, cb, j, sb, &wb, yb, py);
+      }
+    }
+  }
+  deriv_fwd_row<1, FA, 1>(oa, ca, N - 6, sa, &sa.bot[0][0], ya, py);
+  deriv_fwd_row<1, FB, 1>(ob, cb, N - 6, sb, &sb.bot[0][0], yb, py);
+  deriv_fwd_row<1, FA, 2>(oa, ca, N - 5, sa, &sa.bot[0][1], ya, py);
+  deriv_fwd_row<1, FB, 2>(ob, cb, N - 5, sb, &sb.bot[0][1], yb, py);
+  deriv_fwd_row<1, FA, 3>(oa, ca, N - 4, sa, &sa.bot[0][2], ya, py);
+  deriv_fwd_row<1, FB, 3>(ob, cb, N - 4, sb, &sb.bot[0][2], yb, py);
+  border_rows<1, FA>(oa, ca, sa, ya, py);
+  border_rows<1, FB>(ob, cb, sb, yb, py);
 }
 
-__global__ void __launch_bounds__(64)
-k_twist(int N, BaseFactor b1, BaseFactor b2, YCoef c, const cplx* __restrict__ shp_arr,
-        TwistTail* __restrict__ out, int nk, const DevScalars* sc) {
-  if (sc && sc->halted) return;
-  const int kx = blockIdx.x, op = blockIdx.y, lane = threadIdx.x;
-  const BaseFactor b = op == 0 ? b1 : b2;
-  const double* a = op == 0 ? c.pr1 : c.pr2;
-  const double a0 = a[3], a1 = a[4], a2 = a[5], a3 = a[6];
-  const cplx t = shp_arr[kx];
-  __shared__ cplx sLe[3], sLf[3], sLg[3];
-  auto Ler = [&](int k) { return crmul(b.Ler[k], t); };
-  auto Lfr = [&](int k) { return crmul(b.Lfr[k], t); };
-  auto Lgr = [&](int k) { return crmul(b.Lgr[k], t); };
-  auto rec = [&](cplx x3, cplx x2, cplx x1, int i, double av) {
-    cplx r;
-    r.x = b.Dm1[i] * (av - x3.x * b.D[i - 3] * b.Lc[i] - x2.x * b.D[i - 2] * b.Lb[i] - x1.x * b.D[i - 1] * b.La[i]);
-    r.y = b.Dm1[i] * (0.0 - x3.y * b.D[i - 3] * b.Lc[i] - x2.y * b.D[i - 2] * b.Lb[i] - x1.y * b.D[i - 1] * b.La[i]);
-    return r;
+// Backward sweeps of two operators; the sink receives x = [xa, xb].
+template <int D, class Sink>
+__device__ __forceinline__ void deriv_backward_dual(const DOp& oa, const DOp& ob, const cplx* ysa,
+                                                    const cplx* ysb, long py, Sink& sink) {
+  const int N = oa.N;
+  const int mtop = N - 7;
+  BwdState<1> sa, sb;
+  auto border = [&](const DOp& o, const cplx* ys, BwdState<1>& s) {
+    const cplx cg2 = cconj(o.lg2), cf3 = cconj(o.lf2), cg3 = cconj(o.lg1);
+    const cplx yN1 = gld(ys + (long)(N - 1) * py);
+    const cplx yN2 = gld(ys + (long)(N - 2) * py);
+    const cplx yN3 = gld(ys + (long)(N - 3) * py);
+    s.xN1[0] = crmul(o.dm2, yN1);
+    s.xN2[0] = csub(crmul(o.dm1, yN2), cmul(cg2, s.xN1[0]));
+    s.xN3[0] = csub(csub(crmul(o.dm0, yN3), cmul(cf3, s.xN2[0])), cmul(cg3, s.xN1[0]));
+    s.x1[0] = s.xN3[0];
+    s.x2[0] = s.xN2[0];
+    s.x3[0] = s.xN1[0];
   };
-  if (lane == 0) {
-    const cplx e0 = rec(Ler(N - 9), Ler(N - 8), Ler(N - 7), N - 6, a3);
-    const cplx e1 = rec(Ler(N - 8), Ler(N - 7), e0, N - 5, a2);
-    const cplx e2 = rec(Ler(N - 7), e0, e1, N - 4, a1);
-    sLe[0] = e0; sLe[1] = e1; sLe[2] = e2;
-    const cplx f0 = rec(Lfr(N - 8), Lfr(N - 7), Lfr(N - 6), N - 5, a3);
-    const cplx f1 = rec(Lfr(N - 7), Lfr(N - 6), f0, N - 4, a2);
-    sLf[0] = f0; sLf[1] = f1;
-    sLg[0] = rec(Lgr(N - 7), Lgr(N - 6), Lgr(N - 5), N - 4, a3);
+  cplx pa[D], pb[D];
+  typename Sink::In pin_[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    const int r = mtop - q;
+    pa[q] = gld(ysa + (long)r * py);
+    pb[q] = gld(ysb + (long)r * py);
+    pin_[q] = sink.load(r);
   }
-  __syncthreads();
-  auto Le = [&](int k) { return k < N - 6 ? Ler(k) : sel3(sLe, k - (N - 6)); };
-  auto Lf = [&](int k) { return k < N - 5 ? Lfr(k) : sel3(sLf, k - (N - 5)); };
-  auto Lg = [&](int k) { return k < N - 4 ? Lgr(k) : sel3(sLg, k - (N - 4)); };
-  // D(N-3) = a0 - sum_{k<=N-4} |Le|^2 D
-  double s_ee = 0.0;
-  cplx s_fe = mkc(0, 0), s_ge = mkc(0, 0);
-  for (int k = lane; k <= N - 4; k += 64) {
-    const cplx le = Le(k), lf = Lf(k), lg = Lg(k);
-    s_ee += cabs2(le) * b.D[k];
-    s_fe = cadd(s_fe, crmul(b.D[k], cmulc(lf, le)));
-    s_ge = cadd(s_ge, crmul(b.D[k], cmulc(lg, le)));
+  {
+    const typename Sink::In t1 = sink.load(N - 1), t2 = sink.load(N - 2), t3 = sink.load(N - 3);
+    border(oa, ysa, sa);
+    border(ob, ysb, sb);
+    cplx x2[2];
+    x2[0] = sa.xN1[0]; x2[1] = sb.xN1[0];
+    sink.apply(N - 1, t1, x2);
+    x2[0] = sa.xN2[0]; x2[1] = sb.xN2[0];
+    sink.apply(N - 2, t2, x2);
+    x2[0] = sa.xN3[0]; x2[1] = sb.xN3[0];
+    sink.apply(N - 3, t3, x2);
   }
-  s_ee = wave_sum(s_ee);
-  s_fe.x = wave_sum(s_fe.x); s_fe.y = wave_sum(s_fe.y);
-  s_ge.x = wave_sum(s_ge.x); s_ge.y = wave_sum(s_ge.y);
-  const double DN3 = a0 - s_ee;
-  const double dN3 = 1.0 / DN3;
-  const cplx lfN3 = crmul(dN3, csub(mkc(a1, 0.0), s_fe));
-  const cplx lgN3 = crmul(dN3, csub(mkc(a2, 0.0), s_ge));
-  if (lane == 0) { sLf[2] = lfN3; sLg[1] = lgN3; }
-  __syncthreads();
-  double s_ff = 0.0;
-  cplx s_gf = mkc(0, 0);
-  for (int k = lane; k <= N - 3; k += 64) {
-    const double Dk = (k == N - 3) ? DN3 : b.D[k];
-    const cplx lf = Lf(k), lg = Lg(k);
-    s_ff += cabs2(lf) * Dk;
-    s_gf = cadd(s_gf, crmul(Dk, cmulc(lg, lf)));
+  // one row of both sweeps with explicit border coefficients
+  auto row = [&](int i, const cplx& ya, const cplx& yb, const typename Sink::In& in,
+                 cplx cea, cplx cfa, cplx cga, cplx ceb, cplx cfb, cplx cgb) {
+    cplx xo[2];
+    xo[0] = bwd_row(ya, cld(oa.b.Dm1 + i), cld(oa.b.La + i + 1), cld(oa.b.Lb + i + 2),
+                    cld(oa.b.Lc + i + 3), cea, cfa, cga, sa.xN3[0], sa.xN2[0], sa.xN1[0],
+                    sa.x1[0], sa.x2[0], sa.x3[0]);
+    xo[1] = bwd_row(yb, cld(ob.b.Dm1 + i), cld(ob.b.La + i + 1), cld(ob.b.Lb + i + 2),
+                    cld(ob.b.Lc + i + 3), ceb, cfb, cgb, sb.xN3[0], sb.xN2[0], sb.xN1[0],
+                    sb.x1[0], sb.x2[0], sb.x3[0]);
+    sink.apply(i, in, xo);
+  };
+  {
+    int i = N - 4;
+    row(i, gld(ysa + (long)i * py), gld(ysb + (long)i * py), sink.load(i), cconj(oa.le2),
+        cconj(oa.lf1), cconj(oa.lg0), cconj(ob.le2), cconj(ob.lf1), cconj(ob.lg0));
+    i = N - 5;
+    row(i, gld(ysa + (long)i * py), gld(ysb + (long)i * py), sink.load(i), cconj(oa.le1),
+        cconj(oa.lf0), cconj(dop_g(oa, i)), cconj(ob.le1), cconj(ob.lf0), cconj(dop_g(ob, i)));
+    i = N - 6;
+    row(i, gld(ysa + (long)i * py), gld(ysb + (long)i * py), sink.load(i), cconj(oa.le0),
+        cconj(dop_f(oa, i)), cconj(dop_g(oa, i)), cconj(ob.le0), cconj(dop_f(ob, i)),
+        cconj(dop_g(ob, i)));
   }
-  s_ff = wave_sum(s_ff);
-  s_gf.x = wave_sum(s_gf.x); s_gf.y = wave_sum(s_gf.y);
-  const double DN2 = a0 - s_ff;
-  const double dN2 = 1.0 / DN2;
-  const cplx lgN2 = crmul(dN2, csub(mkc(a1, 0.0), s_gf));
-  if (lane == 0) sLg[2] = lgN2;
-  __syncthreads();
-  double s_gg = 0.0;
-  for (int k = lane; k <= N - 2; k += 64) {
-    const double Dk = (k == N - 3) ? DN3 : ((k == N - 2) ? DN2 : b.D[k]);
-    s_gg += cabs2(Lg(k)) * Dk;
-  }
-  s_gg = wave_sum(s_gg);
-  const double DN1 = a0 - s_gg;
-  if (lane == 0) {
-    TwistTail tt;
-    for (int q = 0; q < 3; ++q) { tt.Le[q] = sLe[q]; tt.Lf[q] = sLf[q]; tt.Lg[q] = sLg[q]; }
-    tt.Dm1[0] = dN3;
-    tt.Dm1[1] = dN2;
-    tt.Dm1[2] = 1.0 / DN1;
-    out[op * nk + kx] = tt;
+  for (int ib = mtop; ib >= 0; ib -= D) {
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const int i = ib - q;
+      if (i >= 0) {
+        const typename Sink::In in = pin_[q];
+        const cplx ya = pa[q], yb = pb[q];
+        const int rn = i - D;
+        if (rn >= 0) {
+          pa[q] = gld(ysa + (long)rn * py);
+          pb[q] = gld(ysb + (long)rn * py);
+          pin_[q] = sink.load(rn);
+        }
+        row(i, ya, yb, in, cconj(dop_e(oa, i)), cconj(dop_f(oa, i)), cconj(dop_g(oa, i)),
+            cconj(dop_e(ob, i)), cconj(dop_f(ob, i)), cconj(dop_g(ob, i)));
+      }
+    }
   }
 }
 
@@ -130,28 +160,16 @@ struct YArgs {
   const TwistTail* tw_old;                  // [2][kxn] D1, D2 at the current twist
   const TwistTail* tw_new;                  // [2][kxn] after the shift update
   const cplx *shp_old, *shm_old, *shp_new, *shm_new;
-  cplx *vor, *phi, *hg, *hv, *dvordy, *vorwk, *phiwk;
-  cplx *s1, *s2;                            // forward-sweep scratch, stride PS
+  cplx *vor, *phi, *vorwk, *phiwk;          // state and RK accumulators
+  cplx *v, *dome, *dv;                      // v, d(omega_y)/dy, dv/dy for the x-z stage
+  cplx *rdome, *rhg, *rhv;                  // x-z stage outputs (cross.f90:1533-1544)
+  cplx *s1, *s2, *s3;                       // forward-sweep scratch, stride PS
   HScratch hs;                              // Helmholtz factor scratch, stride PS
   cplx *u00c, *w00c, *u00wk, *w00wk;        // 00 lines (real data, imag 0), stride 1
   const double *rf0u, *rf0w;
   double *dudy, *dwdy;
   DevScalars* sc;
   long PS;
-};
-
-struct AddSubSink {   // out[j] = base[j] - x  (hv = hv - d(dvordy)/dy)
-  cplx* out;
-  long P;
-  __device__ void operator()(int j, const cplx* x) {
-    const long o = (long)j * P;
-    out[o] = csub(out[o], x[0]);
-  }
-};
-
-struct RealOutSink {  // dd[j] = Re x
-  double* dd;
-  __device__ void operator()(int j, const cplx* x) { dd[j] = x[0].x; }
 };
 
 // lap matrix rh2 - k2 pr2 (lapcdy, diffy_shear_hepta.f90:475)
@@ -169,42 +187,47 @@ __device__ __forceinline__ void line_geom(const Geom& g, long L, int& i, int& k,
   k2 = kx * kx + kz * kz;
 }
 
-// D1 of vor and v with the given twist -> dvordy, hv (the derivyc calls at
-// the top of each substep, cross.f90:434-445)
-__device__ __forceinline__ void d1_pair(const YArgs& a, long L, int i, const TwistTail* tw,
-                                        const cplx* shp, const cplx* shm) {
-  const long P = a.g.P;
-  DOp o{a.b1, tw[i], shp[i], a.g.my};
-  const cplx* in[2] = {a.vor + L, a.hg + L};
-  cplx* ys[2] = {a.s1 + L, a.s2 + L};
-  deriv_forward<2, true>(o, a.c.rh1, in, P, ys, a.PS, shm[i]);
-  cplx* outp[2] = {a.dvordy + L, a.hv + L};
-  StoreSink<2> sk{outp, P};
-  deriv_backward<2>(o, ys, a.PS, sk);
+template <int D>
+__device__ __forceinline__ void d1_one(const YArgs& a, const DOp& o, cplx shm, const cplx* in,
+                                       long pin, cplx* ys, cplx* out, long pout) {
+  const cplx* ins[1] = {in};
+  cplx* yy[1] = {ys};
+  deriv_forward<1, true, D>(o, a.c.rh1, ins, pin, yy, a.PS, shm);
+  cplx* outs[1] = {out};
+  StoreSink<1> sk{outs, pout};
+  deriv_backward<1, D>(o, yy, a.PS, sk);
 }
 
 __device__ __forceinline__ void helm_line(const YArgs& a, long L, double k2, cplx shp, cplx shm,
-                                          const cplx* in, long pin, cplx* out, long pout) {
+                                          const cplx* in, long pin, cplx* ys, cplx* out,
+                                          long pout) {
   double lm[7];
   lapmat(a.c, k2, lm);
   HScratch hs{a.hs.hr + L, a.hs.hc + L};
   HTail tail;
   const cplx* ins[1] = {in};
-  cplx* ys[1] = {a.s1 + L};
-  helm_forward<1>(lm, a.c.pr2, a.g.my, shp, shm, ins, pin, ys, a.PS, hs, tail);
+  cplx* yy[1] = {ys};
+  helm_forward<1, 8>(lm, a.c.pr2, a.g.my, shp, shm, ins, pin, yy, a.PS, hs, tail);
   cplx* outs[1] = {out};
   StoreSink<1> sk{outs, pout};
-  helm_backward<1>(a.g.my, ys, a.PS, hs, tail, sk);
+  helm_backward<1, 4>(a.g.my, yy, a.PS, hs, tail, sk);
 }
 
-// D1 of a 00 line (real periodic) -> dd
+struct RealOutSink {  // dd[j] = Re x
+  typedef NoIn In;
+  double* dd;
+  __device__ In load(int) const { return In(); }
+  __device__ void apply(int j, const In&, const cplx* x) { gstd(dd + j, x[0].x); }
+};
+
+// D1 of a 00 line (real periodic, derivyc m=1) -> dd
 __device__ __forceinline__ void d1_real(const YArgs& a, long L, const cplx* u, double* dd) {
-  DOp o{a.b1, a.id1, mkc(1.0, 0.0), a.g.my};
+  const DOp o = make_dop(a.b1, a.id1, mkc(1.0, 0.0), a.g.my);
   const cplx* in[1] = {u};
   cplx* ys[1] = {a.s1 + L};
-  deriv_forward<1, true>(o, a.c.rh1, in, 1, ys, a.PS, mkc(1.0, 0.0));
+  deriv_forward<1, true, 4>(o, a.c.rh1, in, 1, ys, a.PS, mkc(1.0, 0.0));
   RealOutSink sk{dd};
-  deriv_backward<1>(o, ys, a.PS, sk);
+  deriv_backward<1, 4>(o, ys, a.PS, sk);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,8 +244,10 @@ k_y_first(YArgs a) {
     int i, k;
     double kx, k2;
     line_geom(a.g, L, i, k, kx, k2);
-    if (L != 0) helm_line(a, L, k2, a.shp_old[i], a.shm_old[i], a.phi + L, P, a.hg + L, P);
-    d1_pair(a, L, i, a.tw_old, a.shp_old, a.shm_old);
+    if (L != 0) helm_line(a, L, k2, gld(a.shp_old + i), gld(a.shm_old + i), a.phi + L, P, a.s2 + L, a.v + L, P);
+    const DOp o = make_dop(a.b1, a.tw_old + i, gld(a.shp_old + i), a.g.my);
+    d1_one<8>(a, o, gld(a.shm_old + i), a.vor + L, P, a.s1 + L, a.dome + L, P);
+    d1_one<8>(a, o, gld(a.shm_old + i), a.v + L, P, a.s2 + L, a.dv + L, P);
   } else if (L == P) {
     d1_real(a, L, a.u00c, a.dudy);
   } else {
@@ -231,66 +256,15 @@ k_y_first(YArgs a) {
 }
 
 // dU/dy and dW/dy only (top of hvhg, cross.f90:1198-1205)
-__global__ void k_d1_00(YArgs a) {
+__global__ void __launch_bounds__(64)
+k_d1_00(YArgs a) {
   if (threadIdx.x == 0) d1_real(a, a.g.P, a.u00c, a.dudy);
   if (threadIdx.x == 1) d1_real(a, a.g.P + 1, a.w00c, a.dwdy);
 }
 
 // ---------------------------------------------------------------------------
-// explicit substep (cross.f90:565-796), fused per line
+// explicit substep (cross.f90:565-796)
 // ---------------------------------------------------------------------------
-struct ExplicitRKSink {
-  cplx *vor, *phi, *hg, *hv, *vorwk, *phiwk;
-  long P;
-  double fac, k2, kx, dt, gam, zet, f1, f2, dy, Ly;
-  int rk, is00;
-  __device__ void operator()(int j, const cplx* x) {
-    const long o = (long)j * P;
-    cplx hgj = hg[o], hvj = hv[o];
-    const cplx vj = vor[o], pj = phi[o];
-    if (!is00) {
-      hgj = cadd(hgj, crmul(fac, cadd(crmul(-k2, vj), x[0])));
-      hvj = cadd(hvj, crmul(fac, cadd(crmul(-k2, pj), x[1])));
-    }
-    cplx vwk, pwk;
-    if (rk == 1) {
-      vwk = crmul(gam, hgj);
-      pwk = crmul(gam, hvj);
-    } else {
-      vwk = cadd(vorwk[o], crmul(gam, hgj));
-      pwk = cadd(phiwk[o], crmul(gam, hvj));
-    }
-    cplx vn = cadd(vj, crmul(dt, vwk));
-    cplx pn = cadd(pj, crmul(dt, pwk));
-    const double yj = (double)j * dy - 0.5 * Ly;
-    if (rk != 3) {
-      const cplx m1 = mframe(kx, yj, f1);
-      vwk = cmul(crmul(zet, hgj), m1);
-      pwk = cmul(crmul(zet, hvj), m1);
-    }
-    const cplx m2 = mframe(kx, yj, f2);
-    vor[o] = cmul(vn, m2);
-    phi[o] = cmul(pn, m2);
-    vorwk[o] = vwk;
-    phiwk[o] = pwk;
-  }
-};
-
-struct Explicit00Sink {
-  cplx *u, *wk;
-  const double* rf;
-  double fac, dt, gam, zet;
-  int rk;
-  __device__ void operator()(int j, const cplx* x) {
-    const double rfj = rf[j] + fac * (-0.0 * u[j].x + x[0].x);
-    double wkj = (rk == 1) ? gam * rfj : wk[j].x + gam * rfj;
-    const double un = u[j].x + dt * wkj;
-    if (rk != 3) wkj = rfj * zet;
-    u[j] = mkc(un, 0.0);
-    wk[j] = mkc(wkj, 0.0);
-  }
-};
-
 __device__ __forceinline__ void substep_factors(const DevScalars* sc, double s, int rk,
                                                 double& dt, double& f1, double& f2) {
   dt = sc->Deltat;
@@ -300,105 +274,198 @@ __device__ __forceinline__ void substep_factors(const DevScalars* sc, double s, 
   f2 = (rk == 1) ? -(s * r1) : -(s * (c_ca[ir] - c_ca[ir - 1]) * dt);
 }
 
+struct C3 {
+  cplx a, b, c;
+};
+
+// vor chain: hg + nu (d2 vor - k2 vor) (visc3d), RK accumulation, update,
+// zeta*hg with the phase map, un-mapping of vor (cross.f90:652-781)
+struct VorSink {
+  typedef C3 In;   // rhg, vor, vorwk
+  const cplx *rhg;
+  cplx *vor, *vorwk;
+  long P;
+  double fac, k2, kx, dt, gam, zet, f1, f2, dy, Ly;
+  int rk, is00;
+  __device__ In load(int j) const {
+    const long o = (long)j * P;
+    return In{gld(rhg + o), gld(vor + o), rk == 1 ? mkc(0, 0) : gld(vorwk + o)};
+  }
+  __device__ void apply(int j, const In& in, const cplx* x) {
+    const long o = (long)j * P;
+    cplx hgj = in.a;
+    if (!is00) hgj = cadd(hgj, crmul(fac, cadd(crmul(-k2, in.b), x[0])));
+    cplx vwk = (rk == 1) ? crmul(gam, hgj) : cadd(in.c, crmul(gam, hgj));
+    const cplx vn = cadd(in.b, crmul(dt, vwk));
+    const double yj = (double)j * dy - 0.5 * Ly;
+    if (rk != 3) vwk = cmul(crmul(zet, hgj), mframe(kx, yj, f1));
+    gst(vor + o, cmul(vn, mframe(kx, yj, f2)));
+    gst(vorwk + o, vwk);
+  }
+};
+
+// phi chain: hv - d(rdome)/dy, + nu (d2 phi - k2 phi), RK update, maps
+// (cross.f90:607, 652-781)
+struct PhiSink {
+  typedef C3 In;   // rhv, phi, phiwk
+  const cplx* rhv;
+  cplx *phi, *phiwk;
+  long P;
+  double fac, k2, kx, dt, gam, zet, f1, f2, dy, Ly;
+  int rk, is00;
+  __device__ In load(int j) const {
+    const long o = (long)j * P;
+    return In{gld(rhv + o), gld(phi + o), rk == 1 ? mkc(0, 0) : gld(phiwk + o)};
+  }
+  __device__ void apply(int j, const In& in, const cplx* x) {
+    const long o = (long)j * P;
+    cplx hvj = csub(in.a, x[0]);
+    if (!is00) hvj = cadd(hvj, crmul(fac, cadd(crmul(-k2, in.b), x[1])));
+    cplx pwk = (rk == 1) ? crmul(gam, hvj) : cadd(in.c, crmul(gam, hvj));
+    const cplx pn = cadd(in.b, crmul(dt, pwk));
+    const double yj = (double)j * dy - 0.5 * Ly;
+    if (rk != 3) pwk = cmul(crmul(zet, hvj), mframe(kx, yj, f1));
+    gst(phi + o, cmul(pn, mframe(kx, yj, f2)));
+    gst(phiwk + o, pwk);
+  }
+};
+
+struct R2 {
+  double u, wk;
+};
+
+// 00 lines: rf + nu d2u (visc3d m=1, rk=0), RK update of U or W
+struct Explicit00Sink {
+  typedef R2 In;
+  cplx *u, *wk;
+  const double* rf;
+  double fac, dt, gam, zet;
+  int rk;
+  __device__ In load(int j) const { return In{gld(u + j).x, rk == 1 ? 0.0 : gld(wk + j).x}; }
+  __device__ void apply(int j, const In& in, const cplx* x) {
+    const double rfj = gldd(rf + j) + fac * (-0.0 * in.u + x[0].x);
+    double wkj = (rk == 1) ? gam * rfj : in.wk + gam * rfj;
+    const double un = in.u + dt * wkj;
+    if (rk != 3) wkj = rfj * zet;
+    gst(u + j, mkc(un, 0.0));
+    gst(wk + j, mkc(wkj, 0.0));
+  }
+};
+
 __global__ void __launch_bounds__(128)
 k_y_explicit(YArgs a, int rk) {
-  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long P = a.g.P;
-  if (L >= P + 2) return;
+  if (t >= 2 * P + 2) return;
   if (a.sc->halted) return;
   const int ir = rk - 1;
   double dt, f1, f2;
   substep_factors(a.sc, a.g.s, rk, dt, f1, f2);
   const double fac = 1.0 / a.g.Re;
-  if (L < P) {
+  const int N = a.g.my;
+  if (t < 2 * P) {
+    const bool vchain = t < P;
+    const long L = vchain ? t : t - P;
     int i, k;
     double kx, k2;
     line_geom(a.g, L, i, k, kx, k2);
-    const TwistTail* tw1o = a.tw_old;
-    const TwistTail* tw2o = a.tw_old + a.g.kxn;
-    // (1) dvordy <- d/dy dvordy ; hv <- hv - dvordy        (cross.f90:565-607)
-    {
-      DOp o{a.b1, tw1o[i], a.shp_old[i], a.g.my};
-      const cplx* in[1] = {a.dvordy + L};
+    const DOp o1 = make_dop(a.b1, a.tw_old + i, gld(a.shp_old + i), N);
+    const DOp o2 = make_dop(a.b2, a.tw_old + a.g.kxn + i, gld(a.shp_old + i), N);
+    const DOp o1n = make_dop(a.b1, a.tw_new + i, gld(a.shp_new + i), N);
+    if (vchain) {
+      const cplx* in[1] = {a.vor + L};
       cplx* ys[1] = {a.s1 + L};
-      deriv_forward<1, true>(o, a.c.rh1, in, P, ys, a.PS, a.shm_old[i]);
-      AddSubSink sk{a.hv + L, P};
-      deriv_backward<1>(o, ys, a.PS, sk);
+      deriv_forward<1, false, 8>(o2, a.c.rh2, in, P, ys, a.PS, gld(a.shm_old + i));
+      VorSink sk{a.rhg + L, a.vor + L, a.vorwk + L, P, fac, k2, kx, dt, c_gama[ir], c_zeta[ir],
+                 f1, f2, a.g.dy, a.g.Ly, rk, L == 0};
+      deriv_backward<1, 4>(o2, ys, a.PS, sk);
+      d1_one<8>(a, o1n, gld(a.shm_new + i), a.vor + L, P, a.s1 + L, a.dome + L, P);
+    } else {
+      deriv_forward_dual<true, false, 8>(o1, a.c.rh1, o2, a.c.rh2, a.rdome + L, a.phi + L, P,
+                                         a.s2 + L, a.s3 + L, a.PS, gld(a.shm_old + i));
+      PhiSink sk{a.rhv + L, a.phi + L, a.phiwk + L, P, fac, k2, kx, dt, c_gama[ir], c_zeta[ir],
+                 f1, f2, a.g.dy, a.g.Ly, rk, L == 0};
+      deriv_backward_dual<4>(o1, o2, a.s2 + L, a.s3 + L, a.PS, sk);
+      if (L != 0)
+        helm_line(a, L, k2, gld(a.shp_new + i), gld(a.shm_new + i), a.phi + L, P, a.s2 + L, a.v + L, P);
+      d1_one<8>(a, o1n, gld(a.shm_new + i), a.v + L, P, a.s2 + L, a.dv + L, P);
     }
-    // (2)-(6) viscous terms, RK update, phase maps          (cross.f90:633-781)
-    {
-      DOp o{a.b2, tw2o[i], a.shp_old[i], a.g.my};
-      const cplx* in[2] = {a.vor + L, a.phi + L};
-      cplx* ys[2] = {a.s1 + L, a.s2 + L};
-      deriv_forward<2, false>(o, a.c.rh2, in, P, ys, a.PS, a.shm_old[i]);
-      ExplicitRKSink sk{a.vor + L, a.phi + L, a.hg + L, a.hv + L, a.vorwk + L, a.phiwk + L, P,
-                        fac, k2, kx, dt, c_gama[ir], c_zeta[ir], f1, f2, a.g.dy, a.g.Ly, rk,
-                        L == 0};
-      deriv_backward<2>(o, ys, a.PS, sk);
-    }
-    // (7) v = lapcdy(phi) with the updated twist            (cross.f90:783-795)
-    if (L != 0) helm_line(a, L, k2, a.shp_new[i], a.shm_new[i], a.phi + L, P, a.hg + L, P);
-    // next substep's D1(vor), D1(v)                          (cross.f90:434-445)
-    d1_pair(a, L, i, a.tw_new, a.shp_new, a.shm_new);
   } else {
+    const long L = P + (t - 2 * P);
     cplx* u = (L == P) ? a.u00c : a.w00c;
     cplx* wk = (L == P) ? a.u00wk : a.w00wk;
     const double* rf = (L == P) ? a.rf0u : a.rf0w;
     double* dd = (L == P) ? a.dudy : a.dwdy;
-    DOp o{a.b2, a.id2, mkc(1.0, 0.0), a.g.my};
+    const DOp o = make_dop(a.b2, a.id2, mkc(1.0, 0.0), N);
     const cplx* in[1] = {u};
     cplx* ys[1] = {a.s1 + L};
-    deriv_forward<1, false>(o, a.c.rh2, in, 1, ys, a.PS, mkc(1.0, 0.0));
+    deriv_forward<1, false, 4>(o, a.c.rh2, in, 1, ys, a.PS, mkc(1.0, 0.0));
     Explicit00Sink sk{u, wk, rf, fac, dt, c_gama[ir], c_zeta[ir], rk};
-    deriv_backward<1>(o, ys, a.PS, sk);
+    deriv_backward<1, 4>(o, ys, a.PS, sk);
     d1_real(a, L, u, dd);
   }
 }
 
 // ---------------------------------------------------------------------------
-// implicit substep (cross.f90:797-990)
+// implicit substep (cross.f90:797-990), one thread per line
 // ---------------------------------------------------------------------------
+struct SubSink {   // rhv = rhv - x  (hv = hv - dvordy)
+  typedef cplx In;
+  cplx* out;
+  long P;
+  __device__ In load(int j) const { return gld(out + (long)j * P); }
+  __device__ void apply(int j, const In& in, const cplx* x) { gst(out + (long)j * P, csub(in, x[0])); }
+};
+
+struct C6 {
+  cplx hg, hv, vor, phi, vwk, pwk;
+};
+
 struct ImplicitRKSink {
-  cplx *vor, *phi, *hg, *hv, *vorwk, *phiwk;
+  typedef C6 In;
+  const cplx *rhg, *rhv;
+  cplx *vor, *phi, *vorwk, *phiwk;
   long P;
   double fac, facb, rdt, k2, kx, gam, f2, dy, Ly;
   int rk, is00;
-  __device__ void operator()(int j, const cplx* x) {
+  __device__ In load(int j) const {
     const long o = (long)j * P;
-    const cplx hgj = hg[o], hvj = hv[o], vj = vor[o], pj = phi[o];
-    cplx vwk, pwk;
-    if (rk == 1) {
-      vwk = crmul(gam, hgj);
-      pwk = crmul(gam, hvj);
-    } else {
-      vwk = cadd(vorwk[o], crmul(gam, hgj));
-      pwk = cadd(phiwk[o], crmul(gam, hvj));
-    }
+    const cplx z = mkc(0, 0);
+    return In{gld(rhg + o), gld(rhv + o), gld(vor + o), gld(phi + o), rk == 1 ? z : gld(vorwk + o),
+              rk == 1 ? z : gld(phiwk + o)};
+  }
+  __device__ void apply(int j, const In& in, const cplx* x) {
+    const long o = (long)j * P;
+    cplx vwk = (rk == 1) ? crmul(gam, in.hg) : cadd(in.vwk, crmul(gam, in.hg));
+    cplx pwk = (rk == 1) ? crmul(gam, in.hv) : cadd(in.pwk, crmul(gam, in.hv));
     if (!is00) {
-      vwk = cadd(vwk, crmul(fac, cadd(crmul(-k2, vj), x[0])));
-      pwk = cadd(pwk, crmul(fac, cadd(crmul(-k2, pj), x[1])));
+      vwk = cadd(vwk, crmul(fac, cadd(crmul(-k2, in.vor), x[0])));
+      pwk = cadd(pwk, crmul(fac, cadd(crmul(-k2, in.phi), x[1])));
     }
-    vwk = cadd(vwk, crmul(rdt, vj));
-    pwk = cadd(pwk, crmul(rdt, pj));
+    vwk = cadd(vwk, crmul(rdt, in.vor));
+    pwk = cadd(pwk, crmul(rdt, in.phi));
     vwk = crmul(facb, vwk);
     pwk = crmul(facb, pwk);
     const double yj = (double)j * dy - 0.5 * Ly;
     const cplx m2 = mframe(kx, yj, f2);
-    vorwk[o] = cmul(vwk, m2);
-    phiwk[o] = cmul(pwk, m2);
+    gst(vorwk + o, cmul(vwk, m2));
+    gst(phiwk + o, cmul(pwk, m2));
   }
 };
 
 struct Implicit00Sink {
+  typedef R2 In;
   cplx *u, *wk;
   const double* rf;
   double fac, facb, rdt, gam;
   int rk;
-  __device__ void operator()(int j, const cplx* x) {
-    double wkj = (rk == 1) ? gam * rf[j] : wk[j].x + gam * rf[j];
-    wkj = wkj + fac * (-0.0 * u[j].x + x[0].x);
-    wkj = facb * (wkj + rdt * u[j].x);
-    wk[j] = mkc(wkj, 0.0);
+  __device__ In load(int j) const { return In{gld(u + j).x, rk == 1 ? 0.0 : gld(wk + j).x}; }
+  __device__ void apply(int j, const In& in, const cplx* x) {
+    const double rfj = gldd(rf + j);
+    double wkj = (rk == 1) ? gam * rfj : in.wk + gam * rfj;
+    wkj = wkj + fac * (-0.0 * in.u + x[0].x);
+    wkj = facb * (wkj + rdt * in.u);
+    gst(wk + j, mkc(wkj, 0.0));
   }
 };
 
@@ -415,28 +482,27 @@ k_y_implicit(YArgs a, int rk) {
   const double facb = -a.g.Re * c_ibeta[ir];
   const double rdt = 1.0 / dt;
   const double dtr1 = a.sc->dtr * c_ibeta[ir];
+  const int N = a.g.my;
   if (L < P) {
     int i, k;
     double kx, k2;
     line_geom(a.g, L, i, k, kx, k2);
-    const TwistTail* tw1o = a.tw_old;
-    const TwistTail* tw2o = a.tw_old + a.g.kxn;
     {
-      DOp o{a.b1, tw1o[i], a.shp_old[i], a.g.my};
-      const cplx* in[1] = {a.dvordy + L};
+      const DOp o = make_dop(a.b1, a.tw_old + i, gld(a.shp_old + i), N);
+      const cplx* in[1] = {a.rdome + L};
       cplx* ys[1] = {a.s1 + L};
-      deriv_forward<1, true>(o, a.c.rh1, in, P, ys, a.PS, a.shm_old[i]);
-      AddSubSink sk{a.hv + L, P};
-      deriv_backward<1>(o, ys, a.PS, sk);
+      deriv_forward<1, true, 4>(o, a.c.rh1, in, P, ys, a.PS, gld(a.shm_old + i));
+      SubSink sk{a.rhv + L, P};
+      deriv_backward<1, 4>(o, ys, a.PS, sk);
     }
     {
-      DOp o{a.b2, tw2o[i], a.shp_old[i], a.g.my};
+      const DOp o = make_dop(a.b2, a.tw_old + a.g.kxn + i, gld(a.shp_old + i), N);
       const cplx* in[2] = {a.vor + L, a.phi + L};
       cplx* ys[2] = {a.s1 + L, a.s2 + L};
-      deriv_forward<2, false>(o, a.c.rh2, in, P, ys, a.PS, a.shm_old[i]);
-      ImplicitRKSink sk{a.vor + L, a.phi + L, a.hg + L, a.hv + L, a.vorwk + L, a.phiwk + L,
+      deriv_forward<2, false, 4>(o, a.c.rh2, in, P, ys, a.PS, gld(a.shm_old + i));
+      ImplicitRKSink sk{a.rhg + L, a.rhv + L, a.vor + L, a.phi + L, a.vorwk + L, a.phiwk + L,
                         P, fac, facb, rdt, k2, kx, c_gama[ir], f2, a.g.dy, a.g.Ly, rk, L == 0};
-      deriv_backward<2>(o, ys, a.PS, sk);
+      deriv_backward<2, 2>(o, ys, a.PS, sk);
     }
     // (8) Helmholtz solves for vor and phi with k2 + dtr1      (cross.f90:898-928)
     if (L != 0) {
@@ -446,267 +512,80 @@ k_y_implicit(YArgs a, int rk) {
       HTail tail;
       const cplx* in[2] = {a.vorwk + L, a.phiwk + L};
       cplx* ys[2] = {a.s1 + L, a.s2 + L};
-      helm_forward<2>(lm, a.c.pr2, a.g.my, a.shp_new[i], a.shm_new[i], in, P, ys, a.PS, hs, tail);
+      helm_forward<2, 4>(lm, a.c.pr2, N, gld(a.shp_new + i), gld(a.shm_new + i), in, P, ys, a.PS, hs, tail);
       cplx* outs[2] = {a.vor + L, a.phi + L};
       StoreSink<2> sk{outs, P};
-      helm_backward<2>(a.g.my, ys, a.PS, hs, tail, sk);
+      helm_backward<2, 4>(N, ys, a.PS, hs, tail, sk);
     }
     // (9) zeta * [hg, hv] -> [vorwk, phiwk] with the phase map (cross.f90:931-944)
     if (rk != 3) {
       const double zet = c_zeta[ir];
-      for (int j = 0; j < a.g.my; ++j) {
+      for (int j = 0; j < N; ++j) {
         const long o = (long)j * P + L;
         const double yj = (double)j * a.g.dy - 0.5 * a.g.Ly;
         const cplx m1 = mframe(kx, yj, f1);
-        a.vorwk[o] = cmul(crmul(zet, a.hg[o]), m1);
-        a.phiwk[o] = cmul(crmul(zet, a.hv[o]), m1);
+        gst(a.vorwk + o, cmul(crmul(zet, gld(a.rhg + o)), m1));
+        gst(a.phiwk + o, cmul(crmul(zet, gld(a.rhv + o)), m1));
       }
     }
-    if (L != 0) helm_line(a, L, k2, a.shp_new[i], a.shm_new[i], a.phi + L, P, a.hg + L, P);
-    d1_pair(a, L, i, a.tw_new, a.shp_new, a.shm_new);
+    if (L != 0) helm_line(a, L, k2, gld(a.shp_new + i), gld(a.shm_new + i), a.phi + L, P, a.s1 + L, a.v + L, P);
+    const DOp o1n = make_dop(a.b1, a.tw_new + i, gld(a.shp_new + i), N);
+    d1_one<8>(a, o1n, gld(a.shm_new + i), a.vor + L, P, a.s1 + L, a.dome + L, P);
+    d1_one<8>(a, o1n, gld(a.shm_new + i), a.v + L, P, a.s2 + L, a.dv + L, P);
   } else {
     cplx* u = (L == P) ? a.u00c : a.w00c;
     cplx* wk = (L == P) ? a.u00wk : a.w00wk;
     const double* rf = (L == P) ? a.rf0u : a.rf0w;
     double* dd = (L == P) ? a.dudy : a.dwdy;
     {
-      DOp o{a.b2, a.id2, mkc(1.0, 0.0), a.g.my};
+      const DOp o = make_dop(a.b2, a.id2, mkc(1.0, 0.0), N);
       const cplx* in[1] = {u};
       cplx* ys[1] = {a.s1 + L};
-      deriv_forward<1, false>(o, a.c.rh2, in, 1, ys, a.PS, mkc(1.0, 0.0));
+      deriv_forward<1, false, 4>(o, a.c.rh2, in, 1, ys, a.PS, mkc(1.0, 0.0));
       Implicit00Sink sk{u, wk, rf, fac, facb, rdt, c_gama[ir], rk};
-      deriv_backward<1>(o, ys, a.PS, sk);
+      deriv_backward<1, 4>(o, ys, a.PS, sk);
     }
-    helm_line(a, L, dtr1, mkc(1.0, 0.0), mkc(1.0, 0.0), wk, 1, u, 1);
+    helm_line(a, L, dtr1, mkc(1.0, 0.0), mkc(1.0, 0.0), wk, 1, a.s1 + L, u, 1);
     if (rk != 3)
-      for (int j = 0; j < a.g.my; ++j) wk[j] = mkc(c_zeta[ir] * rf[j], 0.0);
+      for (int j = 0; j < N; ++j) gst(wk + j, mkc(c_zeta[ir] * gldd(rf + j), 0.0));
     d1_real(a, L, u, dd);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// scalar kernel: statistics accumulation, CFL time step, UNER record,
-// shear-offset update (cross.f90:1598-1712, 2067-2119, 726-746)
-// ---------------------------------------------------------------------------
-struct ScalarArgs {
-  Geom g;
-  DevScalars* sc;
-  int nhist, readflag, explicit_visc;
-  double cflx, cfly, cflz, cflvv, CFL, uprim;
-  unsigned long long* umax_acc;
-  const double* stat_plane;   // [my][STAT_W]
-  const double* phist;        // [my][Nz/2][3]
-  double *stats, *stats2;     // [my][NSTAT]
-  double* ring;               // [cap][24]
-  int cap;
-  cplx *shp_new, *shm_new;
-};
-
-__global__ void __launch_bounds__(256)
-k_scalar(ScalarArgs a, int rk) {
-  DevScalars* sc = a.sc;
-  if (sc->halted) return;
-  const Geom& g = a.g;
-  const int tid = threadIdx.x;
-  const int ihist = (rk == 1) && ((sc->steps_local - 1) % a.nhist == 0);
-  __shared__ double sh_dt;
-  const double dt_old = sc->Deltat;
-  const int nzb = g.mgalz / 2;
-  if (ihist) {
-    for (int j = tid; j < g.my; j += blockDim.x) {
-      const double* sp = a.stat_plane + (long)j * STAT_W;
-      double ph[3] = {0.0, 0.0, 0.0};
-      for (int zb = 0; zb < nzb; ++zb)
-        for (int q = 0; q < 3; ++q) ph[q] += a.phist[((long)j * nzb + zb) * 3 + q];
-      const double o3s = sp[16] - g.s;
-      const double pk = -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);
-      const double v[SNF_NSTAT] = {sp[11], sp[12], sp[13], sp[0], sp[1], sp[2], sp[3], sp[4],
-                                   sp[5], sp[14], sp[15], sp[16], sp[6], sp[7], sp[8], sp[10],
-                                   ph[0], ph[1], ph[2], pk, sp[9], 0.0, 0.0};
-      double* st = a.stats + (long)j * SNF_NSTAT;
-      double* st2 = a.stats2 + (long)j * SNF_NSTAT;
-      for (int q = 0; q < 21; ++q) {
-        st[q] += v[q];
-        st2[q] += v[q] * dt_old;
-      }
-    }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    if (rk == 1) {
-      double um[4];
-      for (int q = 0; q < 4; ++q) {
-        um[q] = __longlong_as_double((long long)a.umax_acc[q]);
-        a.umax_acc[q] = 0ull;
-      }
-      if (sc->steps_local == 1 && a.readflag == 0) {
-        const double init = fabs(g.s) > 0.5 ? 1.0 : a.uprim;
-        for (int q = 0; q < 4; ++q) um[q] = fmax(init, um[q]);
-      }
-      for (int q = 0; q < 4; ++q) sc->umax[q] = um[q];
-      double dt = 1.0 / fmax(fmax(um[0] * a.cflx, um[2] * a.cfly), um[3] * a.cflz);
-      sc->CFLv = a.CFL * a.cflvv * dt;
-      if (sc->CFLv > a.CFL && a.explicit_visc == 1) dt = 1.0 / a.cflvv;
-      double Deltat = dt;
-      double dtr = g.Re / Deltat;
-      if (Deltat < 1e-10) sc->istop = 1;
-      if (sc->time + Deltat > sc->dumptime) {
-        Deltat = fabs(sc->dumptime - sc->time);
-        dtr = g.Re / Deltat;
-        if (sc->dumptime > sc->etime - 1e-13) sc->istop = 1;
-        sc->dumptime = sc->dumptime + sc->dtimag;
-        sc->idump = 1;
-      } else if (sc->time + Deltat > sc->etime) {
-        Deltat = fabs(sc->etime - sc->time);
-        dtr = g.Re / Deltat;
-        sc->istop = 1;
-      }
-      sc->Deltat = Deltat;
-      sc->dtr = dtr;
-    }
-    if (ihist) {
-      double ener[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, ener2[4] = {0, 0, 0, 0};
-      for (int j = 0; j < g.my; ++j) {
-        const double* sp = a.stat_plane + (long)j * STAT_W;
-        for (int q = 0; q < 10; ++q) ener[q] += (q < 9) ? sp[q] : sp[10];
-        const double o3s = sp[16] - g.s;
-        ener2[0] += -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);
-        ener2[1] += sp[9];
-      }
-      sc->nacum += 1;
-      sc->total_dt += sc->Deltat;
-      const double my = (double)g.my;
-      double uner[10];
-      for (int q = 0; q < 3; ++q) uner[q] = sqrt(fabs(ener[q] / my));
-      for (int q = 3; q < 6; ++q) uner[q] = ener[q] / my;
-      for (int q = 6; q < 9; ++q) uner[q] = sqrt(fabs(ener[q] / my));
-      uner[9] = ener[9] / my;
-      double* r = a.ring + (long)(sc->nrec % a.cap) * 24;
-      r[0] = sc->time;
-      r[1] = sc->Deltat;
-      for (int q = 0; q < 10; ++q) r[2 + q] = uner[q];
-      r[12] = a.CFL;
-      r[13] = sc->CFLv;
-      r[14] = sc->umax[1];
-      r[15] = sc->umax[2];
-      r[16] = sc->umax[3];
-      r[17] = sc->xoffb;
-      r[18] = sc->xofft;
-      r[19] = sc->umax[0];
-      for (int q = 0; q < 4; ++q) r[20 + q] = ener2[q] / my;
-      sc->nrec += 1;
-      if (isnan(uner[0])) sc->istop = 1;
-      if (uner[1] / (2.0 * M_PI / g.gam) < 1e-5) sc->istop = 1;
-    }
-    sh_dt = sc->Deltat;
-  }
-  __syncthreads();
-  // shear-periodic offsets for the next stage (cross.f90:726-746 / 866-884)
-  const double Deltat = sh_dt;
-  const double r1 = c_ca[rk - 1] * Deltat;
-  const double thr = a.explicit_visc == 1 ? 0.01 : 0.5;
-  double xwkb = sc->xwkb, xwkt = sc->xwkt;
-  if (fabs(g.s) > thr) {
-    xwkb = sc->xoffb + r1 * (g.s * (-g.Ly));
-    xwkt = sc->xofft + r1 * (g.s * g.Ly);
-  }
-  for (int i = tid; i < g.kxn; i += blockDim.x) {
-    const double kx = g.alp * (double)i;
-    double sn, cs;
-    sincos(kx * xwkt, &sn, &cs);
-    a.shp_new[i] = mkc(cs, -sn);
-    sincos(kx * xwkb, &sn, &cs);
-    a.shm_new[i] = mkc(cs, -sn);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    sc->xwkb = xwkb;
-    sc->xwkt = xwkt;
-    if (rk == 3) {
-      sc->xoffb = xwkb;
-      sc->xofft = xwkt;
-      sc->time = sc->time + Deltat;
-      sc->steps_done += 1;
-    }
-  }
-}
-
-// start of an advance call (cross.f90:81-103, 287-291, 383-406)
-__global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm) {
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    sc->halted = 0;
-    sc->steps_local = 0;
-    sc->istop = 0;
-    sc->idump = 0;
-    sc->xwkb = sc->xoffb;
-    sc->xwkt = sc->xofft;
-  }
-  const double xb = sc->xoffb, xt = sc->xofft;
-  for (int i = tid; i < g.kxn; i += blockDim.x) {
-    const double kx = g.alp * (double)i;
-    double sn, cs;
-    sincos(kx * xt, &sn, &cs);
-    shp[i] = mkc(cs, -sn);
-    sincos(kx * xb, &sn, &cs);
-    shm[i] = mkc(cs, -sn);
-  }
-}
-
-// the reference checks istop/idump at the top of every step except the
-// second one (cross.f90:353-374)
-__global__ void k_step_begin(DevScalars* sc) {
-  if (sc->halted) return;
-  const int next = sc->steps_local + 1;
-  if (next - 1 != 1 && (sc->istop || sc->idump)) {
-    sc->halted = 1;
-    return;
-  }
-  sc->steps_local = next;
-}
-
-// kx = 0 Hermitian fix on the state (cross.f90:287-291, set_conj)
-__global__ void k_set_conj(Geom g, cplx* vor, cplx* phi) {
-  const int j = blockIdx.x;
-  for (int k = g.nz + 1 + threadIdx.x; k < g.mz; k += blockDim.x) {
-    const long b = (long)j * g.P;
-    vor[b + k] = cconj(vor[b + g.mz - k]);
-    phi[b + k] = cconj(phi[b + g.mz - k]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // operator entry points (parity tests): nl independent lines, layout [y][l]
 // ---------------------------------------------------------------------------
-__global__ void k_op_deriv(int N, long nl, YCoef c, BaseFactor b, const TwistTail* tw,
+__global__ void __launch_bounds__(128)
+k_op_deriv(int N, long nl, YCoef c, BaseFactor b, const TwistTail* tw,
                            const cplx* shp, const cplx* shm, const cplx* u, cplx* du, cplx* ys,
                            int order) {
   const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nl) return;
-  DOp o{b, tw[l], shp[l], N};
+  const DOp o = make_dop(b, tw + l, gld(shp + l), N);
   const cplx* in[1] = {u + l};
   cplx* y[1] = {ys + l};
   if (order == 1)
-    deriv_forward<1, true>(o, c.rh1, in, nl, y, nl, shm[l]);
+    deriv_forward<1, true, 4>(o, c.rh1, in, nl, y, nl, gld(shm + l));
   else
-    deriv_forward<1, false>(o, c.rh2, in, nl, y, nl, shm[l]);
+    deriv_forward<1, false, 4>(o, c.rh2, in, nl, y, nl, gld(shm + l));
   cplx* out[1] = {du + l};
   StoreSink<1> sk{out, nl};
-  deriv_backward<1>(o, y, nl, sk);
+  deriv_backward<1, 4>(o, y, nl, sk);
 }
 
-__global__ void k_op_helm(int N, long nl, YCoef c, const double* k2, const cplx* shp,
+__global__ void __launch_bounds__(128)
+k_op_helm(int N, long nl, YCoef c, const double* k2, const cplx* shp,
                           const cplx* shm, const cplx* u, cplx* p, cplx* ys, HScratch hs) {
   const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nl) return;
   double lm[7];
-  lapmat(c, k2[l], lm);
+  lapmat(c, gldd(k2 + l), lm);
   HScratch h{hs.hr + l, hs.hc + l};
   HTail tail;
   const cplx* in[1] = {u + l};
   cplx* y[1] = {ys + l};
-  helm_forward<1>(lm, c.pr2, N, shp[l], shm[l], in, nl, y, nl, h, tail);
+  helm_forward<1, 4>(lm, c.pr2, N, gld(shp + l), gld(shm + l), in, nl, y, nl, h, tail);
   cplx* out[1] = {p + l};
   StoreSink<1> sk{out, nl};
-  helm_backward<1>(N, y, nl, h, tail, sk);
+  helm_backward<1, 4>(N, y, nl, h, tail, sk);
 }
