@@ -6,16 +6,19 @@ operators.f90, initcr.f90).  It exists only to check the HIP product path:
 only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import it.  The product library never calls into it.
 
-The reference itself is not buildable in this image (its FFTW include is an
-HTML error page and it needs an HDF5 Fortran module built by another
-compiler), so each function here is pinned instead by closed forms and
-invariants in ``tests/test_oracle_*.py``:
+The reference as a whole is not buildable in this image (its FFTW include is
+an HTML error page, it needs MPI and an HDF5 Fortran module built by another
+compiler).  Its compact y-operators are: ``oracle/Makefile`` compiles
+mod.f90 + diffy_shear_hepta.f90 in place with ROCm flang into
+``oracle/_ref/libshear_ref.so`` and ``oracle/ref.py`` calls its derivyc and
+lapcdy (``tests/test_ref_compact.py``).  Every function here is pinned by
+the tests:
 
-* ``compact``  -- coefficient design points and Taylor order, the
-  twisted-Fourier eigenvalue identity, dense brute-force solves;
+* ``compact``  -- the reference's own derivyc/lapcdy (above), coefficient
+  design points and Taylor order, the twisted-Fourier eigenvalue identity,
+  dense brute-force solves (``tests/test_oracle_compact.py``);
 * ``fftxz``    -- textbook DFT sums and the 3/2-rule index map;
-* ``cross``    -- divergence/curl identities of the KMM reconstruction,
-  energy neutrality of the dealiased u x omega term, the RDT Kelvin-mode
-  closed form (operators.f90:1516-1562) for the whole RK3 step, and
-  Parseval for the statistics.
+* ``cross``    -- curl/divergence identities of the KMM reconstruction and
+  orthogonality of the dealiased u x omega term to u
+  (``tests/test_oracle_xz.py``).
 """

@@ -1,0 +1,94 @@
+"""The reference's own compact y-operators, compiled from its sources.
+
+TEST INFRASTRUCTURE ONLY.  ``oracle/Makefile`` builds ``oracle/_ref/libshear_ref.so``
+from mod.f90 and diffy_shear_hepta.f90 as they lie in the reference tree
+(ROCm flang, MKL for BLAS ``dcopy``).  The CPU tests call its ``derivyc``
+(diffy_shear_hepta.f90:333-450) and ``lapcdy`` (diffy_shear_hepta.f90:466-525)
+to pin the numpy restatement in ``oracle/compact.py``.  The library keeps its
+grid in Fortran module variables that ``derivadas`` allocates once
+(diffy_shear_hepta.f90:535-560), so every grid size runs in a fresh child
+process.
+"""
+import ctypes
+import multiprocessing as mp
+import os
+
+import numpy as np
+
+LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "libshear_ref.so")
+
+
+def available():
+    return os.path.exists(LIB)
+
+
+def _cplx(z):
+    return (ctypes.c_double * 2)(float(np.real(z)), float(np.imag(z)))
+
+
+def _child(my, dy, jobs, q):
+    try:
+        os.environ.setdefault("OMP_NUM_THREADS", "1")
+        lib = ctypes.CDLL(LIB)
+        n = ctypes.c_int(my)
+        h = ctypes.c_double(dy)
+        lib.derivadas_(ctypes.byref(n), ctypes.byref(h))
+        lib.update_dy_(ctypes.byref(h))
+        out = []
+        for job in jobs:
+            kind = job[0]
+            real = kind.endswith("_real")
+            # complex lines are real(2, my); the 00 lines are real(1, my)
+            m = ctypes.c_int(1 if real else 2)
+            u = np.asarray(job[1], dtype=np.float64 if real else np.complex128)
+            res = np.empty_like(u)
+            for l in range(u.shape[1]):
+                ui = np.ascontiguousarray(u[:, l]).view(np.float64).copy()
+                ro = np.zeros_like(ui)
+                if kind == "deriv_real":
+                    lib.derivyc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
+                                 _cplx(1.0), _cplx(1.0), ctypes.byref(ctypes.c_int(job[2])),
+                                 ctypes.byref(m), ctypes.byref(ctypes.c_int(1)))
+                elif kind == "lap_real":
+                    lib.lapcdy_(ui.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.byref(ctypes.c_double(job[2][l])),
+                                ro.ctypes.data_as(ctypes.c_void_p), _cplx(1.0), _cplx(1.0),
+                                ctypes.byref(m))
+                elif kind == "deriv":
+                    shp, shm, order = job[2][l], job[3][l], job[4]
+                    lib.derivyc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
+                                 _cplx(shp), _cplx(shm), ctypes.byref(ctypes.c_int(order)),
+                                 ctypes.byref(m), ctypes.byref(ctypes.c_int(1)))
+                else:
+                    k2, shp, shm = job[2][l], job[3][l], job[4][l]
+                    lib.lapcdy_(ui.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ctypes.c_double(k2)),
+                                ro.ctypes.data_as(ctypes.c_void_p), _cplx(shp), _cplx(shm),
+                                ctypes.byref(m))
+                res[:, l] = ro if real else ro.view(np.complex128)
+            out.append(res)
+        q.put(("ok", out))
+    except Exception as e:  # reported to the parent
+        q.put(("err", repr(e)))
+
+
+def run(my, dy, jobs, timeout=120):
+    """Run jobs on lines of length my in a child process.
+
+    jobs: ("deriv", u[my, nl], shp[nl], shm[nl], order),
+          ("lap", u[my, nl], k2[nl], shp[nl], shm[nl]),
+          ("deriv_real", u[my, nl] real, order) or ("lap_real", u[my, nl] real, k2[nl]).
+    Returns one array [my, nl] per job.
+    """
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    p = ctx.Process(target=_child, args=(my, dy, jobs, q))
+    p.start()
+    try:
+        status, val = q.get(timeout=timeout)
+    finally:
+        p.join(timeout=10)
+        if p.is_alive():
+            p.kill()
+    if status != "ok":
+        raise RuntimeError(val)
+    return val
