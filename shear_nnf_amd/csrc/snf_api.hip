@@ -437,7 +437,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place)
   h->zb = 8;
-  while (h->zb > 2 && (3 * (size_t)h->zb * mgalx * sizeof(cplx) > 144 * 1024 || mgalz % h->zb != 0 ||
+  while (h->zb > 2 && (3 * (size_t)h->zb * mgalx * sizeof(cplx) > 156 * 1024 || mgalz % h->zb != 0 ||
                        (h->zb / 2) * mgalx > XZ_B_MAXW * XZ_B_THREADS ||
                        fft_max_values(h->px, 3 * h->zb, XZ_B_THREADS) > XZ_MAXV))
     h->zb /= 2;

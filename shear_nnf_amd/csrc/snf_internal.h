@@ -39,6 +39,17 @@ __device__ __forceinline__ void gst(cplx* p, cplx v) {
 __device__ __forceinline__ double gldd(const double* p) { return *(const SNF_AS1 double*)p; }
 __device__ __forceinline__ void gstd(double* p, double v) { *(SNF_AS1 double*)p = v; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  xcd_block renumbers a linear block index so that
+// consecutive logical blocks run on the same XCD at about the same time and
+// share the cache lines they both touch.
+#define SNF_NXCD 8
+__device__ __forceinline__ long xcd_block(long b, long nb) {
+  const long q = nb / SNF_NXCD;
+  if (b >= q * SNF_NXCD) return b;
+  return (b % SNF_NXCD) * q + b / SNF_NXCD;
+}
+
 // Grid description passed by value to every kernel.
 struct Geom {
   int my, kxn, mz, nz, mgalx, mgalz;

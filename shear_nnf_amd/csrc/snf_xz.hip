@@ -150,7 +150,10 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
   if (sc->halted) return;
   extern __shared__ cplx lds[];
   const int Mx = g.mgalx, Nz = g.mgalz, kxn = g.kxn;
-  const int zb = blockIdx.x, j = blockIdx.y, z0 = ZB * zb;
+  // neighbouring z-blocks of one plane share 128-byte lines of the hybrid
+  // field: keep them on one XCD
+  const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
+  const int zb = (int)(lb % gridDim.x), j = (int)(lb / gridDim.x), z0 = ZB * zb;
   const long fstride = (long)g.my * kxn * Nz;
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
   const int nseq = 3 * ZB;    // sequence s = p*ZB + r (pair p, row r)
