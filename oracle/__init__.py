@@ -18,7 +18,13 @@ the tests:
   design points and Taylor order, the twisted-Fourier eigenvalue identity,
   dense brute-force solves (``tests/test_oracle_compact.py``);
 * ``fftxz``    -- textbook DFT sums and the 3/2-rule index map;
-* ``cross``    -- curl/divergence identities of the KMM reconstruction and
-  orthogonality of the dealiased u x omega term to u
-  (``tests/test_oracle_xz.py``).
+* ``cross``    -- curl/divergence identities of the KMM reconstruction,
+  orthogonality of the dealiased u x omega term to u, Parseval for the 'f'
+  second moments and means of ``_hist_f`` (``tests/test_oracle_xz.py``).
+
+Parity unpinned (no check short of the compiled cross.f90, which does not
+build here): the composition inside ``Cross.run`` (order of phase maps,
+shift update, dt clipping, 00-mode and set_conj placement), ``_new_dt``,
+the dissipation sum of ``_hist_f`` and ``_hist_w``.  They follow cross.f90
+line by line, with citations.
 """

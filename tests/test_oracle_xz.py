@@ -139,3 +139,30 @@ def test_dealiased_lamb_vector_is_orthogonal_to_u(chi):
     norm = sum(np.sum(wgt[None, :, None] * np.abs(a) * np.abs(b), axis=(1, 2))
                for a, b in ((u1, h1), (u2, h2), (u3, h3)))
     assert np.max(np.abs(inner) / norm) < 1e-13
+
+
+def test_hist_second_moments_are_plane_means():
+    """'f' statistics (cross.f90:1798-1860) by Parseval: the weighted spectral
+    sums equal the x-z means of the physical products."""
+    g = Grid(24, 9, 24, Re=100.0, alp=math.pi, gam=2 * math.pi, Ly=1.0)
+    rng = np.random.default_rng(21)
+    f = []
+    for _ in range(6):
+        a = rng.normal(size=(g.my, g.kxn, g.mz)) + 1j * rng.normal(size=(g.my, g.kxn, g.mz))
+        a[:, 0, 0] = a[:, 0, 0].real
+        f.append(set_conj(a, g.nz))
+    u1, u2, u3, o1, o2, o3 = f
+    c = Cross(g)
+    c.Deltat = 0.5
+    ener, ener2 = np.zeros(10), np.zeros(4)
+    c._hist_f(u1, u2, u3, o1, o2, o3, u2, ener, ener2)
+    P = [fftxz.to_phys(a, g.mgalx, g.mgalz) for a in f]
+    mean = lambda a, b: np.mean(a * b, axis=(1, 2))
+    pairs = {3: (0, 0), 4: (1, 1), 5: (2, 2), 6: (0, 1), 7: (0, 2), 8: (1, 2),
+             12: (3, 3), 13: (4, 4), 14: (5, 5), 20: (3, 4)}
+    for idx, (a, b) in pairs.items():
+        want = mean(P[a], P[b])
+        assert np.max(np.abs(c.stats[idx] - want)) < 1e-12 * np.max(np.abs(want)), idx
+        assert np.max(np.abs(c.stats2[idx] - 0.5 * want)) < 1e-12 * np.max(np.abs(want)), idx
+    for idx, a in zip((0, 1, 2, 9, 10, 11), range(6)):
+        assert np.allclose(c.stats[idx], np.mean(P[a], axis=(1, 2)), rtol=0, atol=1e-13)
