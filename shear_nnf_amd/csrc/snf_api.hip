@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -92,6 +93,7 @@ struct snf_handle {
   TwistTail id1{}, id2{};
   size_t ldsA = 0, ldsB = 0, ldsC = 0;
   int zb = 8;   // z rows per block of the x-row kernel
+  int bthreads = 1024;   // threads per block of the x-row kernel
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
   // kernel timing
   bool timing = false;
@@ -228,9 +230,12 @@ int launch_xz(snf_handle* h, int rk) {
                                           h->sc, h->p.nhist);
   dim3 gB(g.mgalz / h->zb, g.my);
   { TimedLaunch tl(h, SNF_K_XZ_B);
-  k_xz_B<<<gB, XZ_B_THREADS, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk,
+  {
+    auto kb = h->bthreads == 256 ? k_xz_B<256> : (h->bthreads == 512 ? k_xz_B<512> : k_xz_B<1024>);
+    kb<<<gB, h->bthreads, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk,
                                                    h->umax_acc, h->phist, h->sc, h->p.nhist,
-                                                   h->zb); }
+                                                   h->zb);
+  } }
   { TimedLaunch tl(h, SNF_K_XZ_C);
   k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg, h->rhv, h->rf0u, h->rf0w,
                                           h->pz, h->twz, h->sc); }
@@ -435,14 +440,29 @@ int snf_create(const snf_params* p, snf_handle** out) {
     TRY(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy scalars"));
   }
-  // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place)
-  h->zb = 8;
-  while (h->zb > 2 && (3 * (size_t)h->zb * mgalx * sizeof(cplx) > 156 * 1024 || mgalz % h->zb != 0 ||
-                       (h->zb / 2) * mgalx > XZ_B_MAXW * XZ_B_THREADS ||
-                       fft_max_values(h->px, 3 * h->zb, XZ_B_THREADS) > XZ_MAXV))
-    h->zb /= 2;
-  if (mgalz % h->zb != 0 || fft_max_values(h->px, 3 * h->zb, XZ_B_THREADS) > XZ_MAXV ||
-      (h->zb / 2) * mgalx > XZ_B_MAXW * XZ_B_THREADS) {
+  // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
+  // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning).
+  auto b_fits = [&](int nt, int zb) {
+    return zb >= 2 && mgalz % zb == 0 && 3 * (size_t)zb * mgalx * sizeof(cplx) <= 156 * 1024 &&
+           (zb / 2) * mgalx <= XZ_B_MAXW * nt && fft_max_values(h->px, 3 * zb, nt) <= XZ_MAXV;
+  };
+  // 256 threads x 2 rows measured fastest at 384x385x384 (8 LDS-resident
+  // rows per CU at 4 blocks/CU beat 1-2 larger blocks)
+  h->bthreads = 256;
+  h->zb = 2;
+  if (!b_fits(h->bthreads, h->zb)) {
+    h->bthreads = 1024;
+    h->zb = 8;
+    while (h->zb > 2 && !b_fits(h->bthreads, h->zb)) h->zb /= 2;
+  }
+  if (const char* e = getenv("SNF_XZ_B")) {
+    int nt = 0, zb = 0;
+    if (sscanf(e, "%d,%d", &nt, &zb) == 2 && (nt == 256 || nt == 512 || nt == 1024) && b_fits(nt, zb)) {
+      h->bthreads = nt;
+      h->zb = zb;
+    }
+  }
+  if (!b_fits(h->bthreads, h->zb)) {
     snf_destroy(h);
     return fail(SNF_BAD_PARAM, "x-z transform sizes exceed the LDS plan of this build");
   }
@@ -450,7 +470,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
   h->ldsB = 3 * (size_t)h->zb * mgalx * sizeof(cplx);
   h->ldsC = 2 * 3 * (size_t)mgalz * sizeof(cplx);
   (void)hipFuncSetAttribute((const void*)k_xz_A, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
-  (void)hipFuncSetAttribute((const void*)k_xz_B, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
+  for (const void* kb : {(const void*)k_xz_B<256>, (const void*)k_xz_B<512>, (const void*)k_xz_B<1024>})
+    (void)hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
   (void)hipFuncSetAttribute((const void*)k_xz_C, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
 #undef TRY
   *out = h;

@@ -141,9 +141,9 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
   return (unsigned long long)__double_as_longlong(v);
 }
 
-#define XZ_B_THREADS 1024
-
-__global__ void __launch_bounds__(XZ_B_THREADS)
+// NT threads per block (256, 512 or 1024), ZB z-rows per block
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px,
        const cplx* __restrict__ twx, int rk, unsigned long long* umax_acc,
        double* __restrict__ phist, const DevScalars* sc, int nhist, int ZB) {
@@ -228,7 +228,7 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
     }
   }
   // block reductions: wave shuffles, then one value per wave (fixed order)
-  __shared__ double red[4][XZ_B_THREADS / 64];
+  __shared__ double red[4][NT / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
   if (rk == 1) {
     for (int o = 32; o > 0; o >>= 1) {
