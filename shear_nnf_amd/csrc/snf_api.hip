@@ -16,6 +16,7 @@
 #include "snf_internal.h"
 #include "snf_setup.h"
 #include "snf_xz.hip"
+#include "snf_xzw.hip"
 #include "snf_scalar.hip"
 #include "snf_ykern.hip"
 
@@ -94,6 +95,11 @@ struct snf_handle {
   size_t ldsA = 0, ldsB = 0, ldsC = 0;
   int zb = 8;   // z rows per block of the x-row kernel
   int bthreads = 1024;   // threads per block of the x-row kernel
+  // wave-FFT x-z kernels (planes of 64*R points), rx = mgalx/64, rz = mgalz/64
+  bool wave_xz = false;
+  int rx = 0, rz = 0;
+  size_t ldsWA = 0, ldsWB = 0, ldsWC = 0;
+  double* upart = nullptr;   // per-row umax partials of the wave x-row kernel
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
   // kernel timing
   bool timing = false;
@@ -215,30 +221,78 @@ void collect_timing(snf_handle* h) {
   h->ev_used = 0;
 }
 
+template <int R>
+void launch_xzw_A(snf_handle* h) {
+  const Geom& g = h->g;
+  const unsigned nb = (unsigned)(((long)g.my * g.kxn + WA_WAVES - 1) / WA_WAVES);
+  k_xzw_A<R><<<nb, 64 * WA_WAVES, h->ldsWA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome,
+      h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->twz, h->sc);
+}
+template <int R>
+void launch_xzw_B(snf_handle* h, int rk) {
+  const Geom& g = h->g;
+  dim3 gB(g.mgalz / WB_ROWS, g.my);
+  k_xzw_B<R><<<gB, 64 * WB_ROWS, h->ldsWB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->twx, rk,
+      h->upart, h->phist, h->sc, h->p.nhist);
+}
+template <int R>
+void launch_xzw_C(snf_handle* h) {
+  const Geom& g = h->g;
+  const unsigned nb = (unsigned)(((long)g.my * g.kxn + WC_WAVES - 1) / WC_WAVES);
+  k_xzw_C<R><<<nb, 64 * WC_WAVES, h->ldsWC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg,
+      h->rhv, h->rf0u, h->rf0w, h->twz, h->sc);
+}
+
 int launch_xz(snf_handle* h, int rk) {
   const Geom& g = h->g;
   // the x-z stage reads U, W as real arrays
   k_cplx_to_real<<<(g.my + 255) / 256, 256, 0, h->stream>>>(h->u00c, h->w00c, h->u00r, h->w00r, g.my);
-  dim3 gA(g.kxn, g.my);
-  { TimedLaunch tl(h, SNF_K_XZ_A);
-  k_xz_A<<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
-                                          h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->pz,
-                                          h->twz, h->sc); }
+  if (h->wave_xz) {
+    { TimedLaunch tl(h, SNF_K_XZ_A);
+      switch (h->rz) {
+        case 2: launch_xzw_A<2>(h); break;
+        case 3: launch_xzw_A<3>(h); break;
+        case 4: launch_xzw_A<4>(h); break;
+        default: launch_xzw_A<6>(h); break;
+      } }
+  } else {
+    dim3 gA(g.kxn, g.my);
+    TimedLaunch tl(h, SNF_K_XZ_A);
+    k_xz_A<<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
+                                            h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->pz,
+                                            h->twz, h->sc);
+  }
   if (rk == 1)
     k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                           h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->stat_plane,
                                           h->sc, h->p.nhist);
-  dim3 gB(g.mgalz / h->zb, g.my);
-  { TimedLaunch tl(h, SNF_K_XZ_B);
-  {
-    auto kb = h->bthreads == 256 ? k_xz_B<256> : (h->bthreads == 512 ? k_xz_B<512> : k_xz_B<1024>);
-    kb<<<gB, h->bthreads, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk,
+  if (h->wave_xz) {
+    { TimedLaunch tl(h, SNF_K_XZ_B);
+      switch (h->rx) {
+        case 2: launch_xzw_B<2>(h, rk); break;
+        case 3: launch_xzw_B<3>(h, rk); break;
+        case 4: launch_xzw_B<4>(h, rk); break;
+        default: launch_xzw_B<6>(h, rk); break;
+      } }
+    { TimedLaunch tl(h, SNF_K_XZ_C);
+      switch (h->rz) {
+        case 2: launch_xzw_C<2>(h); break;
+        case 3: launch_xzw_C<3>(h); break;
+        case 4: launch_xzw_C<4>(h); break;
+        default: launch_xzw_C<6>(h); break;
+      } }
+  } else {
+    dim3 gA(g.kxn, g.my);
+    dim3 gB(g.mgalz / h->zb, g.my);
+    { TimedLaunch tl(h, SNF_K_XZ_B);
+      auto kb = h->bthreads == 256 ? k_xz_B<256> : (h->bthreads == 512 ? k_xz_B<512> : k_xz_B<1024>);
+      kb<<<gB, h->bthreads, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk,
                                                    h->umax_acc, h->phist, h->sc, h->p.nhist,
-                                                   h->zb);
-  } }
-  { TimedLaunch tl(h, SNF_K_XZ_C);
-  k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg, h->rhv, h->rf0u, h->rf0w,
-                                          h->pz, h->twz, h->sc); }
+                                                   h->zb); }
+    { TimedLaunch tl(h, SNF_K_XZ_C);
+      k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg, h->rhv,
+                                              h->rf0u, h->rf0w, h->pz, h->twz, h->sc); }
+  }
   HIPCHK(hipGetLastError());
   return SNF_OK;
 }
@@ -250,7 +304,9 @@ ScalarArgs sargs(snf_handle* h) {
   a.nhist = h->p.nhist;
   a.readflag = h->p.readflag;
   a.explicit_visc = h->p.explicit_visc;
-  a.zb_rows = h->zb;
+  a.zb_rows = h->wave_xz ? 1 : h->zb;
+  a.upart = h->wave_xz ? h->upart : nullptr;
+  a.nupart = h->wave_xz ? (long)h->g.my * h->g.mgalz : 0;
   a.cflx = h->cflx;
   a.cfly = h->cfly;
   a.cflz = h->cflz;
@@ -406,6 +462,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->umax_acc, 4));
   TRY(A(&h->stat_plane, (size_t)my * STAT_W));
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
+  TRY(A(&h->upart, (size_t)my * mgalz * 4));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
   TRY(A(&h->ring, (size_t)h->cap * 24));
@@ -465,6 +522,26 @@ int snf_create(const snf_params* p, snf_handle** out) {
   if (!b_fits(h->bthreads, h->zb)) {
     snf_destroy(h);
     return fail(SNF_BAD_PARAM, "x-z transform sizes exceed the LDS plan of this build");
+  }
+  {
+    auto wave_size = [](int n) { return n % 64 == 0 && (n / 64 == 2 || n / 64 == 3 || n / 64 == 4 || n / 64 == 6); };
+    const char* e = getenv("SNF_XZ_PATH");
+    h->wave_xz = wave_size(mgalx) && wave_size(mgalz) && !(e && std::string(e) == "generic");
+    if (h->wave_xz) {
+      h->rx = mgalx / 64;
+      h->rz = mgalz / 64;
+      h->ldsWA = xzw_lds_A(h->rz);
+      h->ldsWC = xzw_lds_C(h->rz);
+      h->ldsWB = xzw_lds_B(h->rx, g.kxn);
+      const void* ka[4] = {(const void*)k_xzw_A<2>, (const void*)k_xzw_A<3>, (const void*)k_xzw_A<4>, (const void*)k_xzw_A<6>};
+      const void* kb[4] = {(const void*)k_xzw_B<2>, (const void*)k_xzw_B<3>, (const void*)k_xzw_B<4>, (const void*)k_xzw_B<6>};
+      const void* kc[4] = {(const void*)k_xzw_C<2>, (const void*)k_xzw_C<3>, (const void*)k_xzw_C<4>, (const void*)k_xzw_C<6>};
+      for (int q = 0; q < 4; ++q) {
+        (void)hipFuncSetAttribute(ka[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWA);
+        (void)hipFuncSetAttribute(kb[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWB);
+        (void)hipFuncSetAttribute(kc[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWC);
+      }
+    }
   }
   h->ldsA = 2 * 3 * (size_t)mgalz * sizeof(cplx);   // ping-pong, 3 sequences
   h->ldsB = 3 * (size_t)h->zb * mgalx * sizeof(cplx);

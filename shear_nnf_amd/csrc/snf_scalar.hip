@@ -128,6 +128,8 @@ struct ScalarArgs {
   int nhist, readflag, explicit_visc, zb_rows;
   double cflx, cfly, cflz, cflvv, CFL, uprim;
   unsigned long long* umax_acc;
+  const double* upart;        // [nupart][4] partial maxima (or null)
+  long nupart;
   const double* stat_plane;   // [my][STAT_W]
   const double* phist;        // [my][Nz/2][3]
   double *stats, *stats2;     // [my][NSTAT]
@@ -165,12 +167,27 @@ k_scalar(ScalarArgs a, int rk) {
       }
     }
   }
+  // maxima of the per-row partials (order-free)
+  __shared__ double red_um[4][256];
+  if (rk == 1) {
+    double lm[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long n = tid; n < a.nupart; n += blockDim.x)
+      for (int q = 0; q < 4; ++q) lm[q] = fmax(lm[q], a.upart[n * 4 + q]);
+    for (int q = 0; q < 4; ++q) red_um[q][tid] = lm[q];
+  }
   __syncthreads();
+  if (rk == 1) {
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (tid < st)
+        for (int q = 0; q < 4; ++q) red_um[q][tid] = fmax(red_um[q][tid], red_um[q][tid + st]);
+      __syncthreads();
+    }
+  }
   if (tid == 0) {
     if (rk == 1) {
       double um[4];
       for (int q = 0; q < 4; ++q) {
-        um[q] = __longlong_as_double((long long)a.umax_acc[q]);
+        um[q] = fmax(__longlong_as_double((long long)a.umax_acc[q]), red_um[q][0]);
         a.umax_acc[q] = 0ull;
       }
       if (sc->steps_local == 1 && a.readflag == 0) {

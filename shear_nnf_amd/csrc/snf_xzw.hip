@@ -1,0 +1,312 @@
+// Nonlinear stage of one RK3 substep (hvhg, cross.f90:1127-1765) for planes
+// of 64*R points in x and z (R = 2, 3, 4, 6), built on the wave-level FFT of
+// snf_wfft.h: every transform runs inside one wave, in registers, with two
+// wave-private LDS exchanges and no block barrier.
+//
+//   WA  one wave per (kx, y) row: loads the 5 spectral inputs with the
+//       fill00 map, KMM reconstruction of u, v, w, omega (cross.f90:1221-
+//       1276), inverse z transform of the 6 fields -> hybrid [f][y][kx][z]
+//   WB  one block per (8 z-rows, y), one wave per row: the 6 hybrid fields
+//       staged through LDS (coalesced 128-byte runs of 8 z values), three
+//       paired c2r x transforms, u x omega, umax and the 'p' statistics
+//       (cross.f90:1430-1462, 2030-2045), r2c x transforms -> hybrid, 3 fields
+//   WC  one wave per (kx, y) row: forward z transform, pack00, normalisation
+//       1/(mgalx mgalz), output combinations and rf0u/rf0w
+//       (cross.f90:1477-1546)
+// Same data layout and the same results (to rounding) as the generic
+// Stockham kernels of snf_xz.hip, which serve every other grid size.
+#include "snf_internal.h"
+#include "snf_wfft.h"
+
+#define WA_WAVES 4
+#define WB_ROWS 8
+#define WC_WAVES 4
+
+// padded z position (0..Nz-1) of lane l, value m -> kz index, or -1 (gap)
+__device__ __forceinline__ int wz_k(int pos, int nz, int mz, int Nz) {
+  if (pos <= nz) return pos;
+  if (pos >= Nz - nz) return pos - Nz + mz;
+  return -1;
+}
+
+__device__ __forceinline__ void stage_roots(cplx* rt, const cplx* tw, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) rt[t] = gld(tw + t);
+}
+
+// ---------------------------------------------------------------------------
+template <int RZ>
+__global__ void __launch_bounds__(64 * WA_WAVES)
+k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+        const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+        const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+        const double* __restrict__ w00, const double* __restrict__ dudy,
+        const double* __restrict__ dwdy, double vbulk, cplx* __restrict__ hyb,
+        const cplx* __restrict__ twz, const DevScalars* sc) {
+  typedef WFFT<RZ, 1> W;
+  if (sc->halted) return;
+  extern __shared__ cplx lds[];
+  cplx* rt = lds;
+  const int Nz = 64 * RZ;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  cplx* buf = lds + Nz + wave * W::LDS;
+  stage_roots(rt, twz, Nz);
+  __syncthreads();
+  const long row = (long)blockIdx.x * WA_WAVES + wave;   // row = j*kxn + i
+  if (row >= (long)g.my * g.kxn) return;
+  const int j = (int)(row / g.kxn), i = (int)(row - (long)j * g.kxn);
+  const double u00j = gldd(u00 + j), w00j = gldd(w00 + j);
+  const double dudyj = gldd(dudy + j), dwdyj = gldd(dwdy + j);
+  cplx d[6][RZ];
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) {
+    const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
+    if (k >= 0) {
+      const long base = (long)j * g.P + (long)i * g.mz;
+      PlaneIn q;
+      if (i == 0 && k > g.nz) {   // set_conj (operators.f90:1382-1429)
+        q.phic = cconj(gld(phi + base + (g.mz - k)));
+        q.ome2c = cconj(gld(vor + base + (g.mz - k)));
+      } else {
+        q.phic = gld(phi + base + k);
+        q.ome2c = gld(vor + base + k);
+      }
+      q.v = gld(hg + base + k);
+      q.dv = gld(hv + base + k);
+      q.dome = gld(dvordy + base + k);
+      cplx o[6];
+      derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, o);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) d[f][m] = o[f];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) d[f][m] = mkc(0.0, 0.0);
+    }
+  }
+  const long fstride = (long)g.my * g.kxn * Nz;
+  cplx* dst = hyb + row * Nz;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    cplx out[8];
+    wfft_l2t<RZ, 1>(d[f], out, buf, rt);
+    if (lane < W::NT) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) gst(dst + f * fstride + lane + 8 * RZ * q, out[q]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int RX>
+__global__ void __launch_bounds__(64 * WB_ROWS)
+k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
+        const cplx* __restrict__ twx, int rk, double* __restrict__ upart,
+        double* __restrict__ phist, const DevScalars* sc, int nhist) {
+  typedef WFFT<RX, 1> WI;
+  typedef WFFT<RX, -1> WF;
+  if (sc->halted) return;
+  extern __shared__ cplx lds[];
+  const int Mx = 64 * RX, Nz = g.mgalz, kxn = g.kxn;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  cplx* rt = lds;
+  cplx* work = lds + Mx;                 // staging, then per-wave FFT buffers
+  cplx* buf = work + wave * WI::LDS;
+  stage_roots(rt, twx, Mx);
+  const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
+  const int zblk = (int)(lb % gridDim.x), j = (int)(lb / gridDim.x);
+  const int z0 = WB_ROWS * zblk, r = wave;
+  const long fstride = (long)g.my * kxn * Nz;
+  const long rowbase = (long)j * kxn * Nz + z0;   // + kx*Nz + r
+  // --- load the 3 field pairs of row r in lane layout (bins l + 64 m) ---
+  cplx zin[3][RX];
+  cplx* S = work;                        // [2][kxn][WB_ROWS + 1]
+  const int SP = WB_ROWS + 1;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 2 * kxn * WB_ROWS; e += blockDim.x) {
+      const int h = e / (kxn * WB_ROWS);
+      const int rem = e - h * kxn * WB_ROWS;
+      const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
+      S[(h * kxn + kx) * SP + rr] = gld(hin + (2 * p + h) * fstride + rowbase + (long)kx * Nz + rr);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < RX; ++m) {
+      const int b = lane + 64 * m;
+      cplx s = mkc(0.0, 0.0);
+      if (b == 0) {
+        const cplx Ga = S[r], Gb = S[kxn * SP + r];
+        s = mkc(Ga.x, Gb.x);     // c2r ignores the kx=0 imaginary part
+      } else if (b < kxn) {
+        const cplx Ga = S[b * SP + r], Gb = S[(kxn + b) * SP + r];
+        s = mkc(Ga.x - Gb.y, Ga.y + Gb.x);
+      } else if (b > Mx - kxn) {
+        const int i = Mx - b;
+        const cplx Ga = S[i * SP + r], Gb = S[(kxn + i) * SP + r];
+        s = mkc(Ga.x + Gb.y, Gb.x - Ga.y);
+      }
+      zin[p][m] = s;
+    }
+  }
+  __syncthreads();   // staging dead: the region becomes the FFT buffers
+  // --- inverse x transforms: task t holds x = t + 8 RX q ---
+  cplx zx[3][8];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) wfft_l2t<RX, 1>(zin[p], zx[p], buf, rt);
+  const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
+  const double c2 = 2.0 * g.chi;
+  const double sy = g.s * ((double)j * g.dy - 0.5 * g.Ly);
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  cplx z12[8], z3[8];
+  if (lane < WI::NT) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double u = zx[0][q].x, v = zx[0][q].y, w = zx[1][q].x, o1 = zx[1][q].y;
+      const double o2 = zx[2][q].x, o3 = zx[2][q].y;
+      m0 = fmax(m0, fabs(u));
+      m1 = fmax(m1, fabs(u + sy));
+      m2 = fmax(m2, fabs(v));
+      m3 = fmax(m3, fabs(w));
+      p0 += v * (u * u);
+      p1 += v * (w * w);
+      p2 += v * v * v;
+      z12[q] = mkc(v * (o3 + c2) - w * o2, w * o1 - u * (o3 + c2));
+      z3[q] = mkc(u * o2 - v * o1, 0.0);
+    }
+  }
+  // --- forward x transforms -> lane layout bins ---
+  cplx a12[RX], a3[RX];
+  wfft_t2l<RX, -1>(z12, a12, buf, rt);
+  wfft_t2l<RX, -1>(z3, a3, buf, rt);
+  // separate H1, H2 from Z12 = H1 + i H2: needs bin Mx - k from another lane
+  wf_sync();
+#pragma unroll
+  for (int m = 0; m < RX; ++m) buf[lane + 64 * m] = a12[m];
+  wf_sync();
+  __syncthreads();   // every wave done with its FFT buffer before the output tile
+  cplx* O = work;    // [3][kxn][WB_ROWS + 1]
+  cplx h1[RX], h2[RX];
+#pragma unroll
+  for (int m = 0; m < RX; ++m) {
+    const int k = lane + 64 * m;
+    if (k < kxn) {
+      const cplx A = buf[k], Am = buf[k == 0 ? 0 : Mx - k];
+      h1[m] = mkc(0.5 * (A.x + Am.x), 0.5 * (A.y - Am.y));
+      h2[m] = mkc(0.5 * (A.y + Am.y), -0.5 * (A.x - Am.x));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < RX; ++m) {
+    const int k = lane + 64 * m;
+    if (k < kxn) {
+      O[(0 * kxn + k) * SP + r] = h1[m];
+      O[(1 * kxn + k) * SP + r] = h2[m];
+      O[(2 * kxn + k) * SP + r] = a3[m];
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 3 * kxn * WB_ROWS; e += blockDim.x) {
+    const int f = e / (kxn * WB_ROWS);
+    const int rem = e - f * kxn * WB_ROWS;
+    const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
+    gst(hout + f * fstride + rowbase + (long)kx * Nz + rr, O[(f * kxn + kx) * SP + rr]);
+  }
+  // --- reductions: one atomic per wave for umax, one partial per row ---
+  if (rk == 1) {
+    for (int o = 32; o > 0; o >>= 1) {
+      m0 = fmax(m0, __shfl_xor(m0, o, 64));
+      m1 = fmax(m1, __shfl_xor(m1, o, 64));
+      m2 = fmax(m2, __shfl_xor(m2, o, 64));
+      m3 = fmax(m3, __shfl_xor(m3, o, 64));
+    }
+    if (lane == 0) {
+      double* up = upart + ((long)j * Nz + z0 + r) * 4;
+      gstd(up, m0);
+      gstd(up + 1, m1);
+      gstd(up + 2, m2);
+      gstd(up + 3, m3);
+    }
+  }
+  if (ihist) {
+    for (int o = 32; o > 0; o >>= 1) {
+      p0 += __shfl_xor(p0, o, 64);
+      p1 += __shfl_xor(p1, o, 64);
+      p2 += __shfl_xor(p2, o, 64);
+    }
+    if (lane == 0) {
+      double* ph = phist + ((long)j * Nz + z0 + r) * 3;
+      gstd(ph, p0);
+      gstd(ph + 1, p1);
+      gstd(ph + 2, p2);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int RZ>
+__global__ void __launch_bounds__(64 * WC_WAVES)
+k_xzw_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
+        cplx* __restrict__ dvordy, cplx* __restrict__ hg, cplx* __restrict__ hv,
+        double* __restrict__ rf0u, double* __restrict__ rf0w, const cplx* __restrict__ twz,
+        const DevScalars* sc) {
+  typedef WFFT<RZ, -1> W;
+  if (sc->halted) return;
+  extern __shared__ cplx lds[];
+  cplx* rt = lds;
+  const int Nz = 64 * RZ;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  cplx* buf = lds + Nz + wave * W::LDS;
+  stage_roots(rt, twz, Nz);
+  __syncthreads();
+  const long row = (long)blockIdx.x * WC_WAVES + wave;
+  if (row >= (long)g.my * g.kxn) return;
+  const int j = (int)(row / g.kxn), i = (int)(row - (long)j * g.kxn);
+  const long fstride = (long)g.my * g.kxn * Nz;
+  const cplx* src = hout + row * Nz;
+  cplx h[3][RZ];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    cplx v[8];
+    if (lane < W::NT) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = gld(src + f * fstride + lane + 8 * RZ * q);
+    }
+    wfft_t2l<RZ, -1>(v, h[f], buf, rt);
+  }
+  const double norm = 1.0 / ((double)g.mgalx * (double)Nz);
+  const double kx = g.alp * (double)i;
+  const double kx2 = kx * kx;
+  const long base = (long)j * g.P + (long)i * g.mz;
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) {
+    const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
+    if (k >= 0) {
+      const cplx h1 = crmul(norm, h[0][m]), h2 = crmul(norm, h[1][m]), h3 = crmul(norm, h[2][m]);
+      const double kz = kz_of(k, g.nz, g.mz, g.gam);
+      const long idx = base + k;
+      const cplx v = gld(vel + idx);
+      gst(dvordy + idx, cadd(mul_i(kx, h1), mul_i(kz, h3)));
+      const cplx dd = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);
+      gst(hg + idx, cadd(mul_i(-kx, h3), mul_i(kz, dd)));
+      const double kk = kx2 + kz * kz;
+      gst(hv + idx, mkc(-h2.x * kk, -h2.y * kk));
+      if (i == 0 && k == 0) {
+        gstd(rf0u + j, h1.x);
+        gstd(rf0w + j, h3.x);
+      }
+    }
+  }
+}
+
+// LDS bytes of the three kernels for R = Nz/64 (A, C) and Rx = Mx/64 (B)
+inline size_t xzw_lds_A(int R) { return (size_t)(64 * R + WA_WAVES * (R * WF_EP > 8 * R * WF_FP ? R * WF_EP : 8 * R * WF_FP)) * sizeof(cplx); }
+inline size_t xzw_lds_C(int R) { return xzw_lds_A(R); }
+inline size_t xzw_lds_B(int Rx, int kxn) {
+  const size_t fft = (size_t)WB_ROWS * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
+  const size_t stage = (size_t)2 * kxn * (WB_ROWS + 1);
+  const size_t out = (size_t)3 * kxn * (WB_ROWS + 1);
+  size_t w = fft > stage ? fft : stage;
+  w = w > out ? w : out;
+  return (size_t)(64 * Rx + w) * sizeof(cplx);
+}

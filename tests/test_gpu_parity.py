@@ -108,6 +108,58 @@ def test_hvhg(lib):
     assert rel_l2(rf0w, c.rf0w) < OP_TOL
 
 
+def _hvhg_pair(grid, seed=11):
+    vor, phi, u00, w00 = box_ic(*grid)
+    rng = np.random.default_rng(seed)
+    shape = vor.shape
+    v = 0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    dv = 0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    dom = 0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape))
+    box = _box(*grid)
+    box.set_state(vor, phi, u00, w00)
+    got = box.hvhg(v, dv, dom)
+    c = _oracle(*grid)
+    c.set_state(vor, phi, u00, w00)
+    c.hg, c.hv, c.dvordy = v.copy(), dv.copy(), dom.copy()
+    c.istep = 5
+    c.hvhg(2, 0)
+    return got, (c.dvordy, c.hg, c.hv, c.rf0u, c.rf0w)
+
+
+# planes of 64*R points take the wave-FFT kernels (R = 2, 3, 4, 6)
+@pytest.mark.parametrize("grid", [(128, 17, 128), (192, 17, 192), (256, 17, 256),
+                                  (384, 17, 384), (192, 17, 384), (384, 17, 128)])
+def test_hvhg_wave_kernels(lib, grid):
+    got, ref = _hvhg_pair(grid)
+    for a, b in zip(got, ref):
+        assert rel_l2(a, b) < OP_TOL
+
+
+def test_wave_and_generic_xz_paths_agree(lib, monkeypatch):
+    grid = (192, 17, 128)
+    vor, phi, u00, w00 = box_ic(*grid)
+    rng = np.random.default_rng(5)
+    shape = vor.shape
+    fields = [0.1 * (rng.normal(size=shape) + 1j * rng.normal(size=shape)) for _ in range(3)]
+    outs = []
+    for path in ("wave", "generic"):
+        monkeypatch.setenv("SNF_XZ_PATH", path)
+        box = _box(*grid)
+        box.set_state(vor, phi, u00, w00)
+        outs.append(box.hvhg(*fields))
+    # two FFT factorisations of the same transforms: rounding-level agreement
+    for a, b in zip(*outs):
+        assert rel_l2(a, b) < OP_TOL
+
+
+def test_cross_wave_kernels_3_steps(lib):
+    st, (vor, phi, u00, w00), box, c = _run_pair(1, 3, grid=(192, 33, 128))
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
 def _run_pair(explicit, nstep, grid=(48, 49, 48), Re=1000.0, readflag=1, nhist=1):
     vor, phi, u00, w00 = box_ic(*grid)
     box = _box(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist)
