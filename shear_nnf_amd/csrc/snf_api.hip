@@ -306,7 +306,7 @@ ScalarArgs sargs(snf_handle* h) {
   a.explicit_visc = h->p.explicit_visc;
   a.zb_rows = h->wave_xz ? 1 : h->zb;
   a.upart = h->wave_xz ? h->upart : nullptr;
-  a.nupart = h->wave_xz ? (long)h->g.my * h->g.mgalz : 0;
+  a.nupart = h->wave_xz ? (long)h->g.my * (h->g.mgalz / WB_ROWS) : 0;
   a.cflx = h->cflx;
   a.cfly = h->cfly;
   a.cflz = h->cflz;

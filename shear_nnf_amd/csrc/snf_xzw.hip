@@ -117,17 +117,36 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
   const long fstride = (long)g.my * kxn * Nz;
   const long rowbase = (long)j * kxn * Nz + z0;   // + kx*Nz + r
   // --- load the 3 field pairs of row r in lane layout (bins l + 64 m) ---
+  // every load of the block is issued up front (runs of 8 z values, 128 B);
+  // the tile then passes through LDS one field pair at a time
+  constexpr int KXM = 64 * RX / 3;                                  // kxn bound
+  constexpr int NE = (6 * KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);
+  const int tile = kxn * WB_ROWS;                                   // per field
+  cplx pre[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = threadIdx.x + q * 64 * WB_ROWS;
+    if (e < 6 * tile) {
+      const int f = e / tile, rem = e - f * tile;
+      const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
+      pre[q] = gld(hin + f * fstride + rowbase + (long)kx * Nz + rr);
+    }
+  }
   cplx zin[3][RX];
   cplx* S = work;                        // [2][kxn][WB_ROWS + 1]
   const int SP = WB_ROWS + 1;
 #pragma unroll
   for (int p = 0; p < 3; ++p) {
     __syncthreads();
-    for (int e = threadIdx.x; e < 2 * kxn * WB_ROWS; e += blockDim.x) {
-      const int h = e / (kxn * WB_ROWS);
-      const int rem = e - h * kxn * WB_ROWS;
-      const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
-      S[(h * kxn + kx) * SP + rr] = gld(hin + (2 * p + h) * fstride + rowbase + (long)kx * Nz + rr);
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int e = threadIdx.x + q * 64 * WB_ROWS;
+      const int f = e / tile;
+      if (e < 6 * tile && (f >> 1) == p) {
+        const int rem = e - f * tile;
+        const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
+        S[((f & 1) * kxn + kx) * SP + rr] = pre[q];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -220,12 +239,19 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
       m2 = fmax(m2, __shfl_xor(m2, o, 64));
       m3 = fmax(m3, __shfl_xor(m3, o, 64));
     }
+    __shared__ double um[WB_ROWS][4];
     if (lane == 0) {
-      double* up = upart + ((long)j * Nz + z0 + r) * 4;
-      gstd(up, m0);
-      gstd(up + 1, m1);
-      gstd(up + 2, m2);
-      gstd(up + 3, m3);
+      um[wave][0] = m0;
+      um[wave][1] = m1;
+      um[wave][2] = m2;
+      um[wave][3] = m3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {   // one partial per block
+      double m = 0.0;
+#pragma unroll
+      for (int w = 0; w < WB_ROWS; ++w) m = fmax(m, um[w][threadIdx.x]);
+      gstd(upart + ((long)j * gridDim.x + zblk) * 4 + threadIdx.x, m);
     }
   }
   if (ihist) {
