@@ -19,6 +19,7 @@
 #include "snf_xzw.hip"
 #include "snf_scalar.hip"
 #include "snf_ykern.hip"
+#include "snf_yfk.hip"
 
 namespace {
 
@@ -100,6 +101,10 @@ struct snf_handle {
   int rx = 0, rz = 0;
   size_t ldsWA = 0, ldsWB = 0, ldsWC = 0;
   double* upart = nullptr;   // per-row umax partials of the wave x-row kernel
+  // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
+  bool yfft = false;
+  cplx* twy = nullptr;
+  size_t ldsY = 0;
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
   // kernel timing
   bool timing = false;
@@ -176,6 +181,7 @@ YArgs yargs(snf_handle* h) {
   a.dwdy = h->dwdy;
   a.sc = h->sc;
   a.PS = h->PS;
+  a.twy = h->twy;
   return a;
 }
 
@@ -325,6 +331,16 @@ ScalarArgs sargs(snf_handle* h) {
   return a;
 }
 
+void launch_yf(snf_handle* h, int rk) {
+  const unsigned nb = (unsigned)((h->P + YF_LINES - 1) / YF_LINES + 1);
+  const YArgs a = yargs(h);
+  switch (h->g.my) {
+    case 33: k_yf_explicit<3, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    case 49: k_yf_explicit<7, 7><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    default: k_yf_explicit<5, 7, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+  }
+}
+
 int launch_twist(snf_handle* h, int which) {
   dim3 gt(h->g.kxn, 2);
   k_twist<<<gt, 64, 0, h->stream>>>(h->g.my, h->b1, h->b2, h->c, h->shp[which], h->tw[which],
@@ -341,6 +357,9 @@ unsigned ylines_blocks(const snf_handle* h) { return (unsigned)((h->P + 2 + 127)
 extern "C" {
 
 const char* snf_last_error(void) { return g_err.c_str(); }
+
+// line lengths my with a compiled spectral y plan (radices of the passes)
+static bool y_plan(int my) { return my == 33 || my == 49 || my == 385; }
 
 int snf_create(const snf_params* p, snf_handle** out) {
   if (!p || !out) return fail(SNF_BAD_PARAM, "null argument");
@@ -463,6 +482,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->stat_plane, (size_t)my * STAT_W));
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
   TRY(A(&h->upart, (size_t)my * mgalz * 4));
+  TRY(A(&h->twy, (size_t)my));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
   TRY(A(&h->ring, (size_t)h->cap * 24));
@@ -496,6 +516,21 @@ int snf_create(const snf_params* p, snf_handle** out) {
     s.dtimag = p->dtimag;
     TRY(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy scalars"));
+  }
+  {
+    const char* e = getenv("SNF_Y_PATH");
+    h->yfft = p->explicit_visc == 1 && y_plan(my) && !(e && std::string(e) == "banded");
+    std::vector<cplx> ty(my);
+    for (int t = 0; t < my; ++t) {
+      const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / my;
+      ty[t] = mkc((double)cosl(a), (double)sinl(a));
+    }
+    TRY(hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
+            ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y twiddles"));
+    h->ldsY = (size_t)my * 2 * YF_LINES * sizeof(cplx);   // tile + one column per wave
+    for (const void* ky : {(const void*)k_yf_explicit<3, 11>, (const void*)k_yf_explicit<7, 7>,
+                           (const void*)k_yf_explicit<5, 7, 11>})
+      (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
   // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning).
@@ -712,7 +747,9 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
       }
       {
         TimedLaunch tl(h, SNF_K_Y);
-        if (h->p.explicit_visc == 1)
+        if (h->p.explicit_visc == 1 && h->yfft)
+          launch_yf(h, rk);
+        else if (h->p.explicit_visc == 1)
           k_y_explicit<<<(unsigned)((2 * h->P + 2 + 127) / 128), 128, 0, h->stream>>>(yargs(h), rk);
         else
           k_y_implicit<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h), rk);

@@ -1,0 +1,193 @@
+// Spectral application of the compact y-operators on twisted-periodic lines.
+//
+// The compact schemes of diffy_shear_hepta.f90 are constant-coefficient
+// stencils B x = R u on a line whose extension is u(j + N) = shp u(j)
+// (derivyc 358-383, lapcdy 487-514).  Such a twisted-cyclic operator is
+// diagonal in the twisted Fourier modes u_j = exp(i w j), w = (theta + 2 pi m)/N,
+// theta = arg shp: it multiplies mode m by
+//     d/dy   : R1(w) / B1(w)                  (rh1, pr1)
+//     d2/dy2 : R2(w) / B2(w)                  (rh2, pr2)
+//     lapcdy : B2(w) / (R2(w) - k2 B2(w))     ((rh2 - k2 pr2) p = pr2 u)
+// with C(w) = sum_p c(p) exp(i p w), p = -3..3.  The result equals the
+// reference's cyclic L D L^H solve (modChol7cyclic_cmplx 578-755,
+// solve7cyclic 900-954) up to rounding.  So a line is transformed once
+// (x -> X_m = sum_j x_j exp(-i w_m j)), multiplied, and transformed back.
+//
+// Transforms: one wave per line, mixed-radix Stockham passes (radices 2, 3,
+// 4, 5, 7, 8, 11) in place on the line's column of an LDS tile, every pass a
+// read-all / write-all of the wave's own column (no block barrier).
+// Twiddles from the table rt[t] = exp(-2 pi i t / N) in global memory.
+#pragma once
+#include "snf_internal.h"
+#include "snf_wfft.h"
+
+#define YF_LINES 4        // lines (waves) per block; tile rows of 4 lines = 64 B
+
+// Compile-time transform plan: the radices of the passes, in order.  A plan
+// fixes N = prod(Rs) and the number of rows per lane Q = ceil(N / 64).
+template <int... Rs> struct YPlanT;
+template <> struct YPlanT<> {
+  static constexpr int N = 1;
+};
+template <int R, int... Rs> struct YPlanT<R, Rs...> {
+  static constexpr int N = R * YPlanT<Rs...>::N;
+};
+
+// Column accessors.  TileCol: line c of the block's transfer tile, rows of 4
+// lines with the line index XORed with (j >> 2) & 3 so that a wave walking
+// one line down the rows touches distinct LDS banks.  OwnCol: a wave's
+// private contiguous column.
+struct TileCol {
+  int c;
+  __device__ __forceinline__ int operator()(int j) const { return j * YF_LINES + (c ^ ((j >> 2) & 3)); }
+};
+struct OwnCol {
+  int base;
+  __device__ __forceinline__ int operator()(int j) const { return base + j; }
+};
+
+// ---- small odd DFTs by symmetric pairs: y_k = sum_j x_j exp(SIGN 2 pi i jk/R)
+template <int R> struct OddRoots;
+template <> struct OddRoots<5> {
+  static constexpr double c[5] = {1.0, 0.3090169943749474241023, -0.8090169943749474241023,
+                                  -0.8090169943749474241023, 0.3090169943749474241023};
+  static constexpr double s[5] = {0.0, 0.9510565162951535721164, 0.5877852522924731291687,
+                                  -0.5877852522924731291687, -0.9510565162951535721164};
+};
+template <> struct OddRoots<7> {
+  static constexpr double c[7] = {1.0, 0.6234898018587335305250, -0.2225209339563144042889,
+                                  -0.9009688679024191262361, -0.9009688679024191262361,
+                                  -0.2225209339563144042889, 0.6234898018587335305250};
+  static constexpr double s[7] = {0.0, 0.7818314824680298087084, 0.9749279121818236070181,
+                                  0.4338837391175581204758, -0.4338837391175581204758,
+                                  -0.9749279121818236070181, -0.7818314824680298087084};
+};
+template <> struct OddRoots<11> {
+  static constexpr double c[11] = {1.0, 0.8412535328311811688618, 0.4154150130018864255293,
+                                   -0.1423148382732851404438, -0.6548607339452850640569,
+                                   -0.9594929736144973898904, -0.9594929736144973898904,
+                                   -0.6548607339452850640569, -0.1423148382732851404438,
+                                   0.4154150130018864255293, 0.8412535328311811688618};
+  static constexpr double s[11] = {0.0, 0.5406408174555975821076, 0.9096319953545183714117,
+                                   0.9898214418809327323761, 0.7557495743542582837740,
+                                   0.2817325568414296977114, -0.2817325568414296977114,
+                                   -0.7557495743542582837740, -0.9898214418809327323761,
+                                   -0.9096319953545183714117, -0.5406408174555975821076};
+};
+
+template <int R, int SIGN>
+__host__ __device__ __forceinline__ void yf_dft_odd(cplx* v) {
+  constexpr int H = (R - 1) / 2;
+  cplx a[H + 1], b[H + 1];
+  cplx y0 = v[0];
+#pragma unroll
+  for (int j = 1; j <= H; ++j) {
+    a[j] = wf_add(v[j], v[R - j]);
+    b[j] = wf_sub(v[j], v[R - j]);
+    y0 = wf_add(y0, a[j]);
+  }
+  const cplx x0 = v[0];
+  v[0] = y0;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    cplx re = x0, im = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const int t = (j * k) % R;
+      const double cc = OddRoots<R>::c[t], ss = OddRoots<R>::s[t];
+      re.x += cc * a[j].x;
+      re.y += cc * a[j].y;
+      im.x += ss * b[j].x;
+      im.y += ss * b[j].y;
+    }
+    const cplx si = wf_si<SIGN>(im);   // SIGN * i * im
+    v[k] = wf_add(re, si);
+    v[R - k] = wf_sub(re, si);
+  }
+}
+
+template <int R, int SIGN>
+__host__ __device__ __forceinline__ void yf_dft(cplx* v) {
+  if (R == 5 || R == 7 || R == 11)
+    yf_dft_odd<(R == 5 ? 5 : (R == 7 ? 7 : 11)), SIGN>(v);
+  else
+    wf_dft<(R == 2 ? 2 : (R == 3 ? 3 : (R == 4 ? 4 : 8))), SIGN>(v);
+}
+
+// one Stockham pass on column w of the tile: inputs x[b + r nb], outputs
+// (b/Ns) Ns R + b%Ns + r Ns, twiddle exp(SIGN 2 pi i r (b%Ns) / (Ns R))
+template <int R, int SIGN, int N, int Ns, class Col>
+__device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* rt) {
+  constexpr int nb = N / R;
+  constexpr int MB = (nb + 63) / 64;       // butterflies per lane
+  constexpr int stw = N / (Ns * R);
+  const int lane = threadIdx.x & 63;
+  cplx v[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int b = lane + 64 * q;
+    if (b < nb) {
+      const int k = b % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        cplx x = T[col(b + r * nb)];
+        if (r > 0) x = wf_tw<SIGN>(x, gld(rt + r * k * stw));
+        v[q][r] = x;
+      }
+    }
+  }
+  wf_sync();
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int b = lane + 64 * q;
+    if (b < nb) {
+      const int k = b % Ns;
+      yf_dft<R, SIGN>(v[q]);
+      const int o = (b / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) T[col(o + r * Ns)] = v[q][r];
+    }
+  }
+  wf_sync();
+}
+
+// transform of column w in place (natural order in and out), unnormalised
+template <int SIGN, int N, int Ns, class Col>
+__device__ __forceinline__ void yf_passes(cplx*, Col, const cplx*) {}
+template <int SIGN, int N, int Ns, class Col, int R, int... Rs>
+__device__ __forceinline__ void yf_passes(cplx* T, Col col, const cplx* rt) {
+  yf_pass<R, SIGN, N, Ns>(T, col, rt);
+  yf_passes<SIGN, N, Ns * R, Col, Rs...>(T, col, rt);
+}
+template <int SIGN, class Col, int... Rs>
+__device__ __forceinline__ void yf_fft(cplx* T, Col col, const cplx* rt) {
+  wf_sync();   // the column was just written lane by lane
+  yf_passes<SIGN, YPlanT<Rs...>::N, 1, Col, Rs...>(T, col, rt);
+}
+
+// seven stencil coefficients held in registers
+struct C7 {
+  double c[7];
+};
+__device__ __forceinline__ C7 c7_of(const double (&a)[7]) {
+  C7 r;
+#pragma unroll
+  for (int m = 0; m < 7; ++m) r.c[m] = a[m];
+  return r;
+}
+
+// C(w) = sum_{p=-3..3} c[3+p] exp(i p w) from e = exp(i w)
+__device__ __forceinline__ cplx yf_symbol(const C7& k, cplx e1) {
+  const double* c = k.c;
+  const cplx e2 = cmul(e1, e1), e3 = cmul(e2, e1);
+  const double c1 = c[4] + c[2], s1 = c[4] - c[2];
+  const double c2 = c[5] + c[1], s2 = c[5] - c[1];
+  const double c3 = c[6] + c[0], s3 = c[6] - c[0];
+  return mkc(c[3] + c1 * e1.x + c2 * e2.x + c3 * e3.x, s1 * e1.y + s2 * e2.y + s3 * e3.y);
+}
+
+// a / b
+__device__ __forceinline__ cplx yf_div(cplx a, cplx b) {
+  const double d = 1.0 / (b.x * b.x + b.y * b.y);
+  return mkc((a.x * b.x + a.y * b.y) * d, (a.y * b.x - a.x * b.y) * d);
+}

@@ -1,0 +1,409 @@
+// Explicit RK3 substep of the y stage (cross.f90:565-796) with the compact
+// y-operators applied spectrally (snf_yfft.h): one wave per (kx,kz) line,
+// YF_LINES lines per block.  The block moves every field through an LDS tile
+// of [my][YF_LINES] (coalesced rows of the [y][kx][kz] layout); each wave
+// transforms its line in a private LDS column (and, for a second spectrum,
+// in its column of the tile).
+//
+// Per line, with the old twist theta (shp at the start of the substep) and
+// the new twist theta' (after the shift update):
+//   omega_y chain (cross.f90:652-781):
+//     hg  = rhg + nu (D2 - k2) vor           (visc3d; not for the (0,0) mode)
+//     RK update of vor, vorwk, phase maps    -> vor, vorwk
+//     dome = D1(vor)  with theta'            (cross.f90:434-445 of the next stage)
+//   phi chain (cross.f90:607, 652-781):
+//     hv  = rhv - D1(rdome) + nu (D2 - k2) phi
+//     RK update of phi, phiwk, phase maps    -> phi, phiwk
+//     v = lapcdy(phi), dv = D1(v) with theta' (cross.f90:791)
+// Fields read: vor rhg vorwk rdome phi rhv phiwk; written: vor vorwk dome phi
+// phiwk v dv -- each once.  The real x-z means U, W (00 lines, m=1 calls)
+// take one extra block with one wave each.
+#include "snf_yfft.h"
+
+__device__ __forceinline__ cplx yf_expi(double a) {
+  double s, c;
+  sincos(a, &s, &c);
+  return mkc(c, s);
+}
+
+// exp(i (a j + a0)) along the rows j = lane + 64 q of a lane: base * step^q
+struct YPhase {
+  cplx p, st;
+  __device__ YPhase(double a, double a0, int lane) {
+    p = yf_expi(a * (double)lane + a0);
+    st = yf_expi(a * 64.0);
+  }
+  __device__ cplx next() {
+    const cplx r = p;
+    p = cmul(p, st);
+    return r;
+  }
+};
+
+// block-collective transfers of field F (stride P) for lines L0..L0+YF_LINES-1
+__device__ __forceinline__ void yf_fill(cplx* T, const cplx* F, long P, long L0, int N) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    const long L = L0 + c;
+    T[TileCol{c}(j)] = (L < P) ? gld(F + (long)j * P + L) : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void yf_flush(const cplx* T, cplx* F, long P, long L0, int N) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    const long L = L0 + c;
+    if (L < P) gst(F + (long)j * P + L, T[TileCol{c}(j)]);
+  }
+  __syncthreads();
+}
+
+// exp(i w_m) of mode m: exp(2 pi i m/N) exp(i theta/N)
+__device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
+  return cmul(cconj(gld(rt + m)), eth);
+}
+
+template <int... Rs>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_explicit(YArgs a, int rk) {
+  constexpr int N = YPlanT<Rs...>::N, Q = (N + 63) / 64;
+  if (a.sc->halted) return;
+  extern __shared__ cplx T[];
+  const Geom& g = a.g;
+  const long P = g.P;
+  const cplx* rt = a.twy;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long nbl = (P + YF_LINES - 1) / YF_LINES;
+  const int ir = rk - 1;
+  double dt, f1, f2;
+  substep_factors(a.sc, g.s, rk, dt, f1, f2);
+  const double fac = 1.0 / g.Re;
+  const double gam = c_gama[ir], zet = c_zeta[ir];
+  const double invN = 1.0 / (double)N;
+  const C7 rh1 = c7_of(a.c.rh1), pr1 = c7_of(a.c.pr1), rh2 = c7_of(a.c.rh2), pr2 = c7_of(a.c.pr2);
+  const TileCol tc{w};
+  const OwnCol ac{YF_LINES * N + w * N};
+  cplx x[Q];
+
+  if (blockIdx.x == nbl) {
+    // ---- 00 lines: U (wave 0) and W (wave 1), real, untwisted (m=1 calls)
+    if (w >= 2) return;
+    cplx* u = w == 0 ? a.u00c : a.w00c;
+    cplx* uwk = w == 0 ? a.u00wk : a.w00wk;
+    const double* rf = w == 0 ? a.rf0u : a.rf0w;
+    double* dd = w == 0 ? a.dudy : a.dwdy;
+    const cplx one = mkc(1.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) {
+        x[q] = gld(u + j);
+        T[ac(j)] = x[q];
+      }
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        const cplx e1 = yf_mode(rt, m, one);
+        T[ac(m)] = cmul(T[ac(m)], yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1)));
+      }
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) {
+        const double rfj = gldd(rf + j) + fac * (T[ac(j)].x * invN);
+        double wkj = (rk == 1) ? gam * rfj : gld(uwk + j).x + gam * rfj;
+        const double un = x[q].x + dt * wkj;
+        if (rk != 3) wkj = rfj * zet;
+        gst(u + j, mkc(un, 0.0));
+        gst(uwk + j, mkc(wkj, 0.0));
+        T[ac(j)] = mkc(un, 0.0);
+      }
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        const cplx e1 = yf_mode(rt, m, one);
+        T[ac(m)] = cmul(T[ac(m)], yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1)));
+      }
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) gstd(dd + j, T[ac(j)].x * invN);
+    }
+    return;
+  }
+
+  // ---- (kx, kz) lines; every wave takes part in every block transfer.
+  // Neighbouring blocks share the 128-byte rows of the field: keep them on
+  // one XCD.
+  const long L0 = YF_LINES * xcd_block(blockIdx.x, nbl);
+  const long L = L0 + w;
+  int i, k;
+  double kx, k2;
+  line_geom(g, L < P ? L : P - 1, i, k, kx, k2);
+  const bool is00 = (L == 0);
+  const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
+  const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
+  const cplx etho = yf_expi(tho), ethn = yf_expi(thn);
+
+  // ===== omega_y chain =====
+  yf_fill(T, a.vor, P, L0, N);
+  {
+    YPhase ph(-tho, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) {
+        x[q] = T[tc(j)];
+        T[ac(j)] = cmul(x[q], e);
+      }
+    }
+  }
+  yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx e1 = yf_mode(rt, m, etho);
+      const cplx s2 = yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1));
+      const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
+      T[ac(m)] = cmul(T[ac(m)], mul);
+    }
+  }
+  yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+  yf_fill(T, a.rhg, P, L0, N);
+  {
+    YPhase ph(tho, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hg
+    }
+  }
+  if (rk > 1) yf_fill(T, a.vorwk, P, L0, N);
+  {
+    YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
+    YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx p1 = m1.next(), p2 = m2.next();
+      if (j < N) {
+        const cplx hgj = T[ac(j)];
+        cplx vwk = (rk == 1) ? crmul(gam, hgj) : cadd(T[tc(j)], crmul(gam, hgj));
+        const cplx vn = cadd(x[q], crmul(dt, vwk));
+        if (rk != 3) vwk = cmul(crmul(zet, hgj), p1);
+        x[q] = cmul(vn, p2);
+        T[tc(j)] = vwk;
+      }
+    }
+  }
+  yf_flush(T, a.vorwk, P, L0, N);
+  {
+    YPhase ph(-thn, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) {
+        T[tc(j)] = x[q];
+        T[ac(j)] = cmul(x[q], e);
+      }
+    }
+  }
+  yf_flush(T, a.vor, P, L0, N);
+  yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx e1 = yf_mode(rt, m, ethn);
+      const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+      T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
+    }
+  }
+  yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+  {
+    YPhase ph(thn, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
+    }
+  }
+  yf_flush(T, a.dome, P, L0, N);
+
+  // ===== phi chain =====
+  yf_fill(T, a.rdome, P, L0, N);
+  {
+    YPhase ph(-tho, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) T[ac(j)] = cmul(T[tc(j)], e);
+    }
+  }
+  yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx e1 = yf_mode(rt, m, etho);
+      const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+      T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
+    }
+  }
+  yf_fill(T, a.phi, P, L0, N);
+  {
+    YPhase ph(-tho, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) {
+        x[q] = T[tc(j)];
+        T[tc(j)] = cmul(x[q], e);
+      }
+    }
+  }
+  yf_fft<-1, TileCol, Rs...>(T, tc, rt);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx e1 = yf_mode(rt, m, etho);
+      const cplx s2 = yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1));
+      const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
+      T[ac(m)] = cadd(T[ac(m)], cmul(T[tc(m)], mul));
+    }
+  }
+  yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+  yf_fill(T, a.rhv, P, L0, N);
+  {
+    YPhase ph(tho, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hv
+    }
+  }
+  if (rk > 1) yf_fill(T, a.phiwk, P, L0, N);
+  {
+    YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
+    YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx p1 = m1.next(), p2 = m2.next();
+      if (j < N) {
+        const cplx hvj = T[ac(j)];
+        cplx pwk = (rk == 1) ? crmul(gam, hvj) : cadd(T[tc(j)], crmul(gam, hvj));
+        const cplx pn = cadd(x[q], crmul(dt, pwk));
+        if (rk != 3) pwk = cmul(crmul(zet, hvj), p1);
+        x[q] = cmul(pn, p2);
+        T[tc(j)] = pwk;
+      }
+    }
+  }
+  yf_flush(T, a.phiwk, P, L0, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = lane + 64 * q;
+    if (j < N) T[tc(j)] = x[q];
+  }
+  yf_flush(T, a.phi, P, L0, N);
+  // v = lapcdy(phi) and dv = D1(v) with the new twist; the (0,0) mode keeps
+  // its v (lapcdy is not called for it) and only takes dv = D1(v)
+  if (L0 == 0) yf_fill(T, a.v, P, L0, N);
+  if (!is00) {
+    {
+      YPhase ph(-thn, 0.0, lane);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        const cplx e = ph.next();
+        if (j < N) T[ac(j)] = cmul(x[q], e);
+      }
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        const cplx e1 = yf_mode(rt, m, ethn);
+        const cplx R2 = yf_symbol(rh2, e1), B2 = yf_symbol(pr2, e1);
+        const cplx H = yf_div(crmul(invN, B2), csub(R2, crmul(k2, B2)));
+        const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+        const cplx vh = cmul(T[ac(m)], H);
+        T[tc(m)] = cmul(vh, s1);
+        T[ac(m)] = vh;
+      }
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    yf_fft<1, TileCol, Rs...>(T, tc, rt);
+    YPhase ph(thn, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) {
+        x[q] = cmul(T[ac(j)], e);              // v
+        T[tc(j)] = cmul(T[tc(j)], e);          // dv
+      }
+    }
+  } else {
+    {
+      YPhase ph(-thn, 0.0, lane);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        const cplx e = ph.next();
+        if (j < N) {
+          x[q] = T[tc(j)];                     // v of the (0,0) mode, unchanged
+          T[ac(j)] = cmul(x[q], e);
+        }
+      }
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        const cplx e1 = yf_mode(rt, m, ethn);
+        const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+        T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
+      }
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    YPhase ph(thn, 0.0, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.next();
+      if (j < N) T[tc(j)] = cmul(T[ac(j)], e);   // dv
+    }
+  }
+  yf_flush(T, a.dv, P, L0, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = lane + 64 * q;
+    if (j < N) T[tc(j)] = x[q];
+  }
+  yf_flush(T, a.v, P, L0, N);
+}

@@ -12,6 +12,7 @@
 // (m=1 calls of derivyc / visc3d / lapcdy).
 #include "snf_internal.h"
 #include "snf_lines.h"
+#include "snf_yfft.h"
 
 // ---------------------------------------------------------------------------
 // two twisted derivative operators of the same line advanced in lock step
@@ -170,6 +171,7 @@ struct YArgs {
   double *dudy, *dwdy;
   DevScalars* sc;
   long PS;
+  const cplx* twy;                          // spectral y path: exp(-2 pi i t/my), t < my
 };
 
 // lap matrix rh2 - k2 pr2 (lapcdy, diffy_shear_hepta.f90:475)

@@ -160,6 +160,32 @@ def test_cross_wave_kernels_3_steps(lib):
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
 
+# explicit substeps on my = 33, 49, 385 take the spectral y kernels (line
+# transforms of 3*11, 7*7, 5*7*11 points); other my the banded sweeps
+@pytest.mark.parametrize("grid", [(32, 33, 32), (48, 385, 48)])
+def test_cross_spectral_y_3_steps(lib, grid):
+    st, (vor, phi, u00, w00), box, c = _run_pair(1, 3, grid=grid)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+def test_spectral_and_banded_y_paths_agree(lib, monkeypatch):
+    grid = (48, 49, 48)
+    vor, phi, u00, w00 = box_ic(*grid)
+    outs = []
+    for path in ("spectral", "banded"):
+        monkeypatch.setenv("SNF_Y_PATH", path)
+        box = _box(*grid, explicit=1)
+        box.set_state(vor, phi, u00, w00)
+        box.cross(5)
+        outs.append(box.get_state())
+    for a, b in zip(*outs):
+        assert rel_l2(a, b) < STEP_TOL
+
+
 def _run_pair(explicit, nstep, grid=(48, 49, 48), Re=1000.0, readflag=1, nhist=1):
     vor, phi, u00, w00 = box_ic(*grid)
     box = _box(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist)
