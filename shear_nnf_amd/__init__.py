@@ -19,7 +19,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libshear_rk3.so")
+# SNF_LIB selects another build of the same library (A/B timing of variants)
+LIB_PATH = os.environ.get("SNF_LIB") or os.path.join(_HERE, "libshear_rk3.so")
 
 SNF_OK, SNF_BAD_PARAM, SNF_DEVICE_ERROR, SNF_NO_DEVICE, SNF_NOT_READY = 0, 1, 2, 3, 4
 SNF_RUNNING, SNF_STOP, SNF_DUMP = 0, 1, 2

@@ -104,6 +104,7 @@ struct snf_handle {
   // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
   bool yfft = false;
   cplx* twy = nullptr;
+  cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
   size_t ldsY = 0;
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
   // kernel timing
@@ -182,6 +183,8 @@ YArgs yargs(snf_handle* h) {
   a.sc = h->sc;
   a.PS = h->PS;
   a.twy = h->twy;
+  a.ysym_old = h->ysym[h->cur];
+  a.ysym_new = h->ysym[1 - h->cur];
   return a;
 }
 
@@ -345,6 +348,10 @@ int launch_twist(snf_handle* h, int which) {
   dim3 gt(h->g.kxn, 2);
   k_twist<<<gt, 64, 0, h->stream>>>(h->g.my, h->b1, h->b2, h->c, h->shp[which], h->tw[which],
                                     h->g.kxn, h->sc);
+  if (h->yfft) {
+    dim3 gs(h->g.kxn, (h->g.my + 255) / 256);
+    k_ysym<<<gs, 256, 0, h->stream>>>(h->g.my, h->c, h->shp[which], h->twy, h->ysym[which], h->sc);
+  }
   HIPCHK(hipGetLastError());
   return SNF_OK;
 }
@@ -483,6 +490,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
   TRY(A(&h->upart, (size_t)my * mgalz * 4));
   TRY(A(&h->twy, (size_t)my));
+  for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * 2));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
   TRY(A(&h->ring, (size_t)h->cap * 24));
@@ -527,7 +535,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
     TRY(hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y twiddles"));
-    h->ldsY = (size_t)my * 2 * YF_LINES * sizeof(cplx);   // tile + one column per wave
+    h->ldsY = (size_t)my * (2 * YF_LINES + 1) * sizeof(cplx);   // tile, a column per wave, roots
     for (const void* ky : {(const void*)k_yf_explicit<3, 11>, (const void*)k_yf_explicit<7, 7>,
                            (const void*)k_yf_explicit<5, 7, 11>})
       (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);
@@ -598,7 +606,8 @@ int snf_destroy(snf_handle* h) {
                   h->hyb_in, h->hyb_out, h->hr, h->hc, h->u00c, h->w00c, h->u00wk, h->w00wk,
                   h->u00r, h->w00r, h->rf0u, h->rf0w, h->dudy, h->dwdy, h->base, h->tw[0],
                   h->tw[1], h->shp[0], h->shp[1], h->shm[0], h->shm[1], h->twx, h->twz, h->sc,
-                  h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring};
+                  h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring,
+                  h->upart, h->twy, h->ysym[0], h->ysym[1]};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);

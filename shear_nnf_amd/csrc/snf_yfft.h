@@ -16,7 +16,7 @@
 // Transforms: one wave per line, mixed-radix Stockham passes (radices 2, 3,
 // 4, 5, 7, 8, 11) in place on the line's column of an LDS tile, every pass a
 // read-all / write-all of the wave's own column (no block barrier).
-// Twiddles from the table rt[t] = exp(-2 pi i t / N) in global memory.
+// Twiddles from the table rt[t] = exp(-2 pi i t / N), staged in LDS.
 #pragma once
 #include "snf_internal.h"
 #include "snf_wfft.h"
@@ -131,7 +131,7 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* rt) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         cplx x = T[col(b + r * nb)];
-        if (r > 0) x = wf_tw<SIGN>(x, gld(rt + r * k * stw));
+        if (r > 0) x = wf_tw<SIGN>(x, rt[r * k * stw]);
         v[q][r] = x;
       }
     }

@@ -65,6 +65,24 @@ __device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
   return cmul(cconj(gld(rt + m)), eth);
 }
 
+// Multipliers of the twisted d/dy and d2/dy2 for every kx and mode m at the
+// twist shp[kx]: out[(kx N + m) 2 + {0,1}] = R1/B1, R2/B2 at
+// w_m = (2 pi m + arg shp)/N.  (The Helmholtz multiplier of a line is
+// 1/(R2/B2 - k2).)
+__global__ void __launch_bounds__(256)
+k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt,
+       cplx* __restrict__ out, const DevScalars* sc) {
+  if (sc && sc->halted) return;
+  const int kx = blockIdx.x, m = blockIdx.y * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  const cplx t = gld(shp + kx);
+  const cplx e1 = yf_mode(rt, m, yf_expi(atan2(t.y, t.x) / (double)N));
+  const C7 rh1 = c7_of(c.rh1), pr1 = c7_of(c.pr1), rh2 = c7_of(c.rh2), pr2 = c7_of(c.pr2);
+  cplx* o = out + ((long)kx * N + m) * 2;
+  gst(o, yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1)));
+  gst(o + 1, yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1)));
+}
+
 template <int... Rs>
 __global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
 k_yf_explicit(YArgs a, int rk) {
@@ -73,16 +91,18 @@ k_yf_explicit(YArgs a, int rk) {
   extern __shared__ cplx T[];
   const Geom& g = a.g;
   const long P = g.P;
-  const cplx* rt = a.twy;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long nbl = (P + YF_LINES - 1) / YF_LINES;
+  // LDS: tile [N][YF_LINES], one column per wave, the N-th roots
+  cplx* rt = T + 2 * YF_LINES * N;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.twy + t);
+  __syncthreads();
   const int ir = rk - 1;
   double dt, f1, f2;
   substep_factors(a.sc, g.s, rk, dt, f1, f2);
   const double fac = 1.0 / g.Re;
   const double gam = c_gama[ir], zet = c_zeta[ir];
   const double invN = 1.0 / (double)N;
-  const C7 rh1 = c7_of(a.c.rh1), pr1 = c7_of(a.c.pr1), rh2 = c7_of(a.c.rh2), pr2 = c7_of(a.c.pr2);
   const TileCol tc{w};
   const OwnCol ac{YF_LINES * N + w * N};
   cplx x[Q];
@@ -94,7 +114,7 @@ k_yf_explicit(YArgs a, int rk) {
     cplx* uwk = w == 0 ? a.u00wk : a.w00wk;
     const double* rf = w == 0 ? a.rf0u : a.rf0w;
     double* dd = w == 0 ? a.dudy : a.dwdy;
-    const cplx one = mkc(1.0, 0.0);
+    const cplx* sy = a.ysym_new;   // kx = 0 row: untwisted
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -108,8 +128,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx e1 = yf_mode(rt, m, one);
-        T[ac(m)] = cmul(T[ac(m)], yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1)));
+        T[ac(m)] = cmul(T[ac(m)], gld(sy + 2 * m + 1));
       }
     }
     yf_fft<1, OwnCol, Rs...>(T, ac, rt);
@@ -131,8 +150,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx e1 = yf_mode(rt, m, one);
-        T[ac(m)] = cmul(T[ac(m)], yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1)));
+        T[ac(m)] = cmul(T[ac(m)], gld(sy + 2 * m));
       }
     }
     yf_fft<1, OwnCol, Rs...>(T, ac, rt);
@@ -155,7 +173,8 @@ k_yf_explicit(YArgs a, int rk) {
   const bool is00 = (L == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  const cplx etho = yf_expi(tho), ethn = yf_expi(thn);
+  const cplx* syo = a.ysym_old + (long)i * N * 2;   // d/dy, d2/dy2 at the old twist
+  const cplx* syn = a.ysym_new + (long)i * N * 2;   // ... at the new twist
 
   // ===== omega_y chain =====
   yf_fill(T, a.vor, P, L0, N);
@@ -176,8 +195,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx e1 = yf_mode(rt, m, etho);
-      const cplx s2 = yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1));
+      const cplx s2 = gld(syo + 2 * m + 1);
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
@@ -230,8 +248,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx e1 = yf_mode(rt, m, ethn);
-      const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+      const cplx s1 = gld(syn + 2 * m);
       T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
     }
   }
@@ -263,8 +280,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx e1 = yf_mode(rt, m, etho);
-      const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+      const cplx s1 = gld(syo + 2 * m);
       T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
     }
   }
@@ -286,8 +302,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx e1 = yf_mode(rt, m, etho);
-      const cplx s2 = yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1));
+      const cplx s2 = gld(syo + 2 * m + 1);
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cadd(T[ac(m)], cmul(T[tc(m)], mul));
     }
@@ -346,10 +361,8 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx e1 = yf_mode(rt, m, ethn);
-        const cplx R2 = yf_symbol(rh2, e1), B2 = yf_symbol(pr2, e1);
-        const cplx H = yf_div(crmul(invN, B2), csub(R2, crmul(k2, B2)));
-        const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+        const cplx s1 = gld(syn + 2 * m), s2 = gld(syn + 2 * m + 1);
+        const cplx H = yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y));   // lapcdy
         const cplx vh = cmul(T[ac(m)], H);
         T[tc(m)] = cmul(vh, s1);
         T[ac(m)] = vh;
@@ -385,8 +398,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx e1 = yf_mode(rt, m, ethn);
-        const cplx s1 = yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1));
+        const cplx s1 = gld(syn + 2 * m);
         T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
       }
     }

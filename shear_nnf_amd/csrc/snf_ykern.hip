@@ -172,6 +172,7 @@ struct YArgs {
   DevScalars* sc;
   long PS;
   const cplx* twy;                          // spectral y path: exp(-2 pi i t/my), t < my
+  const cplx *ysym_old, *ysym_new;          // [kxn][my][2] multipliers of d/dy, d2/dy2
 };
 
 // lap matrix rh2 - k2 pr2 (lapcdy, diffy_shear_hepta.f90:475)
