@@ -335,7 +335,7 @@ ScalarArgs sargs(snf_handle* h) {
 }
 
 void launch_yf(snf_handle* h, int rk) {
-  const unsigned nb = (unsigned)((h->P + YF_LINES - 1) / YF_LINES + 1);
+  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   switch (h->g.my) {
     case 33: k_yf_explicit<3, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
@@ -535,7 +535,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
     TRY(hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y twiddles"));
-    h->ldsY = (size_t)my * (2 * YF_LINES + 1) * sizeof(cplx);   // tile, a column per wave, roots
+    // tile, a column per wave, roots, two multiplier rows
+    h->ldsY = (size_t)my * (2 * YF_LINES + 5) * sizeof(cplx);
     for (const void* ky : {(const void*)k_yf_explicit<3, 11>, (const void*)k_yf_explicit<7, 7>,
                            (const void*)k_yf_explicit<5, 7, 11>})
       (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);

@@ -40,22 +40,20 @@ struct YPhase {
   }
 };
 
-// block-collective transfers of field F (stride P) for lines L0..L0+YF_LINES-1
-__device__ __forceinline__ void yf_fill(cplx* T, const cplx* F, long P, long L0, int N) {
+// block-collective transfers of field F (stride P) for lines L0..L0+nv-1
+__device__ __forceinline__ void yf_fill(cplx* T, const cplx* F, long P, long L0, int nv, int N) {
   __syncthreads();
   for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
     const int j = e / YF_LINES, c = e - j * YF_LINES;
-    const long L = L0 + c;
-    T[TileCol{c}(j)] = (L < P) ? gld(F + (long)j * P + L) : mkc(0.0, 0.0);
+    T[TileCol{c}(j)] = (c < nv) ? gld(F + (long)j * P + L0 + c) : mkc(0.0, 0.0);
   }
   __syncthreads();
 }
-__device__ __forceinline__ void yf_flush(const cplx* T, cplx* F, long P, long L0, int N) {
+__device__ __forceinline__ void yf_flush(const cplx* T, cplx* F, long P, long L0, int nv, int N) {
   __syncthreads();
   for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
     const int j = e / YF_LINES, c = e - j * YF_LINES;
-    const long L = L0 + c;
-    if (L < P) gst(F + (long)j * P + L, T[TileCol{c}(j)]);
+    if (c < nv) gst(F + (long)j * P + L0 + c, T[TileCol{c}(j)]);
   }
   __syncthreads();
 }
@@ -92,8 +90,11 @@ k_yf_explicit(YArgs a, int rk) {
   const Geom& g = a.g;
   const long P = g.P;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long nbl = (P + YF_LINES - 1) / YF_LINES;
-  // LDS: tile [N][YF_LINES], one column per wave, the N-th roots
+  // blocks: YF_LINES consecutive kz lines of one kx (nch chunks per kx)
+  const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
+  const long nbl = (long)g.kxn * nch;
+  // LDS: tile [N][YF_LINES], one column per wave, the N-th roots, the
+  // multiplier rows of this block's kx at the old and new twist
   cplx* rt = T + 2 * YF_LINES * N;
   for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.twy + t);
   __syncthreads();
@@ -165,19 +166,25 @@ k_yf_explicit(YArgs a, int rk) {
   // ---- (kx, kz) lines; every wave takes part in every block transfer.
   // Neighbouring blocks share the 128-byte rows of the field: keep them on
   // one XCD.
-  const long L0 = YF_LINES * xcd_block(blockIdx.x, nbl);
-  const long L = L0 + w;
-  int i, k;
-  double kx, k2;
-  line_geom(g, L < P ? L : P - 1, i, k, kx, k2);
-  const bool is00 = (L == 0);
+  const long lb = xcd_block(blockIdx.x, nbl);
+  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
+  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
+  const int k = min(YF_LINES * ch + w, g.mz - 1);
+  const double kx = g.alp * (double)i, kzv = kz_val(k, g.nz, g.mz, g.gam);
+  const double k2 = kx * kx + kzv * kzv;
+  const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  const cplx* syo = a.ysym_old + (long)i * N * 2;   // d/dy, d2/dy2 at the old twist
-  const cplx* syn = a.ysym_new + (long)i * N * 2;   // ... at the new twist
+  cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [m][2]
+  cplx* syn = syo + 2 * N;     // ... at the new twist
+  for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
+    syo[t] = gld(a.ysym_old + (long)i * N * 2 + t);
+    syn[t] = gld(a.ysym_new + (long)i * N * 2 + t);
+  }
 
   // ===== omega_y chain =====
-  yf_fill(T, a.vor, P, L0, N);
+  yf_fill(T, a.vor, P, L0, nv, N);
   {
     YPhase ph(-tho, 0.0, lane);
 #pragma unroll
@@ -195,13 +202,13 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = gld(syo + 2 * m + 1);
+      const cplx s2 = syo[2 * m + 1];
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-  yf_fill(T, a.rhg, P, L0, N);
+  yf_fill(T, a.rhg, P, L0, nv, N);
   {
     YPhase ph(tho, 0.0, lane);
 #pragma unroll
@@ -211,7 +218,7 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hg
     }
   }
-  if (rk > 1) yf_fill(T, a.vorwk, P, L0, N);
+  if (rk > 1) yf_fill(T, a.vorwk, P, L0, nv, N);
   {
     YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
     YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
@@ -229,7 +236,7 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.vorwk, P, L0, N);
+  yf_flush(T, a.vorwk, P, L0, nv, N);
   {
     YPhase ph(-thn, 0.0, lane);
 #pragma unroll
@@ -242,13 +249,13 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.vor, P, L0, N);
+  yf_flush(T, a.vor, P, L0, nv, N);
   yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = gld(syn + 2 * m);
+      const cplx s1 = syn[2 * m];
       T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
     }
   }
@@ -262,10 +269,10 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
     }
   }
-  yf_flush(T, a.dome, P, L0, N);
+  yf_flush(T, a.dome, P, L0, nv, N);
 
   // ===== phi chain =====
-  yf_fill(T, a.rdome, P, L0, N);
+  yf_fill(T, a.rdome, P, L0, nv, N);
   {
     YPhase ph(-tho, 0.0, lane);
 #pragma unroll
@@ -280,11 +287,11 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = gld(syo + 2 * m);
+      const cplx s1 = syo[2 * m];
       T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
     }
   }
-  yf_fill(T, a.phi, P, L0, N);
+  yf_fill(T, a.phi, P, L0, nv, N);
   {
     YPhase ph(-tho, 0.0, lane);
 #pragma unroll
@@ -302,13 +309,13 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = gld(syo + 2 * m + 1);
+      const cplx s2 = syo[2 * m + 1];
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cadd(T[ac(m)], cmul(T[tc(m)], mul));
     }
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-  yf_fill(T, a.rhv, P, L0, N);
+  yf_fill(T, a.rhv, P, L0, nv, N);
   {
     YPhase ph(tho, 0.0, lane);
 #pragma unroll
@@ -318,7 +325,7 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hv
     }
   }
-  if (rk > 1) yf_fill(T, a.phiwk, P, L0, N);
+  if (rk > 1) yf_fill(T, a.phiwk, P, L0, nv, N);
   {
     YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
     YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
@@ -336,16 +343,16 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.phiwk, P, L0, N);
+  yf_flush(T, a.phiwk, P, L0, nv, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
     if (j < N) T[tc(j)] = x[q];
   }
-  yf_flush(T, a.phi, P, L0, N);
+  yf_flush(T, a.phi, P, L0, nv, N);
   // v = lapcdy(phi) and dv = D1(v) with the new twist; the (0,0) mode keeps
   // its v (lapcdy is not called for it) and only takes dv = D1(v)
-  if (L0 == 0) yf_fill(T, a.v, P, L0, N);
+  if (L0 == 0) yf_fill(T, a.v, P, L0, nv, N);   // block of the (0,0) mode
   if (!is00) {
     {
       YPhase ph(-thn, 0.0, lane);
@@ -361,7 +368,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx s1 = gld(syn + 2 * m), s2 = gld(syn + 2 * m + 1);
+        const cplx s1 = syn[2 * m], s2 = syn[2 * m + 1];
         const cplx H = yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y));   // lapcdy
         const cplx vh = cmul(T[ac(m)], H);
         T[tc(m)] = cmul(vh, s1);
@@ -398,7 +405,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx s1 = gld(syn + 2 * m);
+        const cplx s1 = syn[2 * m];
         T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
       }
     }
@@ -411,11 +418,11 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[tc(j)] = cmul(T[ac(j)], e);   // dv
     }
   }
-  yf_flush(T, a.dv, P, L0, N);
+  yf_flush(T, a.dv, P, L0, nv, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
     if (j < N) T[tc(j)] = x[q];
   }
-  yf_flush(T, a.v, P, L0, N);
+  yf_flush(T, a.v, P, L0, nv, N);
 }
