@@ -135,19 +135,21 @@ struct WFFT {
   }
 
   // ---- T -> L ----
-  // task t = n1 + R0 q2 holds v[q1] = x[t + 8 R0 q1]
+  // task t = n1 + R0 q2 holds v[q1] = x[t + 8 R0 q1]; rows of F are
+  // (l1, n1) with n1 fastest, which keeps these writes on distinct banks
   __host__ __device__ static void t2l_a(int t, cplx* v, cplx* F, const cplx* rt) {
     const int n1 = t % R0, q2 = t / R0;
     wf_dft<8, SIGN>(v);
-    F[(n1 * 8 + 0) * WF_FP + q2] = v[0];
+    F[(0 * R0 + n1) * WF_FP + q2] = v[0];
 #pragma unroll
     for (int l1 = 1; l1 < 8; ++l1)
-      F[(n1 * 8 + l1) * WF_FP + q2] = wf_tw<SIGN>(v[l1], rt[q2 * l1 * R0]);
+      F[(l1 * R0 + n1) * WF_FP + q2] = wf_tw<SIGN>(v[l1], rt[q2 * l1 * R0]);
   }
-  // task t' = l1 + 8 n1
+  // task t' = l1 + 8 n1 reads row (l1, n1)
   __host__ __device__ static void t2l_b_read(int t, const cplx* F, cplx* u) {
+    const int row = (t & 7) * R0 + (t >> 3);
 #pragma unroll
-    for (int q2 = 0; q2 < 8; ++q2) u[q2] = F[t * WF_FP + q2];   // row n1*8 + l1 = t
+    for (int q2 = 0; q2 < 8; ++q2) u[q2] = F[row * WF_FP + q2];
   }
   __host__ __device__ static void t2l_b_write(int t, cplx* u, cplx* E) {
     const int l1 = t & 7, n1 = t >> 3;

@@ -6,8 +6,8 @@
 //   WA  one wave per (kx, y) row: loads the 5 spectral inputs with the
 //       fill00 map, KMM reconstruction of u, v, w, omega (cross.f90:1221-
 //       1276), inverse z transform of the 6 fields -> hybrid [f][y][kx][z]
-//   WB  one block per (8 z-rows, y), one wave per row: the 6 hybrid fields
-//       staged through LDS (coalesced 128-byte runs of 8 z values), three
+//   WB  one block per (4 z-rows, y), one wave per row: the 6 hybrid fields
+//       staged through LDS (coalesced 64-byte runs of 4 z values), three
 //       paired c2r x transforms, u x omega, umax and the 'p' statistics
 //       (cross.f90:1430-1462, 2030-2045), r2c x transforms -> hybrid, 3 fields
 //   WC  one wave per (kx, y) row: forward z transform, pack00, normalisation
@@ -19,7 +19,7 @@
 #include "snf_wfft.h"
 
 #define WA_WAVES 4
-#define WB_ROWS 8
+#define WB_ROWS 4
 #define WC_WAVES 4
 
 // padded z position (0..Nz-1) of lane l, value m -> kz index, or -1 (gap)

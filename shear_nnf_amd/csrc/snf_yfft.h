@@ -21,7 +21,10 @@
 #include "snf_internal.h"
 #include "snf_wfft.h"
 
+#ifndef YF_LINES
 #define YF_LINES 4        // lines (waves) per block; tile rows of 4 lines = 64 B
+#endif
+#define YF_SH (YF_LINES == 2 ? 3 : (YF_LINES == 4 ? 2 : 1))   // log2(16 / YF_LINES)
 
 // Compile-time transform plan: the radices of the passes, in order.  A plan
 // fixes N = prod(Rs) and the number of rows per lane Q = ceil(N / 64).
@@ -33,13 +36,15 @@ template <int R, int... Rs> struct YPlanT<R, Rs...> {
   static constexpr int N = R * YPlanT<Rs...>::N;
 };
 
-// Column accessors.  TileCol: line c of the block's transfer tile, rows of 4
-// lines with the line index XORed with (j >> 2) & 3 so that a wave walking
-// one line down the rows touches distinct LDS banks.  OwnCol: a wave's
+// Column accessors.  TileCol: line c of the block's transfer tile, rows of
+// YF_LINES lines with the line index XORed with a function of j so that a
+// wave walking one line down the rows touches distinct LDS banks.  OwnCol: a wave's
 // private contiguous column.
 struct TileCol {
   int c;
-  __device__ __forceinline__ int operator()(int j) const { return j * YF_LINES + (c ^ ((j >> 2) & 3)); }
+  __device__ __forceinline__ int operator()(int j) const {
+    return j * YF_LINES + (c ^ ((j >> YF_SH) & (YF_LINES - 1)));
+  }
 };
 struct OwnCol {
   int base;
