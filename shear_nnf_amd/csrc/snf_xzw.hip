@@ -19,7 +19,9 @@
 #include "snf_wfft.h"
 
 #define WA_WAVES 4
+#ifndef WB_ROWS
 #define WB_ROWS 4
+#endif
 #define WC_WAVES 4
 
 // padded z position (0..Nz-1) of lane l, value m -> kz index, or -1 (gap)
