@@ -62,6 +62,9 @@ typedef struct snf_params {
   double etime, dumptime, dtimag;  /* stop / dump times (cross.f90:1680-1712) */
   int32_t precision;          /* SNF_PREC_FP64 (0) or SNF_PREC_MIXED (1)  */
   int32_t device;             /* HIP device ordinal                       */
+  int32_t ydealias;           /* iydealiasing: 0 none (default), 3 compact
+                                 filter of vor, phi, U, W every substep
+                                 (cross.f90:476-488); 1, 2 not built      */
 } snf_params;
 
 #define SNF_PREC_FP64 0

@@ -336,16 +336,27 @@ def solve(f, b):
 
 
 def stencil(c, u, shp, shm, skip_center=False):
-    """Twisted-cyclic 7-point product: out(j) = sum_m c(m) u(j+m), with
-    u(j+N) = shp u(j), u(j-N) = shm u(j) (derivyc 358-383, lapcdy 487-514)."""
+    """Twisted-cyclic (2 na + 1)-point product, na = (len(c) - 1) / 2:
+    out(j) = sum_m c(m) u(j+m), with u(j+N) = shp u(j), u(j-N) = shm u(j)
+    (derivyc 358-383, lapcdy 487-514, matvecNcyclic 960-996)."""
     N = u.shape[0]
-    ext = np.concatenate([u[N - 3:] * shm, u, u[:3] * shp], axis=0)
+    na = (len(c) - 1) // 2
+    ext = np.concatenate([u[N - na:] * shm, u, u[:na] * shp], axis=0)
     out = np.zeros_like(u, dtype=np.complex128)
-    for m in (-3, -2, -1, 0, 1, 2, 3):
+    for m in range(-na, na + 1):
         if m == 0 and skip_center:
             continue
-        out = out + c[m + 3] * ext[3 + m:3 + m + N]
+        out = out + c[m + na] * ext[na + m:na + m + N]
     return out
+
+
+# Compact filter of filter_shear_hepta.f90:53-69 (prefilter): symmetric
+# hepta-diagonal LHS, symmetric 9-point RHS, not scaled by the grid step.
+FILTER_PR = np.array([0.053003478448814, 0.315003368623046, 0.738981897704866, 1.0,
+                      0.738981897704866, 0.315003368623046, 0.053003478448814])
+FILTER_RH = np.array([-0.000179828066167, 0.054442102978149, 0.309968182770373,
+                      0.749052269410218, 0.987412035368317, 0.749052269410218,
+                      0.309968182770373, 0.054442102978149, -0.000179828066167])
 
 
 class YOps:
@@ -387,6 +398,23 @@ class YOps:
             r = stencil(self.rh2, u.astype(np.complex128), one, one)
             f = _bcast(self.base2, u.shape[1])
         return solve(f, r).real
+
+    def filter(self, u, shp, shm):
+        """filterc m=2 (filter_shear_hepta.f90:128-165): B_f x = R_f u on
+        twisted lines, factor re-used from the untwisted one as in derivyc."""
+        if not hasattr(self, "basef"):
+            self.basef = modchol(FILTER_PR, self.N, np.ones(1, dtype=np.complex128))
+        f = twist_base(self.basef, FILTER_PR, shp)
+        return solve(f, stencil(FILTER_RH, u, shp, shm))
+
+    def filter_real(self, u):
+        """filterc m=1: real periodic line (the 00 modes)."""
+        if not hasattr(self, "basef"):
+            self.basef = modchol(FILTER_PR, self.N, np.ones(1, dtype=np.complex128))
+        u = np.asarray(u, dtype=np.float64).reshape(self.N, -1)
+        one = np.ones(u.shape[1], dtype=np.complex128)
+        r = stencil(FILTER_RH, u.astype(np.complex128), one, one)
+        return solve(_bcast(self.basef, u.shape[1]), r).real
 
     def lap_matrix(self, k2):
         k2 = np.atleast_1d(np.asarray(k2, dtype=np.float64))

@@ -44,7 +44,7 @@ def moving_frame(g, f, m):
 
 class Cross:
     def __init__(self, grid: Grid, explicit=1, nhist=1, nimag=10 ** 9,
-                 readflag=1, uprim=0.1, vbulk=0.0):
+                 readflag=1, uprim=0.1, vbulk=0.0, ydealias=0):
         self.g = grid
         self.ops = YOps(grid.my, grid.dy)
         self.explicit = int(explicit)
@@ -53,6 +53,9 @@ class Cross:
         self.readflag = int(readflag)
         self.uprim = float(uprim)
         self.vbulk = float(vbulk)
+        if ydealias not in (0, 3):
+            raise NotImplementedError("iydealiasing 1 and 2 are not restated")
+        self.ydealias = int(ydealias)
         g = grid
         self.vor = g.zeros()
         self.phi = g.zeros()
@@ -299,6 +302,19 @@ class Cross:
             self.dtr = g.Re / self.Deltat
             self.istop = 1
 
+    def _filter_state(self, sht, shb):
+        """iydealiasing=3: compact filter of vor, phi (twisted) and U, W
+        (cross.f90:214-225, 476-488; filterc filter_shear_hepta.f90:128-165)"""
+        g = self.g
+        ops = self.ops
+        self.u00 = ops.filter_real(self.u00)[:, 0]
+        self.w00 = ops.filter_real(self.w00)[:, 0]
+        t = np.repeat(sht, g.mz)
+        tm = np.repeat(shb, g.mz)
+        for name in ("vor", "phi"):
+            f = getattr(self, name).reshape(g.my, -1)
+            setattr(self, name, ops.filter(f, t, tm).reshape(g.my, g.kxn, g.mz))
+
     # ------------------------------------------------------------------ cross
     def run(self, nstep, on_dump=None):
         """cross.f90:3-1101. Returns the number of completed steps."""
@@ -315,6 +331,9 @@ class Cross:
         self.idump = 0
         set_conj(self.vor, g.nz)
         set_conj(self.phi, g.nz)
+        if self.ydealias == 3:
+            # before the step loop, with the twist of the current offsets
+            self._filter_state(self._shift(self.xofft), self._shift(self.xoffb))
         done = 0
         for istep in range(1, nstep + 1):
             self.istep = istep
@@ -345,6 +364,8 @@ class Cross:
                 ir = rk - 1
                 self.dvordy = self._lines_d(self.vor, shwkt, shwkb, 1)
                 self.hv = self._lines_d(self.hg, shwkt, shwkb, 1)
+                if self.ydealias == 3:
+                    self._filter_state(shwkt, shwkb)
                 self.hvhg(rk, ihist)
                 if ihist and rk == 1:
                     ihist = 0

@@ -1,10 +1,11 @@
-"""The reference's own compact y-operators, compiled from its sources.
+"""The reference's own compact y-operators and filter, compiled from its sources.
 
 TEST INFRASTRUCTURE ONLY.  ``oracle/Makefile`` builds ``oracle/_ref/libshear_ref.so``
 from mod.f90 and diffy_shear_hepta.f90 as they lie in the reference tree
 (ROCm flang, MKL for BLAS ``dcopy``).  The CPU tests call its ``derivyc``
-(diffy_shear_hepta.f90:333-450) and ``lapcdy`` (diffy_shear_hepta.f90:466-525)
-to pin the numpy restatement in ``oracle/compact.py``.  The library keeps its
+(diffy_shear_hepta.f90:333-450), ``lapcdy`` (diffy_shear_hepta.f90:466-525) and
+``filterc`` (filter_shear_hepta.f90:128-165) to pin the numpy restatement in
+``oracle/compact.py``.  The library keeps its
 grid in Fortran module variables that ``derivadas`` allocates once
 (diffy_shear_hepta.f90:535-560), so every grid size runs in a fresh child
 process.
@@ -34,6 +35,8 @@ def _child(my, dy, jobs, q):
         h = ctypes.c_double(dy)
         lib.derivadas_(ctypes.byref(n), ctypes.byref(h))
         lib.update_dy_(ctypes.byref(h))
+        if any(job[0].startswith("filter") for job in jobs):
+            lib.filteras_(ctypes.byref(n))
         out = []
         for job in jobs:
             kind = job[0]
@@ -45,7 +48,14 @@ def _child(my, dy, jobs, q):
             for l in range(u.shape[1]):
                 ui = np.ascontiguousarray(u[:, l]).view(np.float64).copy()
                 ro = np.zeros_like(ui)
-                if kind == "deriv_real":
+                if kind == "filter_real":
+                    lib.filterc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
+                                 _cplx(1.0), _cplx(1.0), ctypes.byref(m), ctypes.byref(ctypes.c_int(0)))
+                elif kind == "filter":
+                    lib.filterc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
+                                 _cplx(job[2][l]), _cplx(job[3][l]), ctypes.byref(m),
+                                 ctypes.byref(ctypes.c_int(1)))
+                elif kind == "deriv_real":
                     lib.derivyc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
                                  _cplx(1.0), _cplx(1.0), ctypes.byref(ctypes.c_int(job[2])),
                                  ctypes.byref(m), ctypes.byref(ctypes.c_int(1)))
@@ -76,7 +86,8 @@ def run(my, dy, jobs, timeout=120):
 
     jobs: ("deriv", u[my, nl], shp[nl], shm[nl], order),
           ("lap", u[my, nl], k2[nl], shp[nl], shm[nl]),
-          ("deriv_real", u[my, nl] real, order) or ("lap_real", u[my, nl] real, k2[nl]).
+          ("deriv_real", u[my, nl] real, order), ("lap_real", u[my, nl] real, k2[nl]),
+          ("filter", u[my, nl], shp[nl], shm[nl]) or ("filter_real", u[my, nl] real).
     Returns one array [my, nl] per job.
     """
     ctx = mp.get_context("spawn")

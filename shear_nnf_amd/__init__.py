@@ -46,6 +46,7 @@ class SnfParams(ctypes.Structure):
         ("uprim", ctypes.c_double), ("vbulk", ctypes.c_double),
         ("etime", ctypes.c_double), ("dumptime", ctypes.c_double), ("dtimag", ctypes.c_double),
         ("precision", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("ydealias", ctypes.c_int32),
     ]
 
 
@@ -123,10 +124,10 @@ class ShearBox:
 
     def __init__(self, mgalx, my, mgalz, Re, alp=np.pi, gam=2 * np.pi, Ly=1.0, s=1.0,
                  chi=0.0, CFL=0.5, explicit=1, nhist=1, readflag=1, uprim=0.1, vbulk=0.0,
-                 etime=1e10, dumptime=1e10, dtimag=1e10, device=0):
+                 etime=1e10, dumptime=1e10, dtimag=1e10, device=0, ydealias=0):
         self.lib = load_library()
         p = SnfParams(mgalx, my, mgalz, Re, alp, gam, Ly, s, chi, CFL, explicit, nhist,
-                      readflag, uprim, vbulk, etime, dumptime, dtimag, 0, device)
+                      readflag, uprim, vbulk, etime, dumptime, dtimag, 0, device, ydealias)
         h = ctypes.c_void_p()
         self._check(self.lib.snf_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h

@@ -103,6 +103,7 @@ struct snf_handle {
   double* upart = nullptr;   // per-row umax partials of the wave x-row kernel
   // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
   bool yfft = false;
+  bool ysym_on = false;   // multiplier tables maintained (spectral y or filter)
   cplx* twy = nullptr;
   cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
   size_t ldsY = 0;
@@ -344,11 +345,22 @@ void launch_yf(snf_handle* h, int rk) {
   }
 }
 
+// iydealiasing=3: compact filter of vor, phi, U, W at the current twist
+void launch_yfilter(snf_handle* h) {
+  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
+  const YArgs a = yargs(h);
+  switch (h->g.my) {
+    case 33: k_yf_filter<3, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    case 49: k_yf_filter<7, 7><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    default: k_yf_filter<5, 7, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+  }
+}
+
 int launch_twist(snf_handle* h, int which) {
   dim3 gt(h->g.kxn, 2);
   k_twist<<<gt, 64, 0, h->stream>>>(h->g.my, h->b1, h->b2, h->c, h->shp[which], h->tw[which],
                                     h->g.kxn, h->sc);
-  if (h->yfft) {
+  if (h->ysym_on) {
     dim3 gs(h->g.kxn, (h->g.my + 255) / 256);
     k_ysym<<<gs, 256, 0, h->stream>>>(h->g.my, h->c, h->shp[which], h->twy, h->ysym[which], h->sc);
   }
@@ -380,6 +392,10 @@ int snf_create(const snf_params* p, snf_handle** out) {
   if (!(p->Re > 0) || !(p->alp > 0) || !(p->gam > 0) || !(p->Ly > 0) || !(p->CFL > 0))
     return fail(SNF_BAD_PARAM, "Re, alp, gam, Ly, CFL must be positive");
   if (p->precision != SNF_PREC_FP64) return fail(SNF_BAD_PARAM, "only SNF_PREC_FP64 is built");
+  if (p->ydealias != 0 && p->ydealias != 3)
+    return fail(SNF_BAD_PARAM, "ydealias: 0 (none) or 3 (compact filter); 1 and 2 are not built");
+  if (p->ydealias == 3 && !y_plan(p->my))
+    return fail(SNF_BAD_PARAM, "ydealias=3 needs a compiled y-transform plan (my = 33, 49, 385)");
   snf_handle* h = new snf_handle();
   h->p = *p;
   int ndev = 0;
@@ -490,7 +506,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
   TRY(A(&h->upart, (size_t)my * mgalz * 4));
   TRY(A(&h->twy, (size_t)my));
-  for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * 2));
+  for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * YS));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
   TRY(A(&h->ring, (size_t)h->cap * 24));
@@ -528,6 +544,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   {
     const char* e = getenv("SNF_Y_PATH");
     h->yfft = p->explicit_visc == 1 && y_plan(my) && !(e && std::string(e) == "banded");
+    h->ysym_on = h->yfft || p->ydealias == 3;
     std::vector<cplx> ty(my);
     for (int t = 0; t < my; ++t) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / my;
@@ -538,7 +555,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // tile, a column per wave, roots, two multiplier rows
     h->ldsY = (size_t)my * (2 * YF_LINES + 5) * sizeof(cplx);
     for (const void* ky : {(const void*)k_yf_explicit<3, 11>, (const void*)k_yf_explicit<7, 7>,
-                           (const void*)k_yf_explicit<5, 7, 11>})
+                           (const void*)k_yf_explicit<5, 7, 11>, (const void*)k_yf_filter<3, 11>,
+                           (const void*)k_yf_filter<7, 7>, (const void*)k_yf_filter<5, 7, 11>})
       (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
@@ -744,11 +762,15 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur]);
   k_set_conj<<<g.my, 128, 0, h->stream>>>(g, h->vor, h->phi);
   if ((rc = launch_twist(h, h->cur))) return rc;
+  if (h->p.ydealias == 3) launch_yfilter(h);   // before the step loop (cross.f90:214-225)
   k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
   HIPCHK(hipGetLastError());
   for (int step = 0; step < nstep; ++step) {
     k_step_begin<<<1, 1, 0, h->stream>>>(h->sc);
     for (int rk = 1; rk <= 3; ++rk) {
+      // d(omega_y)/dy and dv/dy of this substep were formed from the
+      // unfiltered fields, as in cross.f90:434-445 before the filter of 476-488
+      if (h->p.ydealias == 3) launch_yfilter(h);
       if ((rc = launch_xz(h, rk))) return rc;
       {
         TimedLaunch tl(h, SNF_K_SMALL);

@@ -191,6 +191,19 @@ __device__ __forceinline__ cplx yf_symbol(const C7& k, cplx e1) {
   return mkc(c[3] + c1 * e1.x + c2 * e2.x + c3 * e3.x, s1 * e1.y + s2 * e2.y + s3 * e3.y);
 }
 
+// Compact filter of filter_shear_hepta.f90:53-69 (prefilter): symmetric
+// hepta LHS p(-3..3) and symmetric 9-point RHS r(-4..4), not scaled by the
+// grid step.  Its multiplier is R_f(w) / B_f(w), both real.
+__device__ __forceinline__ double yf_filter_symbol(cplx e1) {
+  const double p0 = 1.0, p1 = 0.738981897704866, p2 = 0.315003368623046, p3 = 0.053003478448814;
+  const double r0 = 0.987412035368317, r1 = 0.749052269410218, r2 = 0.309968182770373,
+               r3 = 0.054442102978149, r4 = -0.000179828066167;
+  const cplx e2 = cmul(e1, e1), e3 = cmul(e2, e1), e4 = cmul(e2, e2);
+  const double B = p0 + 2.0 * (p1 * e1.x + p2 * e2.x + p3 * e3.x);
+  const double R = r0 + 2.0 * (r1 * e1.x + r2 * e2.x + r3 * e3.x + r4 * e4.x);
+  return R / B;
+}
+
 // a / b
 __device__ __forceinline__ cplx yf_div(cplx a, cplx b) {
   const double d = 1.0 / (b.x * b.x + b.y * b.y);

@@ -20,6 +20,8 @@
 // take one extra block with one wave each.
 #include "snf_yfft.h"
 
+#define YS 3   // multipliers per mode in the tables: d/dy, d2/dy2, filter
+
 __device__ __forceinline__ cplx yf_expi(double a) {
   double s, c;
   sincos(a, &s, &c);
@@ -63,10 +65,10 @@ __device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
   return cmul(cconj(gld(rt + m)), eth);
 }
 
-// Multipliers of the twisted d/dy and d2/dy2 for every kx and mode m at the
-// twist shp[kx]: out[(kx N + m) 2 + {0,1}] = R1/B1, R2/B2 at
-// w_m = (2 pi m + arg shp)/N.  (The Helmholtz multiplier of a line is
-// 1/(R2/B2 - k2).)
+// Multipliers of the twisted d/dy, d2/dy2 and compact filter for every kx
+// and mode m at the twist shp[kx]: out[(kx N + m) YS + {0,1,2}] = R1/B1,
+// R2/B2, R_f/B_f at w_m = (2 pi m + arg shp)/N.  (The Helmholtz multiplier
+// of a line is 1/(R2/B2 - k2).)
 __global__ void __launch_bounds__(256)
 k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt,
        cplx* __restrict__ out, const DevScalars* sc) {
@@ -76,9 +78,10 @@ k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt
   const cplx t = gld(shp + kx);
   const cplx e1 = yf_mode(rt, m, yf_expi(atan2(t.y, t.x) / (double)N));
   const C7 rh1 = c7_of(c.rh1), pr1 = c7_of(c.pr1), rh2 = c7_of(c.rh2), pr2 = c7_of(c.pr2);
-  cplx* o = out + ((long)kx * N + m) * 2;
+  cplx* o = out + ((long)kx * N + m) * YS;
   gst(o, yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1)));
   gst(o + 1, yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1)));
+  gst(o + 2, mkc(yf_filter_symbol(e1), 0.0));
 }
 
 template <int... Rs>
@@ -129,7 +132,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        T[ac(m)] = cmul(T[ac(m)], gld(sy + 2 * m + 1));
+        T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m + 1));
       }
     }
     yf_fft<1, OwnCol, Rs...>(T, ac, rt);
@@ -151,7 +154,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        T[ac(m)] = cmul(T[ac(m)], gld(sy + 2 * m));
+        T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m));
       }
     }
     yf_fft<1, OwnCol, Rs...>(T, ac, rt);
@@ -179,8 +182,9 @@ k_yf_explicit(YArgs a, int rk) {
   cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [m][2]
   cplx* syn = syo + 2 * N;     // ... at the new twist
   for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
-    syo[t] = gld(a.ysym_old + (long)i * N * 2 + t);
-    syn[t] = gld(a.ysym_new + (long)i * N * 2 + t);
+    const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
+    syo[t] = gld(a.ysym_old + src);
+    syn[t] = gld(a.ysym_new + src);
   }
 
   // ===== omega_y chain =====
@@ -425,4 +429,105 @@ k_yf_explicit(YArgs a, int rk) {
     if (j < N) T[tc(j)] = x[q];
   }
   yf_flush(T, a.v, P, L0, nv, N);
+}
+
+// iydealiasing=3: the compact filter (filterc, filter_shear_hepta.f90:128-165)
+// applied in place to vor and phi at the current twist and to U, W
+// (cross.f90:214-225, 476-488), then dU/dy, dW/dy of the filtered means
+// (read by the x-z stage, cross.f90:1198-1205).  Same block layout as
+// k_yf_explicit.
+template <int... Rs>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_filter(YArgs a) {
+  constexpr int N = YPlanT<Rs...>::N, Q = (N + 63) / 64;
+  if (a.sc->halted) return;
+  extern __shared__ cplx T[];
+  const Geom& g = a.g;
+  const long P = g.P;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
+  const long nbl = (long)g.kxn * nch;
+  cplx* rt = T + 2 * YF_LINES * N;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.twy + t);
+  __syncthreads();
+  const double invN = 1.0 / (double)N;
+  const TileCol tc{w};
+  const OwnCol ac{YF_LINES * N + w * N};
+  if (blockIdx.x == nbl) {
+    if (w >= 2) return;
+    cplx* u = w == 0 ? a.u00c : a.w00c;
+    double* dd = w == 0 ? a.dudy : a.dwdy;
+    const cplx* sy = a.ysym_old;   // kx = 0 row: untwisted
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) T[ac(j)] = gld(u + j);
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) T[ac(m)] = crmul(invN * gld(sy + YS * m + 2).x, T[ac(m)]);
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) {
+        const cplx uf = mkc(T[ac(j)].x, 0.0);
+        gst(u + j, uf);
+        T[ac(j)] = uf;
+      }
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, gld(sy + YS * m)));
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) gstd(dd + j, T[ac(j)].x);
+    }
+    return;
+  }
+  const long lb = xcd_block(blockIdx.x, nbl);
+  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
+  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
+  const cplx so = gld(a.shp_old + i);
+  const double tho = atan2(so.y, so.x) * invN;
+  const cplx* sy = a.ysym_old + (long)i * N * YS;
+  for (int f = 0; f < 2; ++f) {
+    cplx* F = f == 0 ? a.vor : a.phi;
+    yf_fill(T, F, P, L0, nv, N);
+    {
+      YPhase ph(-tho, 0.0, lane);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        const cplx e = ph.next();
+        if (j < N) T[ac(j)] = cmul(T[tc(j)], e);
+      }
+    }
+    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) T[ac(m)] = crmul(invN * gld(sy + YS * m + 2).x, T[ac(m)]);
+    }
+    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    {
+      YPhase ph(tho, 0.0, lane);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        const cplx e = ph.next();
+        if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
+      }
+    }
+    yf_flush(T, F, P, L0, nv, N);
+  }
 }

@@ -172,6 +172,17 @@ def test_cross_spectral_y_3_steps(lib, grid):
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
 
+# iydealiasing=3: compact filter of vor, phi, U, W at every substep
+@pytest.mark.parametrize("grid,explicit", [((32, 33, 32), 1), ((48, 49, 48), 1), ((48, 49, 48), 0)])
+def test_cross_compact_filter(lib, grid, explicit):
+    st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 4, grid=grid, ydealias=3)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
 def test_spectral_and_banded_y_paths_agree(lib, monkeypatch):
     grid = (48, 49, 48)
     vor, phi, u00, w00 = box_ic(*grid)
@@ -186,13 +197,13 @@ def test_spectral_and_banded_y_paths_agree(lib, monkeypatch):
         assert rel_l2(a, b) < STEP_TOL
 
 
-def _run_pair(explicit, nstep, grid=(48, 49, 48), Re=1000.0, readflag=1, nhist=1):
+def _run_pair(explicit, nstep, grid=(48, 49, 48), Re=1000.0, readflag=1, nhist=1, ydealias=0):
     vor, phi, u00, w00 = box_ic(*grid)
-    box = _box(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist)
+    box = _box(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist, ydealias=ydealias)
     box.set_state(vor, phi, u00, w00)
     st = box.cross(nstep)
     gv = box.get_state()
-    c = _oracle(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist)
+    c = _oracle(*grid, Re=Re, explicit=explicit, readflag=readflag, nhist=nhist, ydealias=ydealias)
     c.set_state(vor, phi, u00, w00)
     c.run(nstep)
     return st, gv, box, c

@@ -1,6 +1,7 @@
 """The numpy restatement of the compact y-operators against the reference's
-own derivyc / lapcdy (diffy_shear_hepta.f90:333-525), compiled from its
-Fortran sources by oracle/Makefile into oracle/_ref.
+own derivyc / lapcdy (diffy_shear_hepta.f90:333-525) and compact filter
+filterc (filter_shear_hepta.f90:128-165), compiled from its Fortran sources by
+oracle/Makefile into oracle/_ref.
 
 Twisted complex lines (m=2) and real periodic lines (m=1, the U(y), W(y)
 means) at several my; relative L2 bar 1e-13 (both sides are FP64 LDL^H
@@ -51,3 +52,18 @@ def test_real_lines(my):
     assert rel_l2(ops.deriv_real(u, 1), d1) < TOL
     assert rel_l2(ops.deriv_real(u, 2), d2) < TOL
     assert rel_l2(ops.helmholtz_real(u, 37.5), lp) < TOL
+
+
+@pytest.mark.parametrize("my", [33, 49, 97])
+def test_compact_filter(my):
+    rng = np.random.default_rng(700 + my)
+    nl = 8
+    dy = 1.0 / my
+    u = rng.normal(size=(my, nl)) + 1j * rng.normal(size=(my, nl))
+    th = rng.uniform(-math.pi, math.pi, nl)
+    shp = np.exp(1j * th)
+    ur = rng.normal(size=(my, 3))
+    f, fr = ref.run(my, dy, [("filter", u, shp, np.conj(shp)), ("filter_real", ur)])
+    ops = YOps(my, dy)
+    assert rel_l2(ops.filter(u, shp, np.conj(shp)), f) < TOL
+    assert rel_l2(ops.filter_real(ur), fr) < TOL
