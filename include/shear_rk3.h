@@ -132,6 +132,14 @@ int snf_op_derivy(snf_handle* h, const double* u, double* du, int32_t nl,
 /* lapcdy (diffy_shear_hepta.f90:466-525): (d_yy - k2[l]) p = u           */
 int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl,
                      const double* k2, const double* shp, const double* shm);
+/* getv (operators.f90:735-793) on fields in the device layout [y][kx][kz]
+ * (interleaved complex, my*kxn*mz values) with the twist of the handle's
+ * current offsets: dir=+1 v = lapcdy(phi) (phi -> v), dir=-1 phi =
+ * d2v/dy2 - k2 v (v -> phi).  The (0,0) line of out is not written.     */
+int snf_op_getv(snf_handle* h, const double* in, double* out, int32_t dir);
+/* energy (operators.f90:870-924) of the device state: out[0..3] = sums of
+ * |vor|^2, |phi|^2 (kx > 0 counted twice), U^2, W^2.                    */
+int snf_energy(snf_handle* h, double* out);
 /* fourxz (fou3D_thread.f:22-73) on nplanes planes: dir=+1 spectral
  * (kxn x mz complex, [kx][kz]) -> physical (mgalx x mgalz, [x][z]);
  * dir=-1 the reverse with the 1/(mgalx mgalz) scaling.                   */

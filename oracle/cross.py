@@ -124,6 +124,33 @@ class Cross:
         out = solve(fac, r).reshape(g.my, g.kxn, g.mz)
         return out
 
+    def getv(self, f, iopt):
+        """getv (operators.f90:735-793) with the twist of the current offsets:
+        iopt=+1 returns v = lapcdy(phi=f); iopt=-1 returns phi = d2v/dy2 -
+        k2 v (visc3d with scal=1, diffy_shear_hepta.f90:298-318) for v=f.
+        The (0,0) line of the result is zero (the reference leaves it
+        untouched)."""
+        g = self.g
+        sht, shb = self._shift(self.xofft), self._shift(self.xoffb)
+        k2 = g.alp2[:, None] + g.gam2[None, :]
+        if iopt == 1:
+            out = self._lines_helm(f, k2, sht, shb)
+        elif iopt == -1:
+            out = self._lines_d(f, sht, shb, 2) - k2[None, :, :] * f
+        else:
+            raise ValueError("iopt must be +1 or -1")
+        out[:, 0, 0] = 0.0
+        return out
+
+    def energy(self):
+        """energy (operators.f90:870-924): sums of |vor|^2 and |phi|^2 with
+        kx > 0 counted twice, and of U^2, W^2."""
+        w = np.ones(self.g.kxn)
+        w[1:] = 2.0
+        e1 = float(np.sum(w[None, :, None] * np.abs(self.vor) ** 2))
+        e2 = float(np.sum(w[None, :, None] * np.abs(self.phi) ** 2))
+        return np.array([e1, e2, float(np.sum(self.u00 ** 2)), float(np.sum(self.w00 ** 2))])
+
     # ------------------------------------------------------------------- hvhg
     def derived(self, phic, ome2c, rhgc, rhvc, ome1c, dudy, dwdy):
         """KMM reconstruction at every plane (cross.f90:1221-1276).

@@ -30,7 +30,7 @@ EXPORTS = (
     "snf_state_upload", "snf_state_download", "snf_state_upload_ykxkz",
     "snf_state_download_ykxkz", "snf_advance", "snf_history", "snf_statistics",
     "snf_op_derivy", "snf_op_helmholtz", "snf_op_fourxz", "snf_op_hvhg",
-    "snf_set_timing", "snf_get_timing", "snf_copy_probe",
+    "snf_set_timing", "snf_get_timing", "snf_copy_probe", "snf_op_getv", "snf_energy",
 )
 KERNEL_CLASSES = ("xz_A", "xz_B", "xz_C", "y_lines", "small")
 
@@ -93,6 +93,8 @@ def load_library(path=LIB_PATH):
     lib.snf_set_timing.argtypes = [H, ctypes.c_int32]
     lib.snf_get_timing.argtypes = [H, dp, P(ctypes.c_int32), ctypes.c_int32]
     lib.snf_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, dp]
+    lib.snf_op_getv.argtypes = [H, dp, dp, ctypes.c_int32]
+    lib.snf_energy.argtypes = [H, dp]
     _lib = lib
     return lib
 
@@ -279,6 +281,23 @@ class ShearBox:
             out = np.empty((a.shape[0], self.kxn, self.mz), dtype=np.complex128)
             self._check(self.lib.snf_op_fourxz(self.h, _ptr(a), _ptr(out.view(np.float64)),
                                                a.shape[0], -1))
+        return out
+
+    def getv(self, f, iopt):
+        """getv (operators.f90:735-793) on a (my, kxn, mz) field with the twist
+        of the current offsets: iopt=+1 phi -> v, iopt=-1 v -> phi.  The
+        (0,0) line of the result is zero."""
+        f = _c128(f)
+        out = np.zeros_like(f)
+        self._check(self.lib.snf_op_getv(self.h, _ptr(f.view(np.float64)),
+                                         _ptr(out.view(np.float64)), int(iopt)))
+        return out
+
+    def energy(self):
+        """energy (operators.f90:870-924) of the device state: sums of
+        |vor|^2, |phi|^2 (kx > 0 counted twice), U^2, W^2."""
+        out = np.empty(4)
+        self._check(self.lib.snf_energy(self.h, _ptr(out)))
         return out
 
     def hvhg(self, v, dvdy, dvordy):

@@ -904,6 +904,81 @@ int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl, cons
   return SNF_OK;
 }
 
+int snf_energy(snf_handle* h, double* out) {
+  if (!h || !out) return fail(SNF_BAD_PARAM, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const int nb = 1024;
+  double *part, *res;
+  HIPCHK(hipMalloc(&part, 2 * (size_t)nb * sizeof(double)));
+  HIPCHK(hipMalloc(&res, 4 * sizeof(double)));
+  k_energy_part<<<nb, 256, 0, h->stream>>>(h->g, h->vor, h->phi, part);
+  k_energy_final<<<1, 256, 0, h->stream>>>(part, nb, h->u00c, h->w00c, h->g.my, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  hipFree(part);
+  hipFree(res);
+  return SNF_OK;
+}
+
+int snf_op_getv(snf_handle* h, const double* in, double* out, int32_t dir) {
+  if (!h || !in || !out || (dir != 1 && dir != -1)) return fail(SNF_BAD_PARAM, "bad argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const Geom& g = h->g;
+  const long P = g.P;
+  const int N = g.my;
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  // per-line k2 and the twist of the current offsets (operators.f90:754-759)
+  std::vector<double> k2(P);
+  std::vector<cplx> sp(P), sm(P);
+  for (long L = 0; L < P; ++L) {
+    const int i = (int)(L / g.mz), k = (int)(L % g.mz);
+    const double kx = g.alp * i;
+    const double kz = kz_of(k, g.nz, g.mz, g.gam);
+    k2[L] = (L == 0) ? 1.0 : kx * kx + kz * kz;
+    sp[L] = mkc(std::cos(-kx * s.xofft), std::sin(-kx * s.xofft));
+    sm[L] = mkc(std::cos(-kx * s.xoffb), std::sin(-kx * s.xoffb));
+  }
+  // the (0,0) line of out is not written
+  std::vector<double> keep(2 * (size_t)N);
+  for (int j = 0; j < N; ++j) {
+    keep[2 * j] = out[2 * (size_t)j * P];
+    keep[2 * j + 1] = out[2 * (size_t)j * P + 1];
+  }
+  int rc;
+  if (dir == 1) {
+    rc = snf_op_helmholtz(h, in, out, (int32_t)P, k2.data(), (const double*)sp.data(),
+                          (const double*)sm.data());
+  } else {
+    std::vector<double> d2(2 * (size_t)N * P);
+    rc = snf_op_derivy(h, in, d2.data(), (int32_t)P, (const double*)sp.data(),
+                       (const double*)sm.data(), 2);
+    if (rc == SNF_OK) {
+      const size_t n = (size_t)N * P;
+      cplx *vd, *dd, *pd;
+      HIPCHK(hipMalloc(&vd, n * sizeof(cplx)));
+      HIPCHK(hipMalloc(&dd, n * sizeof(cplx)));
+      HIPCHK(hipMalloc(&pd, n * sizeof(cplx)));
+      HIPCHK(hipMemcpy(vd, in, n * sizeof(cplx), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(dd, d2.data(), n * sizeof(cplx), hipMemcpyHostToDevice));
+      k_lap_combine<<<1024, 256, 0, h->stream>>>(g, vd, dd, pd);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipMemcpy(out, pd, n * sizeof(cplx), hipMemcpyDeviceToHost));
+      hipFree(vd);
+      hipFree(dd);
+      hipFree(pd);
+    }
+  }
+  if (rc != SNF_OK) return rc;
+  for (int j = 0; j < N; ++j) {
+    out[2 * (size_t)j * P] = keep[2 * j];
+    out[2 * (size_t)j * P + 1] = keep[2 * j + 1];
+  }
+  return SNF_OK;
+}
+
 int snf_op_fourxz(snf_handle* h, const double* in, double* out, int32_t nplanes, int32_t dir) {
   if (!h || !in || !out || nplanes <= 0 || (dir != 1 && dir != -1)) return fail(SNF_BAD_PARAM, "bad argument");
   HIPCHK(hipSetDevice(h->dev));

@@ -50,6 +50,11 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
   return (b % SNF_NXCD) * q + b / SNF_NXCD;
 }
 
+// kz of z-index k: 0..nz non-negative, nz+1..mz-1 negative
+__host__ __device__ __forceinline__ double kz_of(int k, int nz, int mz, double gam) {
+  return k <= nz ? gam * (double)k : -gam * (double)(mz - k);
+}
+
 // Grid description passed by value to every kernel.
 struct Geom {
   int my, kxn, mz, nz, mgalx, mgalz;

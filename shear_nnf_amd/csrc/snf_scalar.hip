@@ -12,9 +12,6 @@ __constant__ double c_ibeta[3] = {160.0 / 37.0, 24.0 / 5.0, 6.0};
 __constant__ double c_zeta[3] = {-17.0 / 60.0, -5.0 / 12.0, 0.0};
 __constant__ double c_ca[3] = {8.0 / 15.0, 2.0 / 3.0, 1.0};
 
-__device__ __forceinline__ double kz_val(int k, int nz, int mz, double gam) {
-  return k <= nz ? gam * (double)k : -gam * (double)(mz - k);
-}
 
 // exp(i kx y_j m) of moving_frame (operators.f90:86-145)
 __device__ __forceinline__ cplx mframe(double kx, double yj, double m) {

@@ -19,9 +19,6 @@
 __device__ __forceinline__ int kz_pos(int k, int nz, int mz, int Nz) {
   return k <= nz ? k : k - mz + Nz;
 }
-__device__ __forceinline__ double kz_of(int k, int nz, int mz, double gam) {
-  return k <= nz ? gam * (double)k : -gam * (double)(mz - k);
-}
 // (a + ib) / (i g) = (b/g, -a/g)
 __device__ __forceinline__ cplx div_i(cplx a, double g) { return mkc(a.y / g, -a.x / g); }
 // (0 + i c) * a

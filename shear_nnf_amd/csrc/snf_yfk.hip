@@ -174,7 +174,7 @@ k_yf_explicit(YArgs a, int rk) {
   const long L0 = (long)i * g.mz + YF_LINES * ch;
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
   const int k = min(YF_LINES * ch + w, g.mz - 1);
-  const double kx = g.alp * (double)i, kzv = kz_val(k, g.nz, g.mz, g.gam);
+  const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
   const double k2 = kx * kx + kzv * kzv;
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);

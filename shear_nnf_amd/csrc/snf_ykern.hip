@@ -186,7 +186,7 @@ __device__ __forceinline__ void line_geom(const Geom& g, long L, int& i, int& k,
   i = (int)(L / g.mz);
   k = (int)(L - (long)i * g.mz);
   kx = g.alp * (double)i;
-  const double kz = kz_val(k, g.nz, g.mz, g.gam);
+  const double kz = kz_of(k, g.nz, g.mz, g.gam);
   k2 = kx * kx + kz * kz;
 }
 
@@ -591,4 +591,75 @@ k_op_helm(int N, long nl, YCoef c, const double* k2, const cplx* shp,
   cplx* out[1] = {p + l};
   StoreSink<1> sk{out, nl};
   helm_backward<1, 4>(N, y, nl, h, tail, sk);
+}
+
+// ---------------------------------------------------------------------------
+// energy (operators.f90:870-924): sums of |vor|^2 and |phi|^2 over every
+// (y, kx, kz) with kx > 0 counted twice, and of U^2, W^2; a fixed-order
+// two-pass reduction (per-block partials, then one block).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_energy_part(Geom g, const cplx* vor, const cplx* phi, double* part) {
+  const long n = (long)g.my * g.P;
+  double sa = 0.0, sb = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long L = e % g.P;
+    const double wt = (L < g.mz) ? 1.0 : 2.0;   // kx index 0 <=> L < mz
+    sa += wt * cabs2(gld(vor + e));
+    sb += wt * cabs2(gld(phi + e));
+  }
+  __shared__ double ra[256], rb[256];
+  ra[threadIdx.x] = sa;
+  rb[threadIdx.x] = sb;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      ra[threadIdx.x] += ra[threadIdx.x + st];
+      rb[threadIdx.x] += rb[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    gstd(part + 2 * blockIdx.x, ra[0]);
+    gstd(part + 2 * blockIdx.x + 1, rb[0]);
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_energy_final(const double* part, int nb, const cplx* u00, const cplx* w00, int my, double* out) {
+  __shared__ double r[4][256];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    s[0] += gldd(part + 2 * b);
+    s[1] += gldd(part + 2 * b + 1);
+  }
+  for (int j = threadIdx.x; j < my; j += blockDim.x) {
+    const double u = gld(u00 + j).x, w = gld(w00 + j).x;
+    s[2] += u * u;
+    s[3] += w * w;
+  }
+  for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 4; ++q) r[q][threadIdx.x] += r[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) gstd(out + threadIdx.x, r[threadIdx.x][0]);
+}
+
+// phi = d2v - k2 v on every line but the (0,0) one (getv iopt=-1, visc3d
+// with scal = 1, operators.f90:773-790)
+__global__ void __launch_bounds__(256)
+k_lap_combine(Geom g, const cplx* v, const cplx* d2v, cplx* phi) {
+  const long n = (long)g.my * g.P;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long L = e % g.P;
+    if (L == 0) continue;
+    const int i = (int)(L / g.mz), k = (int)(L - (long)i * g.mz);
+    const double kx = g.alp * (double)i, kz = kz_of(k, g.nz, g.mz, g.gam);
+    const double k2 = kx * kx + kz * kz;
+    const cplx a = gld(d2v + e), b = gld(v + e);
+    gst(phi + e, mkc(a.x - k2 * b.x, a.y - k2 * b.y));
+  }
 }
