@@ -20,11 +20,16 @@ the tests:
 * ``fftxz``    -- textbook DFT sums and the 3/2-rule index map;
 * ``cross``    -- curl/divergence identities of the KMM reconstruction,
   orthogonality of the dealiased u x omega term to u, Parseval for the 'f'
-  second moments and means of ``_hist_f`` (``tests/test_oracle_xz.py``).
+  second moments and means of ``_hist_f`` (``tests/test_oracle_xz.py``);
+  the Kelvin-mode closed form (``rdt``, rdtfunc) for the composition of
+  ``Cross.run`` (convergence at the scheme's order) and for every UNER
+  record and per-plane statistic (``tests/test_oracle_rdt.py``); getv round
+  trips, twisted eigenmodes and Parseval for ``getv``/``energy``
+  (``tests/test_oracle_getv.py``);
+* ``rdt``      -- a high-accuracy integration of the Kelvin-mode equations.
 
 Parity unpinned (no check short of the compiled cross.f90, which does not
-build here): the composition inside ``Cross.run`` (order of phase maps,
-shift update, dt clipping, 00-mode and set_conj placement), ``_new_dt``,
-the dissipation sum of ``_hist_f`` and ``_hist_w``.  They follow cross.f90
+build here): the 00-mode forcing path (rf0u/rf0w into U, W), the placement
+of set_conj, ``_new_dt`` beyond the etime clipping.  They follow cross.f90
 line by line, with citations.
 """

@@ -26,6 +26,12 @@ TINY = 1e-13
 NSTAT = 23
 
 
+def _div(a, b):
+    """a / b with IEEE semantics (b = 0 gives inf, as the compiled reference
+    does when a step is clipped to zero length at the etime stop)"""
+    return a / b if b != 0.0 else math.copysign(math.inf, a)
+
+
 def set_conj(f, nz):
     """kx=0 Hermitian symmetry f(0,k) = conj f(0,mz-k), k > nz
     (operators.f90:1382-1429); in place on a [y][kx][kz] array."""
@@ -314,19 +320,19 @@ class Cross:
         if self.CFLv > g.CFL and self.explicit == 1:
             dt = 1.0 / g.cflvv
         self.Deltat = dt
-        self.dtr = g.Re / self.Deltat
+        self.dtr = _div(g.Re, self.Deltat)
         if self.Deltat < 1e-10:
             self.istop = 1
         if self.time + self.Deltat > self.dumptime:
             self.Deltat = abs(self.dumptime - self.time)
-            self.dtr = g.Re / self.Deltat
+            self.dtr = _div(g.Re, self.Deltat)
             if self.dumptime > self.etime - TINY:
                 self.istop = 1
             self.dumptime += self.dtimag
             self.idump = 1
         elif self.time + self.Deltat > self.etime:
             self.Deltat = abs(self.etime - self.time)
-            self.dtr = g.Re / self.Deltat
+            self.dtr = _div(g.Re, self.Deltat)
             self.istop = 1
 
     def _filter_state(self, sht, shb):
@@ -463,13 +469,13 @@ class Cross:
                         self.w00wk = self.w00wk + GAMA[ir] * self.rf0w
                     self.u00wk = self.u00wk + fac * (-0.0 * self.u00 + ops.deriv_real(self.u00, 2)[:, 0])
                     self.w00wk = self.w00wk + fac * (-0.0 * self.w00 + ops.deriv_real(self.w00, 2)[:, 0])
-                    self.vorwk = self.vorwk + (1.0 / self.Deltat) * self.vor
-                    self.phiwk = self.phiwk + (1.0 / self.Deltat) * self.phi
+                    self.vorwk = self.vorwk + _div(1.0, self.Deltat) * self.vor
+                    self.phiwk = self.phiwk + _div(1.0, self.Deltat) * self.phi
                     fac = -g.Re * IBETA[ir]
                     self.vorwk = fac * self.vorwk
                     self.phiwk = fac * self.phiwk
-                    self.u00wk = fac * (self.u00wk + (1.0 / self.Deltat) * self.u00)
-                    self.w00wk = fac * (self.w00wk + (1.0 / self.Deltat) * self.w00)
+                    self.u00wk = fac * (self.u00wk + _div(1.0, self.Deltat) * self.u00)
+                    self.w00wk = fac * (self.w00wk + _div(1.0, self.Deltat) * self.w00)
                     if abs(s) > 0.5:
                         xwkb = self.xoffb + r1 * u00b
                         xwkt = self.xofft + r1 * u00t
