@@ -113,8 +113,12 @@ int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi,
 
 /* ---- the time advance -------------------------------------------------
  * Advances up to nstep RK3 steps (cross.f90:336-1029); returns early when
- * istop or idump is raised.  UNER records are kept on the device; fetch
- * them with snf_history.                                                 */
+ * istop or idump is raised.  Each call is one `call cross` (its own step
+ * count: nhist phase, the reference's unchecked second step), except a
+ * call that follows an early return at a dump (stop == SNF_DUMP with fewer
+ * than nstep steps done): it continues that run, as the reference's loop
+ * continues after writing the dump.  snf_state_upload* starts a new run.
+ * UNER records are kept on the device; fetch them with snf_history.      */
 int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st);
 /* copies up to max_records records (SNF_UNER_LEN doubles each) produced
  * since the last call; *n receives the count                             */

@@ -23,13 +23,15 @@ the tests:
   second moments and means of ``_hist_f`` (``tests/test_oracle_xz.py``);
   the Kelvin-mode closed form (``rdt``, rdtfunc) for the composition of
   ``Cross.run`` (convergence at the scheme's order) and for every UNER
-  record and per-plane statistic (``tests/test_oracle_rdt.py``); getv round
+  record and per-plane statistic, the lift-up closed form of a kx = 0 mode
+  and diffusing mean profiles for the kx = 0 plane and the U, W path
+  (``tests/test_oracle_rdt.py``); getv round
   trips, twisted eigenmodes and Parseval for ``getv``/``energy``
   (``tests/test_oracle_getv.py``);
 * ``rdt``      -- a high-accuracy integration of the Kelvin-mode equations.
 
 Parity unpinned (no check short of the compiled cross.f90, which does not
-build here): the 00-mode forcing path (rf0u/rf0w into U, W), the placement
-of set_conj, ``_new_dt`` beyond the etime clipping.  They follow cross.f90
-line by line, with citations.
+build here): the values of the 00-mode forcing rf0u/rf0w (every closed-form
+state leaves them at rounding level), ``_new_dt`` beyond the etime and
+dump clippings.  They follow cross.f90 line by line, with citations.
 """

@@ -11,6 +11,8 @@ writes that mode as the time-advance state (omega_y, phi = lap v, U, W)
 on the lines of a box, and ``kelvin_lines`` the state the closed form
 predicts at time t, both on the lab-frame y grid y_j = j dy - Ly/2 that the
 shear-periodic lines use (twist exp(-i kx xofft) with xofft = S Ly t).
+``StreamwiseMode`` is the kx = 0 case (lift-up, ``liftup_velocity``), with
+optional sinusoidal mean profiles U(y), W(y) that diffuse viscously.
 """
 import math
 
@@ -78,3 +80,63 @@ class KelvinMode:
         rp = np.linalg.norm(lp - ep) / np.linalg.norm(ep)
         rest = math.sqrt(max(np.linalg.norm(vor) ** 2 - np.linalg.norm(lv) ** 2, 0.0))
         return rv, rp, rest / np.linalg.norm(ev)
+
+
+def liftup_velocity(u0, v0, w0, ky, kz, nu, t, s=1.0):
+    """A streamwise-independent mode (kx = 0) in the shear: its wave vector
+    does not tilt and the pressure term 2 S kx v k/|k|^2 vanishes, so
+    v, w decay viscously and u grows by lift-up, u = (u0 - S v0 t) e^(-nu|k|^2 t)."""
+    if abs(ky * v0 + kz * w0) > 1e-12:
+        raise ValueError("initial mode is not solenoidal")
+    d = math.exp(-nu * (ky * ky + kz * kz) * t)
+    return ((u0 - s * v0 * t) * d, v0 * d, w0 * d)
+
+
+class StreamwiseMode:
+    """The kx = 0 mode at kz index k (and its Hermitian partner at mz - k),
+    n periods over Ly, velocity amplitude amp; optional mean profiles
+    U = A sin(kappa y), W = A cos(kappa y + 0.4), kappa = 2 pi m / Ly, which
+    decay as exp(-nu kappa^2 t)."""
+
+    def __init__(self, g, k=1, n=1, amp=0.1, A=0.0, m=1):
+        if not 1 <= k <= g.nz:
+            raise ValueError("kz index must be in 1..nz")
+        self.g, self.k = g, k
+        self.kz, self.ky = float(g.kz[k]), 2.0 * math.pi * n / g.Ly
+        self.v0 = amp * (1.0 + 0.3j)
+        self.u0 = amp * (0.2 - 0.5j)
+        self.w0 = -self.ky * self.v0 / self.kz
+        self.A, self.kap = float(A), 2.0 * math.pi * m / g.Ly
+
+    def lines(self, t):
+        g = self.g
+        u1, u2, _ = liftup_velocity(self.u0, self.v0, self.w0, self.ky, self.kz, 1.0 / g.Re, t, g.s)
+        e = np.exp(1j * self.ky * g.y)
+        return 1j * self.kz * u1 * e, -(self.ky ** 2 + self.kz ** 2) * u2 * e
+
+    def means(self, t):
+        g = self.g
+        d = math.exp(-self.kap ** 2 * t / g.Re)
+        return self.A * d * np.sin(self.kap * g.y), self.A * d * np.cos(self.kap * g.y + 0.4)
+
+    def state(self):
+        g = self.g
+        vor, phi = g.zeros(), g.zeros()
+        lv, lp = self.lines(0.0)
+        vor[:, 0, self.k], phi[:, 0, self.k] = lv, lp
+        vor[:, 0, g.mz - self.k], phi[:, 0, g.mz - self.k] = np.conj(lv), np.conj(lp)
+        u00, w00 = self.means(0.0)
+        return vor, phi, u00, w00
+
+    def error(self, vor, phi, u00, w00, t):
+        """relative L2 errors: omega_y and phi of the mode's line, of its
+        Hermitian partner line, and, with profiles, of U and W"""
+        g = self.g
+        ev, ep = self.lines(t)
+        rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+        out = [rel(vor[:, 0, self.k], ev), rel(phi[:, 0, self.k], ep),
+               rel(vor[:, 0, g.mz - self.k], np.conj(ev))]
+        if self.A:
+            eu, ew = self.means(t)
+            out += [rel(u00, eu), rel(w00, ew)]
+        return out

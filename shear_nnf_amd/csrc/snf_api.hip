@@ -664,6 +664,7 @@ static int upload_common(snf_handle* h, const std::vector<cplx>& hv, const std::
   s.xofft = xofft;
   s.xwkb = xoffb;
   s.xwkt = xofft;
+  s.halted = 0;   // the next advance starts a new cross call
   HIPCHK(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice));
   h->uploaded = true;
   return SNF_OK;
@@ -757,12 +758,16 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
   DevScalars s0{};
   HIPCHK(hipMemcpy(&s0, h->sc, sizeof(s0), hipMemcpyDeviceToHost));
   const int done0 = s0.steps_done, nrec0 = s0.nrec;
+  // a call that follows a return at a dump continues that run; any other
+  // call is a new cross call
+  const int resume = s0.halted && s0.idump && !s0.istop;
   // start of cross: kx=0 Hermitian fix, twist of the current offsets,
   // v = lapcdy(phi) and the first D1(vor), D1(v)
-  k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur]);
-  k_set_conj<<<g.my, 128, 0, h->stream>>>(g, h->vor, h->phi);
+  k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], resume);
+  if (!resume) k_set_conj<<<g.my, 128, 0, h->stream>>>(g, h->vor, h->phi);
   if ((rc = launch_twist(h, h->cur))) return rc;
-  if (h->p.ydealias == 3) launch_yfilter(h);   // before the step loop (cross.f90:214-225)
+  // before the step loop (cross.f90:214-225)
+  if (h->p.ydealias == 3 && !resume) launch_yfilter(h);
   k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
   HIPCHK(hipGetLastError());
   for (int step = 0; step < nstep; ++step) {
@@ -1021,7 +1026,7 @@ int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const do
   HIPCHK(hipMemcpy(h->dv, hv_dv, NC * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dome, dvordy, NC * sizeof(cplx), hipMemcpyHostToDevice));
   // U, W derivatives as at the top of hvhg (cross.f90:1198-1205)
-  k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur]);
+  k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], 0);
   k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
   int rc;
   if ((rc = launch_xz(h, 2))) return rc;

@@ -279,13 +279,18 @@ k_scalar(ScalarArgs a, int rk) {
   }
 }
 
-// start of an advance call (cross.f90:81-103, 287-291, 383-406)
-__global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm) {
+// start of an advance call (cross.f90:81-103, 287-291, 383-406).  resume:
+// the previous call stopped at a dump and this one continues the same run,
+// as the reference's step loop does after writing the dump (cross.f90:356-373):
+// the step count (nhist phase, the step-2 rule) carries on.
+__global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm, int resume) {
   const int tid = threadIdx.x;
   if (tid == 0) {
     sc->halted = 0;
-    sc->steps_local = 0;
-    sc->istop = 0;
+    if (!resume) {
+      sc->steps_local = 0;
+      sc->istop = 0;
+    }
     sc->idump = 0;
     sc->xwkb = sc->xoffb;
     sc->xwkt = sc->xofft;

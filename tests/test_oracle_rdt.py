@@ -145,3 +145,52 @@ def test_kelvin_mode_history_records():
     c.run(100000)
     assert len(c.history) == c.nacum
     check_kelvin_history(mode, c.history, c.stats)
+
+
+def test_liftup_closed_form_solves_the_kelvin_equations():
+    from oracle.rdt import liftup_velocity
+    nu, t, ky, kz = 1.0 / 300.0, 0.9, 2 * math.pi, 4 * math.pi
+    u0, v0 = 0.02 - 0.05j, 0.1 + 0.03j
+    w0 = -ky * v0 / kz
+    got = liftup_velocity(u0, v0, w0, ky, kz, nu, t)
+    ref = _kelvin_ode(u0, v0, w0, (0.0, ky, kz), nu, t)
+    assert np.linalg.norm(np.array(got) - ref) < 1e-11 * np.linalg.norm(ref)
+
+
+def streamwise_run(CFL, explicit, amp=0.1, A=0.0, nfix=None, grid=(16, 33, 16), Re=300.0):
+    """A run from the kx = 0 mode (and mean profiles); nfix > 0 fixes the
+    step to T_END/nfix through the dump-time clipping (cross.f90:1686-1698)."""
+    from oracle.rdt import StreamwiseMode
+    g = Grid(*grid, Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0, s=1.0, CFL=CFL)
+    mode = StreamwiseMode(g, amp=amp, A=A)
+    c = Cross(g, explicit=explicit)
+    c.set_state(*mode.state())
+    c.etime = T_END
+    if nfix:
+        c.dumptime = c.dtimag = T_END / nfix
+    c.run(100000)
+    assert c.time == pytest.approx(T_END, abs=1e-14)
+    return mode.error(c.vor, c.phi, c.u00, c.w00, c.time), (c.vor, c.phi, c.u00, c.w00)
+
+
+@pytest.mark.parametrize("explicit,order", [(1, 3), (0, 2)])
+def test_streamwise_mode_converges(explicit, order):
+    """the kx = 0 plane: KMM reconstruction with kx = 0, the Hermitian
+    partner line, lift-up by the mean shear"""
+    e1, _ = streamwise_run(0.25, explicit)
+    e2, _ = streamwise_run(0.125, explicit)
+    assert e1[0] / e2[0] > 0.75 * 2 ** order
+    assert e2[0] < (1e-9 if explicit else 1e-7)
+    assert e2[1] < (1e-9 if explicit else 1e-7)
+    assert e2[2] == pytest.approx(e2[0], rel=1e-6)
+
+
+@pytest.mark.parametrize("explicit,nfix,tol", [(1, None, 1e-8), (0, 50, 2e-8)])
+def test_mean_profiles_diffuse(explicit, nfix, tol):
+    """the 00 path (U, W with their RK3 accumulators and viscous terms):
+    sinusoidal profiles decay as exp(-nu kappa^2 t).  A weak kx = 0 mode
+    keeps the run above the laminar stop; its coupling to the profiles is
+    below 3e-9 here.  A wrong 00-mode constant gives errors of order
+    nu kappa^2 dt ~ 1e-3."""
+    e, _ = streamwise_run(0.25, explicit, amp=1.5e-5, A=1e-3, nfix=nfix)
+    assert e[3] < tol and e[4] < tol
