@@ -62,9 +62,12 @@ typedef struct snf_params {
   double etime, dumptime, dtimag;  /* stop / dump times (cross.f90:1680-1712) */
   int32_t precision;          /* SNF_PREC_FP64 (0) or SNF_PREC_MIXED (1)  */
   int32_t device;             /* HIP device ordinal                       */
-  int32_t ydealias;           /* iydealiasing: 0 none (default), 3 compact
-                                 filter of vor, phi, U, W every substep
-                                 (cross.f90:476-488); 1, 2 not built      */
+  int32_t ydealias;           /* iydealiasing (cross.f90:459-491), applied
+                                 to vor, phi, U, W every substep: 0 none
+                                 (default); 1 y-Fourier cut of the kx = 0
+                                 lines and of U, W; 2 shifted cuts of every
+                                 line in the moving frame, offsets wrapped
+                                 to one period; 3 compact filter          */
 } snf_params;
 
 #define SNF_PREC_FP64 0

@@ -59,8 +59,8 @@ class Cross:
         self.readflag = int(readflag)
         self.uprim = float(uprim)
         self.vbulk = float(vbulk)
-        if ydealias not in (0, 3):
-            raise NotImplementedError("iydealiasing 1 and 2 are not restated")
+        if ydealias not in (0, 1, 2, 3):
+            raise ValueError("iydealiasing must be 0, 1, 2 or 3")
         self.ydealias = int(ydealias)
         g = grid
         self.vor = g.zeros()
@@ -335,6 +335,64 @@ class Cross:
             self.dtr = _div(g.Re, self.Deltat)
             self.istop = 1
 
+    def _wrap(self, xoff):
+        """(xoff/Lx - int(xoff/Lx)) Lx, Fortran int() truncating toward zero
+        (cross.f90:349, 734)"""
+        Lx = self.g.Lx
+        return (xoff / Lx - math.trunc(xoff / Lx)) * Lx
+
+    def _dealias_y(self, xwkb):
+        """iydealiasing=1, 2 at the top of a substep (cross.f90:459-475):
+        the state is taken to the moving frame exp(i kx y fac) with fac from
+        the bottom offset, cut in y-Fourier space by dealiasing_y
+        (operators.f90:352-486), and taken back."""
+        g = self.g
+        Lx, Ly = g.Lx, g.Ly
+        fac = self._wrap(xwkb) / Ly
+        if fac < -Lx / Ly * 0.5:
+            fac = Lx / Ly + fac
+        moving_frame(g, self.vor, fac)
+        moving_frame(g, self.phi, fac)
+        self._dealias_lines(fac)
+        moving_frame(g, self.vor, -fac)
+        moving_frame(g, self.phi, -fac)
+
+    def _dealias_lines(self, fac):
+        """dealiasing_y (operators.f90:352-486), variants 1 and 2.  rfty and
+        cfty are FFTW transforms scaled by 1/n forward (rftdoublew.F:133-158,
+        cftdoublew.F:151-201): numpy's forward/inverse pair has the same
+        product.  U, W: the real-to-complex coefficients from real index
+        my23 = 2 (my/3) on are zeroed, i.e. modes k >= my/3.  vor, phi:
+        coefficient j of a line is zeroed inside the window of its kx index."""
+        g = self.g
+        my, Lx, Ly = g.my, g.Lx, g.Ly
+        kcut = (2 * (my // 3)) // 2
+        for name in ("u00", "w00"):
+            c = np.fft.rfft(getattr(self, name))
+            c[kcut:] = 0.0
+            setattr(self, name, np.fft.irfft(c, n=my))
+        my23 = 2 * (my // 3)
+        ny1 = ((my23 - 1) - 1) // 2
+        ny2 = my - ny1
+        mx1 = g.kxn - 1
+        j = np.arange(my)
+        rows = (0,) if self.ydealias == 1 else range(g.kxn)
+        for i in rows:
+            if self.ydealias == 1 or fac <= 0.0:
+                djda = ny1 / mx1 * (-fac - Lx / Ly) * Ly / Lx * i
+            else:
+                djda = ny1 / mx1 * (-fac + Lx / Ly) * Ly / Lx * i
+            djdb = ny1 / mx1 * (-fac) * Ly / Lx * i
+            if self.ydealias == 1:
+                cut = (j > ny1 + djdb) & (j < ny2 + djdb)
+            else:
+                cut = ((j >= ny1 + djdb) & (j <= ny2 + djdb)) | \
+                      ((j > ny1 + djda) & (j < ny2 + djda))
+            for f in (self.vor, self.phi):
+                c = np.fft.fft(f[:, i, :], axis=0)
+                c[cut, :] = 0.0
+                f[:, i, :] = np.fft.ifft(c, axis=0)
+
     def _filter_state(self, sht, shb):
         """iydealiasing=3: compact filter of vor, phi (twisted) and U, W
         (cross.f90:214-225, 476-488; filterc filter_shear_hepta.f90:128-165)"""
@@ -372,6 +430,9 @@ class Cross:
             self.istep = istep
             if (istep - 1) % self.nhist == 0:
                 ihist = 1
+            if self.ydealias == 2:
+                self.xoffb = self._wrap(self.xoffb)
+                self.xofft = -self.xoffb
             xwkt, xwkb = self.xofft, self.xoffb
             if istep - 1 != 1:
                 if (((istep - 1) % self.nimag == 0 and istep != 1)
@@ -399,6 +460,8 @@ class Cross:
                 self.hv = self._lines_d(self.hg, shwkt, shwkb, 1)
                 if self.ydealias == 3:
                     self._filter_state(shwkt, shwkb)
+                elif self.ydealias in (1, 2):
+                    self._dealias_y(xwkb)
                 self.hvhg(rk, ihist)
                 if ihist and rk == 1:
                     ihist = 0
@@ -431,6 +494,9 @@ class Cross:
                     if abs(s) > 0.01:
                         xwkb = self.xoffb + r1 * u00b
                         xwkt = self.xofft + r1 * u00t
+                        if self.ydealias == 2:
+                            xwkb = self._wrap(xwkb)
+                            xwkt = -xwkb
                         shwkb = self._shift(xwkb)
                         shwkt = self._shift(xwkt)
                     if rk != 3:

@@ -20,6 +20,7 @@
 #include "snf_scalar.hip"
 #include "snf_ykern.hip"
 #include "snf_yfk.hip"
+#include "snf_ydeal.hip"
 
 namespace {
 
@@ -332,6 +333,7 @@ ScalarArgs sargs(snf_handle* h) {
   a.cap = h->cap;
   a.shp_new = h->shp[1 - h->cur];
   a.shm_new = h->shm[1 - h->cur];
+  a.wrap_lx = h->p.ydealias == 2 ? 2.0 * M_PI / h->g.alp : 0.0;
   return a;
 }
 
@@ -346,6 +348,28 @@ void launch_yf(snf_handle* h, int rk) {
 }
 
 // iydealiasing=3: compact filter of vor, phi, U, W at the current twist
+// iydealiasing 1, 2 at the top of a substep (cross.f90:459-475)
+void launch_ydealias(snf_handle* h) {
+  const Geom& g = h->g;
+  YDealiasArgs a;
+  a.g = g;
+  a.vor = h->vor;
+  a.phi = h->phi;
+  a.u00c = h->u00c;
+  a.w00c = h->w00c;
+  a.twy = h->twy;
+  a.sc = h->sc;
+  a.variant = h->p.ydealias;
+  const int my23 = 2 * (g.my / 3), ny = ((my23 - 1) - 1) / 2;
+  a.ny1 = ny;
+  a.ny2 = g.my - ny;
+  a.kcut = my23 / 2;
+  const long lines = a.variant == 1 ? g.mz : g.P;
+  k_ydealias<<<(unsigned)(lines + 1), 64, 3 * (size_t)g.my * sizeof(cplx), h->stream>>>(a);
+  // dU/dy, dW/dy of the cut profiles, as hvhg forms them (cross.f90:1198-1205)
+  k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
+}
+
 void launch_yfilter(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
@@ -392,8 +416,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
   if (!(p->Re > 0) || !(p->alp > 0) || !(p->gam > 0) || !(p->Ly > 0) || !(p->CFL > 0))
     return fail(SNF_BAD_PARAM, "Re, alp, gam, Ly, CFL must be positive");
   if (p->precision != SNF_PREC_FP64) return fail(SNF_BAD_PARAM, "only SNF_PREC_FP64 is built");
-  if (p->ydealias != 0 && p->ydealias != 3)
-    return fail(SNF_BAD_PARAM, "ydealias: 0 (none) or 3 (compact filter); 1 and 2 are not built");
+  if (p->ydealias < 0 || p->ydealias > 3)
+    return fail(SNF_BAD_PARAM, "ydealias: 0 (none), 1, 2 (y-Fourier cuts) or 3 (compact filter)");
   if (p->ydealias == 3 && !y_plan(p->my))
     return fail(SNF_BAD_PARAM, "ydealias=3 needs a compiled y-transform plan (my = 33, 49, 385)");
   snf_handle* h = new snf_handle();
@@ -608,6 +632,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
   h->ldsA = 2 * 3 * (size_t)mgalz * sizeof(cplx);   // ping-pong, 3 sequences
   h->ldsB = 3 * (size_t)h->zb * mgalx * sizeof(cplx);
   h->ldsC = 2 * 3 * (size_t)mgalz * sizeof(cplx);
+  (void)hipFuncSetAttribute((const void*)k_ydealias, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(3 * (size_t)my * sizeof(cplx)));
   (void)hipFuncSetAttribute((const void*)k_xz_A, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
   for (const void* kb : {(const void*)k_xz_B<256>, (const void*)k_xz_B<512>, (const void*)k_xz_B<1024>})
     (void)hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
@@ -771,11 +797,12 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
   k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
   HIPCHK(hipGetLastError());
   for (int step = 0; step < nstep; ++step) {
-    k_step_begin<<<1, 1, 0, h->stream>>>(h->sc);
+    k_step_begin<<<1, 1, 0, h->stream>>>(h->sc, h->p.ydealias == 2 ? 2.0 * M_PI / g.alp : 0.0);
     for (int rk = 1; rk <= 3; ++rk) {
       // d(omega_y)/dy and dv/dy of this substep were formed from the
       // unfiltered fields, as in cross.f90:434-445 before the filter of 476-488
       if (h->p.ydealias == 3) launch_yfilter(h);
+      if (h->p.ydealias == 1 || h->p.ydealias == 2) launch_ydealias(h);
       if ((rc = launch_xz(h, rk))) return rc;
       {
         TimedLaunch tl(h, SNF_K_SMALL);

@@ -133,6 +133,7 @@ struct ScalarArgs {
   double* ring;               // [cap][24]
   int cap;
   cplx *shp_new, *shm_new;
+  double wrap_lx;             // Lx when the offsets wrap (iydealiasing=2), else 0
 };
 
 __global__ void __launch_bounds__(256)
@@ -257,6 +258,11 @@ k_scalar(ScalarArgs a, int rk) {
   if (fabs(g.s) > thr) {
     xwkb = sc->xoffb + r1 * (g.s * (-g.Ly));
     xwkt = sc->xofft + r1 * (g.s * g.Ly);
+    if (a.explicit_visc == 1 && a.wrap_lx > 0.0) {   // iydealiasing=2 (cross.f90:733-736)
+      const double q = xwkb / a.wrap_lx;
+      xwkb = (q - trunc(q)) * a.wrap_lx;
+      xwkt = -xwkb;
+    }
   }
   for (int i = tid; i < g.kxn; i += blockDim.x) {
     const double kx = g.alp * (double)i;
@@ -308,8 +314,17 @@ __global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm, int r
 
 // the reference checks istop/idump at the top of every step except the
 // second one (cross.f90:353-374)
-__global__ void k_step_begin(DevScalars* sc) {
+// wrap_lx > 0 (iydealiasing=2): the offsets are first wrapped into one
+// period, xoffb = (xoffb/Lx - int(xoffb/Lx)) Lx, xofft = -xoffb (cross.f90:348-352)
+__global__ void k_step_begin(DevScalars* sc, double wrap_lx) {
   if (sc->halted) return;
+  if (wrap_lx > 0.0) {
+    const double q = sc->xoffb / wrap_lx;
+    sc->xoffb = (q - trunc(q)) * wrap_lx;
+    sc->xofft = -sc->xoffb;
+    sc->xwkb = sc->xoffb;
+    sc->xwkt = sc->xofft;
+  }
   const int next = sc->steps_local + 1;
   if (next - 1 != 1 && (sc->istop || sc->idump)) {
     sc->halted = 1;
