@@ -60,6 +60,31 @@ __device__ __forceinline__ void yf_flush(const cplx* T, cplx* F, long P, long L0
   __syncthreads();
 }
 
+// The same transfer split in two, so that a field's loads are in flight
+// while the block transforms the previous one: yf_issue loads this thread's
+// Q elements of the tile into registers, yf_commit writes them to the tile.
+template <int Q>
+__device__ __forceinline__ void yf_issue(cplx (&r)[Q], const cplx* F, long P, long L0, int nv,
+                                         int N) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = threadIdx.x + q * 64 * YF_LINES;
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    r[q] = (e < N * YF_LINES && c < nv) ? gld(F + (long)j * P + L0 + c) : mkc(0.0, 0.0);
+  }
+}
+template <int Q>
+__device__ __forceinline__ void yf_commit(cplx* T, const cplx (&r)[Q], int N) {
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = threadIdx.x + q * 64 * YF_LINES;
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    if (e < N * YF_LINES) T[TileCol{c}(j)] = r[q];
+  }
+  __syncthreads();
+}
+
 // exp(i w_m) of mode m: exp(2 pi i m/N) exp(i theta/N)
 __device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
   return cmul(cconj(gld(rt + m)), eth);
@@ -188,7 +213,10 @@ k_yf_explicit(YArgs a, int rk) {
   }
 
   // ===== omega_y chain =====
+  // one field's loads stay in flight in pf while the block transforms
+  cplx pf[Q];
   yf_fill(T, a.vor, P, L0, nv, N);
+  yf_issue(pf, a.rhg, P, L0, nv, N);
   {
     YPhase ph(-tho, 0.0, lane);
 #pragma unroll
@@ -212,7 +240,8 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-  yf_fill(T, a.rhg, P, L0, nv, N);
+  yf_commit(T, pf, N);                     // rhg
+  if (rk > 1) yf_issue(pf, a.vorwk, P, L0, nv, N);
   {
     YPhase ph(tho, 0.0, lane);
 #pragma unroll
@@ -222,7 +251,7 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hg
     }
   }
-  if (rk > 1) yf_fill(T, a.vorwk, P, L0, nv, N);
+  if (rk > 1) yf_commit(T, pf, N);        // vorwk
   {
     YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
     YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
@@ -254,6 +283,7 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_flush(T, a.vor, P, L0, nv, N);
+  yf_issue(pf, a.rdome, P, L0, nv, N);
   yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -276,7 +306,8 @@ k_yf_explicit(YArgs a, int rk) {
   yf_flush(T, a.dome, P, L0, nv, N);
 
   // ===== phi chain =====
-  yf_fill(T, a.rdome, P, L0, nv, N);
+  yf_commit(T, pf, N);                     // rdome
+  yf_issue(pf, a.phi, P, L0, nv, N);
   {
     YPhase ph(-tho, 0.0, lane);
 #pragma unroll
@@ -295,7 +326,8 @@ k_yf_explicit(YArgs a, int rk) {
       T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
     }
   }
-  yf_fill(T, a.phi, P, L0, nv, N);
+  yf_commit(T, pf, N);                     // phi
+  yf_issue(pf, a.rhv, P, L0, nv, N);
   {
     YPhase ph(-tho, 0.0, lane);
 #pragma unroll
@@ -319,7 +351,8 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-  yf_fill(T, a.rhv, P, L0, nv, N);
+  yf_commit(T, pf, N);                     // rhv
+  if (rk > 1) yf_issue(pf, a.phiwk, P, L0, nv, N);
   {
     YPhase ph(tho, 0.0, lane);
 #pragma unroll
@@ -329,7 +362,7 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hv
     }
   }
-  if (rk > 1) yf_fill(T, a.phiwk, P, L0, nv, N);
+  if (rk > 1) yf_commit(T, pf, N);        // phiwk
   {
     YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
     YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
