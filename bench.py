@@ -29,6 +29,9 @@ def parse():
     ap.add_argument("--Re", type=float, default=8000.0)
     ap.add_argument("--explicit", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed precision leg")
+    ap.add_argument("--cmp-steps", type=int, default=500,
+                    help="steps of the FP64-vs-mixed statistics comparison")
     ap.add_argument("--cpu-grid", type=str, default="",
                     help="grid of the CPU baseline sample (default: the workload grid)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -100,6 +103,55 @@ def cpu_baseline(grid, sample_grid, Re, explicit, seed):
                         f"{grid[0]}x{grid[1]}x{grid[2]} by Nc ratio {scale:.4f}")
 
 
+def uner_stats(hist):
+    """energy 0.5 (u'^2 + v'^2 + w'^2), Reynolds stress <uv>, dissipation of
+    every UNER record (time, Deltat, uner(10), ...; cross.f90:2067-2119)"""
+    import numpy as np
+    h = np.asarray(hist)
+    return {"energy": 0.5 * np.sum(h[:, 2:5] ** 2, axis=1), "reynolds_stress": h[:, 5],
+            "dissipation": h[:, 11]}
+
+
+def timed_leg(snf, ic, grid, Re, explicit, steps, warmup, precision):
+    """warm-up, then exactly `steps` steps in one advance call (which
+    synchronises the device before it returns), with per-kernel HIP events"""
+    box = snf.ShearBox(*grid, Re, explicit=explicit, nhist=10 ** 9, precision=precision)
+    box.set_state(*ic)
+    box.advance_async_status(warmup)
+    box.set_timing(True)
+    t0 = time.perf_counter()
+    st = box.advance_async_status(steps)
+    t1 = time.perf_counter()
+    timing = box.get_timing()
+    box.set_timing(False)
+    kxn, mz, nc = box.kxn, box.mz, box.nc
+    box.close()
+    return st, t1 - t0, timing, (kxn, mz, nc)
+
+
+def mixed_error(snf, ic, grid, Re, explicit, nstep):
+    """FP64 and mixed runs of nstep steps from the same state with a UNER
+    record every step: the largest relative difference of each statistic,
+    and the relative L2 difference of the final fields"""
+    import numpy as np
+    recs, fields = {}, {}
+    for prec in ("fp64", "mixed"):
+        box = snf.ShearBox(*grid, Re, explicit=explicit, nhist=1, precision=prec)
+        box.set_state(*ic)
+        box.cross(nstep)
+        recs[prec] = uner_stats(box.history)
+        fields[prec] = box.get_state()
+        box.close()
+    n = min(len(recs["fp64"]["energy"]), len(recs["mixed"]["energy"]))
+    err = {k: float(np.max(np.abs(recs["mixed"][k][:n] - recs["fp64"][k][:n])
+                           / np.abs(recs["fp64"][k][:n]))) for k in recs["fp64"]}
+    names = ("vor", "phi", "u00", "w00")
+    err["fields_rel_l2"] = {nm: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                            for nm, a, b in zip(names, fields["mixed"], fields["fp64"])}
+    err["steps"] = n
+    return err
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -117,25 +169,16 @@ def main():
         return
     mgalx, my, mgalz = (int(x) for x in a.grid.split(","))
     seed = 20260001
-    vor, phi, u00, w00 = make_ic(mgalx, my, mgalz, math.pi, 2 * math.pi, 1.0, seed=seed)
-    box = snf.ShearBox(mgalx, my, mgalz, a.Re, explicit=a.explicit, nhist=10 ** 9)
-    box.set_state(vor, phi, u00, w00)
-    del vor, phi
-    st = box.advance_async_status(a.warmup)          # warm-up (synchronises at the end)
+    ic = make_ic(mgalx, my, mgalz, math.pi, 2 * math.pi, 1.0, seed=seed)
+    grid = (mgalx, my, mgalz)
     # timed region: exactly K steps in one advance call, which synchronises the
     # device before it returns; per-kernel HIP events on the library's stream
-    box.set_timing(True)
-    t0 = time.perf_counter()
-    st2 = box.advance_async_status(a.steps)
-    t1 = time.perf_counter()
-    timing = box.get_timing()
-    box.set_timing(False)
+    st2, wall, timing, (kxn, mz, nc) = timed_leg(snf, ic, grid, a.Re, a.explicit, a.steps,
+                                                 a.warmup, "fp64")
     st = st2
     steps_done = st2.steps_done
-    wall = t1 - t0
     sps = steps_done / wall
-    nc = box.nc
-    ab = algorithmic_bytes(box.kxn, box.mz, my, mgalz)
+    ab = algorithmic_bytes(kxn, mz, my, mgalz)
     per_cls = {}
     for k, (ms, cnt) in timing.items():
         if cnt:
@@ -144,7 +187,7 @@ def main():
     achieved = ab[dom] / (per_cls[dom] * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(dom, (mgalx, my, mgalz), a.explicit)
     out = {
-        "metric": "RK3 steps/s at 384^3 (FP64), % HBM peak",
+        "metric": "RK3 steps/s at 384\u00b3 (FP64 vs mixed/batched, rel err vs FP64); % HBM peak",
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -173,6 +216,19 @@ def main():
         "sim_time_advanced": st2.time,
         "Deltat": st2.Deltat,
     }
+    if not a.no_mixed:
+        stm, wallm, timm, _ = timed_leg(snf, ic, grid, a.Re, a.explicit, a.steps, a.warmup, "mixed")
+        spsm = stm.steps_done / wallm
+        out["mixed"] = {
+            "value": spsm, "unit": "steps/s", "speedup_vs_fp64": spsm / sps,
+            "ms_per_step": 1e3 * wallm / max(stm.steps_done, 1),
+            "dtype": "f32 x-z transforms, hybrid planes and u x omega; f64 state, "
+                     "KMM reconstruction, y operators, RK update, statistics sums",
+            "kernel_ms_per_launch": {k: ms / cnt for k, (ms, cnt) in timm.items() if cnt},
+            "step_roofline_frac": step_bytes(nc) * spsm / 1e9 / HBM_PEAK_GBS,
+            "rel_err_vs_fp64": mixed_error(snf, ic, grid, a.Re, a.explicit, a.cmp_steps)
+            if a.cmp_steps > 0 else None,
+        }
     if not a.no_cpu:
         if a.cpu_grid:
             cg = tuple(int(x) for x in a.cpu_grid.split(","))

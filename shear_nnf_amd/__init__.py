@@ -24,6 +24,7 @@ LIB_PATH = os.environ.get("SNF_LIB") or os.path.join(_HERE, "libshear_rk3.so")
 
 SNF_OK, SNF_BAD_PARAM, SNF_DEVICE_ERROR, SNF_NO_DEVICE, SNF_NOT_READY = 0, 1, 2, 3, 4
 SNF_RUNNING, SNF_STOP, SNF_DUMP = 0, 1, 2
+SNF_PREC_FP64, SNF_PREC_MIXED = 0, 1
 UNER_LEN = 24
 EXPORTS = (
     "snf_create", "snf_destroy", "snf_last_error", "snf_sizes",
@@ -126,10 +127,14 @@ class ShearBox:
 
     def __init__(self, mgalx, my, mgalz, Re, alp=np.pi, gam=2 * np.pi, Ly=1.0, s=1.0,
                  chi=0.0, CFL=0.5, explicit=1, nhist=1, readflag=1, uprim=0.1, vbulk=0.0,
-                 etime=1e10, dumptime=1e10, dtimag=1e10, device=0, ydealias=0):
+                 etime=1e10, dumptime=1e10, dtimag=1e10, device=0, ydealias=0,
+                 precision="fp64"):
+        """precision: "fp64", or "mixed" (FP32 x-z transforms and nonlinear
+        products, FP64 state, y operators and statistics sums)."""
         self.lib = load_library()
+        prec = {"fp64": SNF_PREC_FP64, "mixed": SNF_PREC_MIXED}[precision]
         p = SnfParams(mgalx, my, mgalz, Re, alp, gam, Ly, s, chi, CFL, explicit, nhist,
-                      readflag, uprim, vbulk, etime, dumptime, dtimag, 0, device, ydealias)
+                      readflag, uprim, vbulk, etime, dumptime, dtimag, prec, device, ydealias)
         h = ctypes.c_void_p()
         self._check(self.lib.snf_create(ctypes.byref(p), ctypes.byref(h)))
         self.h = h

@@ -99,6 +99,7 @@ struct snf_handle {
   int bthreads = 1024;   // threads per block of the x-row kernel
   // wave-FFT x-z kernels (planes of 64*R points), rx = mgalx/64, rz = mgalz/64
   bool wave_xz = false;
+  bool mixed = false;     // SNF_PREC_MIXED: FP32 x-z transforms and hybrid planes
   int rx = 0, rz = 0;
   size_t ldsWA = 0, ldsWB = 0, ldsWC = 0;
   double* upart = nullptr;   // per-row umax partials of the wave x-row kernel
@@ -232,26 +233,54 @@ void collect_timing(snf_handle* h) {
   h->ev_used = 0;
 }
 
-template <int R>
+// H: element type of the x-z transforms and hybrid planes (cplx, or cplxf
+// in the mixed precision mode)
+template <int R, class H>
 void launch_xzw_A(snf_handle* h) {
   const Geom& g = h->g;
   const unsigned nb = (unsigned)(((long)g.my * g.kxn + WA_WAVES - 1) / WA_WAVES);
-  k_xzw_A<R><<<nb, 64 * WA_WAVES, h->ldsWA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome,
-      h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->twz, h->sc);
+  k_xzw_A<R, H><<<nb, 64 * WA_WAVES, h->ldsWA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv,
+      h->dome, h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk, reinterpret_cast<H*>(h->hyb_in),
+      h->twz, h->sc);
 }
-template <int R>
+template <int R, class H>
 void launch_xzw_B(snf_handle* h, int rk) {
   const Geom& g = h->g;
   dim3 gB(g.mgalz / WB_ROWS, g.my);
-  k_xzw_B<R><<<gB, 64 * WB_ROWS, h->ldsWB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->twx, rk,
-      h->upart, h->phist, h->sc, h->p.nhist);
+  k_xzw_B<R, H><<<gB, 64 * WB_ROWS, h->ldsWB, h->stream>>>(g, reinterpret_cast<const H*>(h->hyb_in),
+      reinterpret_cast<H*>(h->hyb_out), h->twx, rk, h->upart, h->phist, h->sc, h->p.nhist);
 }
-template <int R>
+template <int R, class H>
 void launch_xzw_C(snf_handle* h) {
   const Geom& g = h->g;
   const unsigned nb = (unsigned)(((long)g.my * g.kxn + WC_WAVES - 1) / WC_WAVES);
-  k_xzw_C<R><<<nb, 64 * WC_WAVES, h->ldsWC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg,
-      h->rhv, h->rf0u, h->rf0w, h->twz, h->sc);
+  k_xzw_C<R, H><<<nb, 64 * WC_WAVES, h->ldsWC, h->stream>>>(g, reinterpret_cast<const H*>(h->hyb_out),
+      h->v, h->rdome, h->rhg, h->rhv, h->rf0u, h->rf0w, h->twz, h->sc);
+}
+template <class H>
+void launch_xzw(snf_handle* h, int which, int rk) {
+  const int r = which == 1 ? h->rx : h->rz;
+  switch (which * 8 + r) {
+    case 2: launch_xzw_A<2, H>(h); break;
+    case 3: launch_xzw_A<3, H>(h); break;
+    case 4: launch_xzw_A<4, H>(h); break;
+    case 6: launch_xzw_A<6, H>(h); break;
+    case 10: launch_xzw_B<2, H>(h, rk); break;
+    case 11: launch_xzw_B<3, H>(h, rk); break;
+    case 12: launch_xzw_B<4, H>(h, rk); break;
+    case 14: launch_xzw_B<6, H>(h, rk); break;
+    case 18: launch_xzw_C<2, H>(h); break;
+    case 19: launch_xzw_C<3, H>(h); break;
+    case 20: launch_xzw_C<4, H>(h); break;
+    default: launch_xzw_C<6, H>(h); break;
+  }
+}
+// which: 0 = A, 1 = B, 2 = C
+void launch_xzw_any(snf_handle* h, int which, int rk) {
+  if (h->mixed)
+    launch_xzw<cplxf>(h, which, rk);
+  else
+    launch_xzw<cplx>(h, which, rk);
 }
 
 int launch_xz(snf_handle* h, int rk) {
@@ -259,13 +288,8 @@ int launch_xz(snf_handle* h, int rk) {
   // the x-z stage reads U, W as real arrays
   k_cplx_to_real<<<(g.my + 255) / 256, 256, 0, h->stream>>>(h->u00c, h->w00c, h->u00r, h->w00r, g.my);
   if (h->wave_xz) {
-    { TimedLaunch tl(h, SNF_K_XZ_A);
-      switch (h->rz) {
-        case 2: launch_xzw_A<2>(h); break;
-        case 3: launch_xzw_A<3>(h); break;
-        case 4: launch_xzw_A<4>(h); break;
-        default: launch_xzw_A<6>(h); break;
-      } }
+    TimedLaunch tl(h, SNF_K_XZ_A);
+    launch_xzw_any(h, 0, rk);
   } else {
     dim3 gA(g.kxn, g.my);
     TimedLaunch tl(h, SNF_K_XZ_A);
@@ -279,19 +303,9 @@ int launch_xz(snf_handle* h, int rk) {
                                           h->sc, h->p.nhist);
   if (h->wave_xz) {
     { TimedLaunch tl(h, SNF_K_XZ_B);
-      switch (h->rx) {
-        case 2: launch_xzw_B<2>(h, rk); break;
-        case 3: launch_xzw_B<3>(h, rk); break;
-        case 4: launch_xzw_B<4>(h, rk); break;
-        default: launch_xzw_B<6>(h, rk); break;
-      } }
+      launch_xzw_any(h, 1, rk); }
     { TimedLaunch tl(h, SNF_K_XZ_C);
-      switch (h->rz) {
-        case 2: launch_xzw_C<2>(h); break;
-        case 3: launch_xzw_C<3>(h); break;
-        case 4: launch_xzw_C<4>(h); break;
-        default: launch_xzw_C<6>(h); break;
-      } }
+      launch_xzw_any(h, 2, rk); }
   } else {
     dim3 gA(g.kxn, g.my);
     dim3 gB(g.mgalz / h->zb, g.my);
@@ -415,13 +429,15 @@ int snf_create(const snf_params* p, snf_handle** out) {
   if (p->nhist < 1) return fail(SNF_BAD_PARAM, "nhist must be >= 1");
   if (!(p->Re > 0) || !(p->alp > 0) || !(p->gam > 0) || !(p->Ly > 0) || !(p->CFL > 0))
     return fail(SNF_BAD_PARAM, "Re, alp, gam, Ly, CFL must be positive");
-  if (p->precision != SNF_PREC_FP64) return fail(SNF_BAD_PARAM, "only SNF_PREC_FP64 is built");
+  if (p->precision != SNF_PREC_FP64 && p->precision != SNF_PREC_MIXED)
+    return fail(SNF_BAD_PARAM, "precision: SNF_PREC_FP64 or SNF_PREC_MIXED");
   if (p->ydealias < 0 || p->ydealias > 3)
     return fail(SNF_BAD_PARAM, "ydealias: 0 (none), 1, 2 (y-Fourier cuts) or 3 (compact filter)");
   if (p->ydealias == 3 && !y_plan(p->my))
     return fail(SNF_BAD_PARAM, "ydealias=3 needs a compiled y-transform plan (my = 33, 49, 385)");
   snf_handle* h = new snf_handle();
   h->p = *p;
+  h->mixed = p->precision == SNF_PREC_MIXED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
     delete h;
@@ -613,16 +629,30 @@ int snf_create(const snf_params* p, snf_handle** out) {
     auto wave_size = [](int n) { return n % 64 == 0 && (n / 64 == 2 || n / 64 == 3 || n / 64 == 4 || n / 64 == 6); };
     const char* e = getenv("SNF_XZ_PATH");
     h->wave_xz = wave_size(mgalx) && wave_size(mgalz) && !(e && std::string(e) == "generic");
+    if (h->mixed && !h->wave_xz) {
+      snf_destroy(h);
+      return fail(SNF_BAD_PARAM, "SNF_PREC_MIXED needs mgalx, mgalz in {128, 192, 256, 384}");
+    }
     if (h->wave_xz) {
       h->rx = mgalx / 64;
       h->rz = mgalz / 64;
-      h->ldsWA = xzw_lds_A(h->rz);
-      h->ldsWC = xzw_lds_C(h->rz);
-      h->ldsWB = xzw_lds_B(h->rx, g.kxn);
-      const void* ka[4] = {(const void*)k_xzw_A<2>, (const void*)k_xzw_A<3>, (const void*)k_xzw_A<4>, (const void*)k_xzw_A<6>};
-      const void* kb[4] = {(const void*)k_xzw_B<2>, (const void*)k_xzw_B<3>, (const void*)k_xzw_B<4>, (const void*)k_xzw_B<6>};
-      const void* kc[4] = {(const void*)k_xzw_C<2>, (const void*)k_xzw_C<3>, (const void*)k_xzw_C<4>, (const void*)k_xzw_C<6>};
-      for (int q = 0; q < 4; ++q) {
+      const size_t es = h->mixed ? sizeof(cplxf) : sizeof(cplx);
+      h->ldsWA = xzw_lds_A(h->rz, es);
+      h->ldsWC = xzw_lds_C(h->rz, es);
+      h->ldsWB = xzw_lds_B(h->rx, g.kxn, es);
+      const void* ka[8] = {(const void*)k_xzw_A<2, cplx>, (const void*)k_xzw_A<3, cplx>,
+                           (const void*)k_xzw_A<4, cplx>, (const void*)k_xzw_A<6, cplx>,
+                           (const void*)k_xzw_A<2, cplxf>, (const void*)k_xzw_A<3, cplxf>,
+                           (const void*)k_xzw_A<4, cplxf>, (const void*)k_xzw_A<6, cplxf>};
+      const void* kb[8] = {(const void*)k_xzw_B<2, cplx>, (const void*)k_xzw_B<3, cplx>,
+                           (const void*)k_xzw_B<4, cplx>, (const void*)k_xzw_B<6, cplx>,
+                           (const void*)k_xzw_B<2, cplxf>, (const void*)k_xzw_B<3, cplxf>,
+                           (const void*)k_xzw_B<4, cplxf>, (const void*)k_xzw_B<6, cplxf>};
+      const void* kc[8] = {(const void*)k_xzw_C<2, cplx>, (const void*)k_xzw_C<3, cplx>,
+                           (const void*)k_xzw_C<4, cplx>, (const void*)k_xzw_C<6, cplx>,
+                           (const void*)k_xzw_C<2, cplxf>, (const void*)k_xzw_C<3, cplxf>,
+                           (const void*)k_xzw_C<4, cplxf>, (const void*)k_xzw_C<6, cplxf>};
+      for (int q = 0; q < 8; ++q) {
         (void)hipFuncSetAttribute(ka[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWA);
         (void)hipFuncSetAttribute(kb[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWB);
         (void)hipFuncSetAttribute(kc[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWC);

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 typedef double2 cplx;
+typedef float2 cplxf;   // FP32 complex: x-z stage of the mixed precision mode
 
 #define SNF_MAXN 1032      // largest my supported (line length)
 #define SNF_MAXFFT 1024    // largest mgalx / mgalz
@@ -36,6 +37,23 @@ __device__ __forceinline__ void gst(cplx* p, cplx v) {
   d[0] = v.x;
   d[1] = v.y;
 }
+__device__ __forceinline__ cplxf gld(const cplxf* p) {
+  const SNF_AS1 float* d = (const SNF_AS1 float*)p;
+  return make_float2(d[0], d[1]);
+}
+__device__ __forceinline__ void gst(cplxf* p, cplxf v) {
+  SNF_AS1 float* d = (SNF_AS1 float*)p;
+  d[0] = v.x;
+  d[1] = v.y;
+}
+// conversions between the FP64 fields and an x-z element type H
+template <class H> __device__ __forceinline__ H to_h(cplx a);
+template <> __device__ __forceinline__ cplx to_h<cplx>(cplx a) { return a; }
+template <> __device__ __forceinline__ cplxf to_h<cplxf>(cplx a) {
+  return make_float2((float)a.x, (float)a.y);
+}
+__device__ __forceinline__ cplx to_d(cplx a) { return a; }
+__device__ __forceinline__ cplx to_d(cplxf a) { return make_double2((double)a.x, (double)a.y); }
 __device__ __forceinline__ double gldd(const double* p) { return *(const SNF_AS1 double*)p; }
 __device__ __forceinline__ void gstd(double* p, double v) { *(SNF_AS1 double*)p = v; }
 

@@ -31,26 +31,29 @@ __device__ __forceinline__ int wz_k(int pos, int nz, int mz, int Nz) {
   return -1;
 }
 
-__device__ __forceinline__ void stage_roots(cplx* rt, const cplx* tw, int n) {
-  for (int t = threadIdx.x; t < n; t += blockDim.x) rt[t] = gld(tw + t);
+template <class H>
+__device__ __forceinline__ void stage_roots(H* rt, const cplx* tw, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) rt[t] = to_h<H>(gld(tw + t));
 }
 
 // ---------------------------------------------------------------------------
-template <int RZ>
+template <int RZ, class H>
 __global__ void __launch_bounds__(64 * WA_WAVES)
 k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
         const cplx* __restrict__ hg, const cplx* __restrict__ hv,
         const cplx* __restrict__ dvordy, const double* __restrict__ u00,
         const double* __restrict__ w00, const double* __restrict__ dudy,
-        const double* __restrict__ dwdy, double vbulk, cplx* __restrict__ hyb,
+        const double* __restrict__ dwdy, double vbulk, H* __restrict__ hyb,
         const cplx* __restrict__ twz, const DevScalars* sc) {
-  typedef WFFT<RZ, 1> W;
+  typedef WFFT<RZ, 1, H> W;
+  typedef typename wf_real<H>::T R;
   if (sc->halted) return;
-  extern __shared__ cplx lds[];
-  cplx* rt = lds;
+  extern __shared__ double4 xzw_smem[];
+  H* lds = reinterpret_cast<H*>(xzw_smem);
+  H* rt = lds;
   const int Nz = 64 * RZ;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  cplx* buf = lds + Nz + wave * W::LDS;
+  H* buf = lds + Nz + wave * W::LDS;
   stage_roots(rt, twz, Nz);
   __syncthreads();
   const long row = (long)blockIdx.x * WA_WAVES + wave;   // row = j*kxn + i
@@ -58,7 +61,7 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
   const int j = (int)(row / g.kxn), i = (int)(row - (long)j * g.kxn);
   const double u00j = gldd(u00 + j), w00j = gldd(w00 + j);
   const double dudyj = gldd(dudy + j), dwdyj = gldd(dwdy + j);
-  cplx d[6][RZ];
+  H d[6][RZ];
 #pragma unroll
   for (int m = 0; m < RZ; ++m) {
     const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
@@ -78,17 +81,17 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
       cplx o[6];
       derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, o);
 #pragma unroll
-      for (int f = 0; f < 6; ++f) d[f][m] = o[f];
+      for (int f = 0; f < 6; ++f) d[f][m] = to_h<H>(o[f]);
     } else {
 #pragma unroll
-      for (int f = 0; f < 6; ++f) d[f][m] = mkc(0.0, 0.0);
+      for (int f = 0; f < 6; ++f) d[f][m] = wf_mk((R)0, (R)0);
     }
   }
   const long fstride = (long)g.my * g.kxn * Nz;
-  cplx* dst = hyb + row * Nz;
+  H* dst = hyb + row * Nz;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
-    cplx out[8];
+    H out[8];
     wfft_l2t<RZ, 1>(d[f], out, buf, rt);
     if (lane < W::NT) {
 #pragma unroll
@@ -98,20 +101,21 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
 }
 
 // ---------------------------------------------------------------------------
-template <int RX>
+template <int RX, class H>
 __global__ void __launch_bounds__(64 * WB_ROWS)
-k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
+k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
         const cplx* __restrict__ twx, int rk, double* __restrict__ upart,
         double* __restrict__ phist, const DevScalars* sc, int nhist) {
-  typedef WFFT<RX, 1> WI;
-  typedef WFFT<RX, -1> WF;
+  typedef WFFT<RX, 1, H> WI;
+  typedef typename wf_real<H>::T R;
   if (sc->halted) return;
-  extern __shared__ cplx lds[];
+  extern __shared__ double4 xzw_smem[];
+  H* lds = reinterpret_cast<H*>(xzw_smem);
   const int Mx = 64 * RX, Nz = g.mgalz, kxn = g.kxn;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  cplx* rt = lds;
-  cplx* work = lds + Mx;                 // staging, then per-wave FFT buffers
-  cplx* buf = work + wave * WI::LDS;
+  H* rt = lds;
+  H* work = lds + Mx;                    // staging, then per-wave FFT buffers
+  H* buf = work + wave * WI::LDS;
   stage_roots(rt, twx, Mx);
   const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
   const int zblk = (int)(lb % gridDim.x), j = (int)(lb / gridDim.x);
@@ -124,7 +128,7 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
   constexpr int KXM = 64 * RX / 3;                                  // kxn bound
   constexpr int NE = (6 * KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);
   const int tile = kxn * WB_ROWS;                                   // per field
-  cplx pre[NE];
+  H pre[NE];
 #pragma unroll
   for (int q = 0; q < NE; ++q) {
     const int e = threadIdx.x + q * 64 * WB_ROWS;
@@ -134,8 +138,8 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
       pre[q] = gld(hin + f * fstride + rowbase + (long)kx * Nz + rr);
     }
   }
-  cplx zin[3][RX];
-  cplx* S = work;                        // [2][kxn][WB_ROWS + 1]
+  H zin[3][RX];
+  H* S = work;                           // [2][kxn][WB_ROWS + 1]
   const int SP = WB_ROWS + 1;
 #pragma unroll
   for (int p = 0; p < 3; ++p) {
@@ -154,49 +158,51 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
 #pragma unroll
     for (int m = 0; m < RX; ++m) {
       const int b = lane + 64 * m;
-      cplx s = mkc(0.0, 0.0);
+      H s = wf_mk((R)0, (R)0);
       if (b == 0) {
-        const cplx Ga = S[r], Gb = S[kxn * SP + r];
-        s = mkc(Ga.x, Gb.x);     // c2r ignores the kx=0 imaginary part
+        const H Ga = S[r], Gb = S[kxn * SP + r];
+        s = wf_mk(Ga.x, Gb.x);   // c2r ignores the kx=0 imaginary part
       } else if (b < kxn) {
-        const cplx Ga = S[b * SP + r], Gb = S[(kxn + b) * SP + r];
-        s = mkc(Ga.x - Gb.y, Ga.y + Gb.x);
+        const H Ga = S[b * SP + r], Gb = S[(kxn + b) * SP + r];
+        s = wf_mk(Ga.x - Gb.y, Ga.y + Gb.x);
       } else if (b > Mx - kxn) {
         const int i = Mx - b;
-        const cplx Ga = S[i * SP + r], Gb = S[(kxn + i) * SP + r];
-        s = mkc(Ga.x + Gb.y, Gb.x - Ga.y);
+        const H Ga = S[i * SP + r], Gb = S[(kxn + i) * SP + r];
+        s = wf_mk(Ga.x + Gb.y, Gb.x - Ga.y);
       }
       zin[p][m] = s;
     }
   }
   __syncthreads();   // staging dead: the region becomes the FFT buffers
   // --- inverse x transforms: task t holds x = t + 8 RX q ---
-  cplx zx[3][8];
+  H zx[3][8];
 #pragma unroll
   for (int p = 0; p < 3; ++p) wfft_l2t<RX, 1>(zin[p], zx[p], buf, rt);
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
-  const double c2 = 2.0 * g.chi;
-  const double sy = g.s * ((double)j * g.dy - 0.5 * g.Ly);
-  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, p0 = 0.0, p1 = 0.0, p2 = 0.0;
-  cplx z12[8], z3[8];
+  const R c2 = (R)(2.0 * g.chi);
+  const R sy = (R)(g.s * ((double)j * g.dy - 0.5 * g.Ly));
+  // maxima in the element type, the 'p' sums in FP64
+  R m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+  H z12[8], z3[8];
   if (lane < WI::NT) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const double u = zx[0][q].x, v = zx[0][q].y, w = zx[1][q].x, o1 = zx[1][q].y;
-      const double o2 = zx[2][q].x, o3 = zx[2][q].y;
+      const R u = zx[0][q].x, v = zx[0][q].y, w = zx[1][q].x, o1 = zx[1][q].y;
+      const R o2 = zx[2][q].x, o3 = zx[2][q].y;
       m0 = fmax(m0, fabs(u));
       m1 = fmax(m1, fabs(u + sy));
       m2 = fmax(m2, fabs(v));
       m3 = fmax(m3, fabs(w));
-      p0 += v * (u * u);
-      p1 += v * (w * w);
-      p2 += v * v * v;
-      z12[q] = mkc(v * (o3 + c2) - w * o2, w * o1 - u * (o3 + c2));
-      z3[q] = mkc(u * o2 - v * o1, 0.0);
+      p0 += (double)(v * (u * u));
+      p1 += (double)(v * (w * w));
+      p2 += (double)(v * v * v);
+      z12[q] = wf_mk(v * (o3 + c2) - w * o2, w * o1 - u * (o3 + c2));
+      z3[q] = wf_mk(u * o2 - v * o1, (R)0);
     }
   }
   // --- forward x transforms -> lane layout bins ---
-  cplx a12[RX], a3[RX];
+  H a12[RX], a3[RX];
   wfft_t2l<RX, -1>(z12, a12, buf, rt);
   wfft_t2l<RX, -1>(z3, a3, buf, rt);
   // separate H1, H2 from Z12 = H1 + i H2: needs bin Mx - k from another lane
@@ -205,15 +211,15 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
   for (int m = 0; m < RX; ++m) buf[lane + 64 * m] = a12[m];
   wf_sync();
   __syncthreads();   // every wave done with its FFT buffer before the output tile
-  cplx* O = work;    // [3][kxn][WB_ROWS + 1]
-  cplx h1[RX], h2[RX];
+  H* O = work;       // [3][kxn][WB_ROWS + 1]
+  H h1[RX], h2[RX];
 #pragma unroll
   for (int m = 0; m < RX; ++m) {
     const int k = lane + 64 * m;
     if (k < kxn) {
-      const cplx A = buf[k], Am = buf[k == 0 ? 0 : Mx - k];
-      h1[m] = mkc(0.5 * (A.x + Am.x), 0.5 * (A.y - Am.y));
-      h2[m] = mkc(0.5 * (A.y + Am.y), -0.5 * (A.x - Am.x));
+      const H A = buf[k], Am = buf[k == 0 ? 0 : Mx - k];
+      h1[m] = wf_mk((R)0.5 * (A.x + Am.x), (R)0.5 * (A.y - Am.y));
+      h2[m] = wf_mk((R)0.5 * (A.y + Am.y), (R)-0.5 * (A.x - Am.x));
     }
   }
   __syncthreads();
@@ -243,10 +249,10 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
     }
     __shared__ double um[WB_ROWS][4];
     if (lane == 0) {
-      um[wave][0] = m0;
-      um[wave][1] = m1;
-      um[wave][2] = m2;
-      um[wave][3] = m3;
+      um[wave][0] = (double)m0;
+      um[wave][1] = (double)m1;
+      um[wave][2] = (double)m2;
+      um[wave][3] = (double)m3;
     }
     __syncthreads();
     if (threadIdx.x < 4) {   // one partial per block
@@ -272,30 +278,31 @@ k_xzw_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout,
 }
 
 // ---------------------------------------------------------------------------
-template <int RZ>
+template <int RZ, class H>
 __global__ void __launch_bounds__(64 * WC_WAVES)
-k_xzw_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
+k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
         cplx* __restrict__ dvordy, cplx* __restrict__ hg, cplx* __restrict__ hv,
         double* __restrict__ rf0u, double* __restrict__ rf0w, const cplx* __restrict__ twz,
         const DevScalars* sc) {
-  typedef WFFT<RZ, -1> W;
+  typedef WFFT<RZ, -1, H> W;
   if (sc->halted) return;
-  extern __shared__ cplx lds[];
-  cplx* rt = lds;
+  extern __shared__ double4 xzw_smem[];
+  H* lds = reinterpret_cast<H*>(xzw_smem);
+  H* rt = lds;
   const int Nz = 64 * RZ;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  cplx* buf = lds + Nz + wave * W::LDS;
+  H* buf = lds + Nz + wave * W::LDS;
   stage_roots(rt, twz, Nz);
   __syncthreads();
   const long row = (long)blockIdx.x * WC_WAVES + wave;
   if (row >= (long)g.my * g.kxn) return;
   const int j = (int)(row / g.kxn), i = (int)(row - (long)j * g.kxn);
   const long fstride = (long)g.my * g.kxn * Nz;
-  const cplx* src = hout + row * Nz;
-  cplx h[3][RZ];
+  const H* src = hout + row * Nz;
+  H h[3][RZ];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    cplx v[8];
+    H v[8];
     if (lane < W::NT) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) v[q] = gld(src + f * fstride + lane + 8 * RZ * q);
@@ -310,7 +317,8 @@ k_xzw_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
   for (int m = 0; m < RZ; ++m) {
     const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
     if (k >= 0) {
-      const cplx h1 = crmul(norm, h[0][m]), h2 = crmul(norm, h[1][m]), h3 = crmul(norm, h[2][m]);
+      const cplx h1 = crmul(norm, to_d(h[0][m])), h2 = crmul(norm, to_d(h[1][m])),
+                 h3 = crmul(norm, to_d(h[2][m]));
       const double kz = kz_of(k, g.nz, g.mz, g.gam);
       const long idx = base + k;
       const cplx v = gld(vel + idx);
@@ -327,14 +335,15 @@ k_xzw_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
   }
 }
 
-// LDS bytes of the three kernels for R = Nz/64 (A, C) and Rx = Mx/64 (B)
-inline size_t xzw_lds_A(int R) { return (size_t)(64 * R + WA_WAVES * (R * WF_EP > 8 * R * WF_FP ? R * WF_EP : 8 * R * WF_FP)) * sizeof(cplx); }
-inline size_t xzw_lds_C(int R) { return xzw_lds_A(R); }
-inline size_t xzw_lds_B(int Rx, int kxn) {
+// LDS bytes of the three kernels for R = Nz/64 (A, C) and Rx = Mx/64 (B),
+// elements of es bytes (16: FP64, 8: FP32)
+inline size_t xzw_lds_A(int R, size_t es) { return (size_t)(64 * R + WA_WAVES * (R * WF_EP > 8 * R * WF_FP ? R * WF_EP : 8 * R * WF_FP)) * es; }
+inline size_t xzw_lds_C(int R, size_t es) { return xzw_lds_A(R, es); }
+inline size_t xzw_lds_B(int Rx, int kxn, size_t es) {
   const size_t fft = (size_t)WB_ROWS * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
   const size_t stage = (size_t)2 * kxn * (WB_ROWS + 1);
   const size_t out = (size_t)3 * kxn * (WB_ROWS + 1);
   size_t w = fft > stage ? fft : stage;
   w = w > out ? w : out;
-  return (size_t)(64 * Rx + w) * sizeof(cplx);
+  return (size_t)(64 * Rx + w) * es;
 }
