@@ -30,6 +30,8 @@ def parse():
     ap.add_argument("--explicit", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed precision leg")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="precision of the main timed leg (mixed: no separate mixed leg)")
     ap.add_argument("--cmp-steps", type=int, default=500,
                     help="steps of the FP64-vs-mixed statistics comparison")
     ap.add_argument("--cpu-grid", type=str, default="",
@@ -39,18 +41,20 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_bytes(kxn, mz, my, mgalz):
-    """compulsory HBM bytes per launch of each kernel class, FP64 complex = 16 B.
+def algorithmic_bytes(kxn, mz, my, mgalz, precision="fp64"):
+    """compulsory HBM bytes per launch of each kernel class: FP64 complex
+    spectral values (16 B) and hybrid values of 16 B (FP64) or 8 B (mixed).
     Nc = kxn*mz*my spectral coefficients, Nh = kxn*mgalz*my hybrid values."""
     nc = kxn * mz * my
     nh = kxn * mgalz * my
+    hb = 16.0 if precision == "fp64" else 8.0
     return {
         # A: reads vor, phi, v, dv/dy, d(omega_y)/dy; writes 6 hybrid fields
-        "xz_A": 16.0 * (5 * nc + 6 * nh),
+        "xz_A": 16.0 * 5 * nc + hb * 6 * nh,
         # B: reads 6 hybrid fields, writes 3
-        "xz_B": 16.0 * (9 * nh),
+        "xz_B": hb * 9 * nh,
         # C: reads 3 hybrid fields and v, writes 3 spectral fields
-        "xz_C": 16.0 * (3 * nh + 4 * nc),
+        "xz_C": hb * 3 * nh + 16.0 * 4 * nc,
         # fused y stage, averaged over the 3 substeps: reads dvordy, hv, hg,
         # vor, phi (+ vorwk, phiwk at rk 2,3); writes vor, phi, vorwk,
         # phiwk (not at rk 3), hg, dvordy, hv
@@ -63,12 +67,12 @@ def step_bytes(nc):
     return 992.0 * nc
 
 
-def measured_traffic(cls, grid, explicit):
+def measured_traffic(cls, grid, explicit, precision="fp64"):
     """HBM bytes per launch of a kernel class, from the committed rocprofv3
     counter summary of the same workload (FETCH_SIZE and WRITE_SIZE per
     dispatch, scaled by the copy-kernel calibration); None if there is none."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "rk3_step_%dx%dx%d_fp64.traffic.json" % grid)
+                        "rk3_step_%dx%dx%d_%s.traffic.json" % (grid + (precision,)))
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -174,18 +178,18 @@ def main():
     # timed region: exactly K steps in one advance call, which synchronises the
     # device before it returns; per-kernel HIP events on the library's stream
     st2, wall, timing, (kxn, mz, nc) = timed_leg(snf, ic, grid, a.Re, a.explicit, a.steps,
-                                                 a.warmup, "fp64")
+                                                 a.warmup, a.precision)
     st = st2
     steps_done = st2.steps_done
     sps = steps_done / wall
-    ab = algorithmic_bytes(kxn, mz, my, mgalz)
+    ab = algorithmic_bytes(kxn, mz, my, mgalz, a.precision)
     per_cls = {}
     for k, (ms, cnt) in timing.items():
         if cnt:
             per_cls[k] = ms / cnt
     dom = max((k for k in per_cls if k in ab), key=lambda k: timing[k][0])
     achieved = ab[dom] / (per_cls[dom] * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(dom, (mgalx, my, mgalz), a.explicit)
+    traffic, traffic_src = measured_traffic(dom, (mgalx, my, mgalz), a.explicit, a.precision)
     out = {
         "metric": "RK3 steps/s at 384\u00b3 (FP64 vs mixed/batched, rel err vs FP64); % HBM peak",
         "value": sps,
@@ -197,11 +201,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.precision == "fp64" else "f32/f64 mixed",
         "data": "synthetic (seeded k^4 exp(-2(k/k0)^2) solenoidal field, k0=4)",
         "config": {"workload": f"shear-periodic box DNS {mgalx}x{my}x{mgalz}, Lx:Ly:Lz=2:1:1, "
                                f"S=1, Re_z={a.Re:g}, CFL 0.5 adaptive dt, "
-                               f"{'explicit' if a.explicit else 'implicit'} viscous, FP64",
+                               f"{'explicit' if a.explicit else 'implicit'} viscous, "
+                               f"{'FP64' if a.precision == 'fp64' else 'mixed precision'}",
                    "grid": [mgalx, my, mgalz], "Nc": nc},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -216,7 +221,7 @@ def main():
         "sim_time_advanced": st2.time,
         "Deltat": st2.Deltat,
     }
-    if not a.no_mixed:
+    if not a.no_mixed and a.precision == "fp64":
         stm, wallm, timm, _ = timed_leg(snf, ic, grid, a.Re, a.explicit, a.steps, a.warmup, "mixed")
         spsm = stm.steps_done / wallm
         out["mixed"] = {

@@ -60,7 +60,11 @@ typedef struct snf_params {
   double uprim;               /* IC velocity scale (isotropic first dt)   */
   double vbulk;               /* v of the (0,0) mode (cross.f90:1276)     */
   double etime, dumptime, dtimag;  /* stop / dump times (cross.f90:1680-1712) */
-  int32_t precision;          /* SNF_PREC_FP64 (0) or SNF_PREC_MIXED (1)  */
+  int32_t precision;          /* SNF_PREC_FP64 (0), or SNF_PREC_MIXED (1):
+                                 FP32 x-z transforms, hybrid planes and
+                                 u x omega; FP64 state, KMM reconstruction,
+                                 y operators, RK update, statistics sums;
+                                 needs mgalx, mgalz in {128,192,256,384} */
   int32_t device;             /* HIP device ordinal                       */
   int32_t ydealias;           /* iydealiasing (cross.f90:459-491), applied
                                  to vor, phi, U, W every substep: 0 none
