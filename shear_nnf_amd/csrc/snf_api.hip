@@ -102,7 +102,7 @@ struct snf_handle {
   bool mixed = false;     // SNF_PREC_MIXED: FP32 x-z transforms and hybrid planes
   int rx = 0, rz = 0;
   size_t ldsWA = 0, ldsWB = 0, ldsWC = 0;
-  double* upart = nullptr;   // per-row umax partials of the wave x-row kernel
+  double* upart = nullptr;   // umax accumulators of the wave x-row kernel ([UMAX_SLOTS][4])
   // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
   bool yfft = false;
   bool ysym_on = false;   // multiplier tables maintained (spectral y or filter)
@@ -330,8 +330,8 @@ ScalarArgs sargs(snf_handle* h) {
   a.readflag = h->p.readflag;
   a.explicit_visc = h->p.explicit_visc;
   a.zb_rows = h->wave_xz ? 1 : h->zb;
-  a.upart = h->wave_xz ? h->upart : nullptr;
-  a.nupart = h->wave_xz ? (long)h->g.my * (h->g.mgalz / WB_ROWS) : 0;
+  a.upart = h->wave_xz ? reinterpret_cast<unsigned long long*>(h->upart) : nullptr;
+  a.nupart = h->wave_xz ? UMAX_SLOTS : 0;
   a.cflx = h->cflx;
   a.cfly = h->cfly;
   a.cflz = h->cflz;
@@ -544,7 +544,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->umax_acc, 4));
   TRY(A(&h->stat_plane, (size_t)my * STAT_W));
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
-  TRY(A(&h->upart, (size_t)my * mgalz * 4));
+  TRY(A(&h->upart, (size_t)UMAX_SLOTS * 4));
   TRY(A(&h->twy, (size_t)my));
   for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * YS));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));

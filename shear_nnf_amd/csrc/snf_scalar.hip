@@ -125,7 +125,7 @@ struct ScalarArgs {
   int nhist, readflag, explicit_visc, zb_rows;
   double cflx, cfly, cflz, cflvv, CFL, uprim;
   unsigned long long* umax_acc;
-  const double* upart;        // [nupart][4] partial maxima (or null)
+  unsigned long long* upart;  // [nupart][4] maxima as double bit patterns (or null)
   long nupart;
   const double* stat_plane;   // [my][STAT_W]
   const double* phist;        // [my][Nz/2][3]
@@ -169,8 +169,12 @@ k_scalar(ScalarArgs a, int rk) {
   __shared__ double red_um[4][256];
   if (rk == 1) {
     double lm[4] = {0.0, 0.0, 0.0, 0.0};
+    // fold the accumulators and clear them for the next rk = 1 stage
     for (long n = tid; n < a.nupart; n += blockDim.x)
-      for (int q = 0; q < 4; ++q) lm[q] = fmax(lm[q], a.upart[n * 4 + q]);
+      for (int q = 0; q < 4; ++q) {
+        lm[q] = fmax(lm[q], __longlong_as_double((long long)a.upart[n * 4 + q]));
+        a.upart[n * 4 + q] = 0ull;
+      }
     for (int q = 0; q < 4; ++q) red_um[q][tid] = lm[q];
   }
   __syncthreads();

@@ -23,6 +23,7 @@
 #define WB_ROWS 4
 #endif
 #define WC_WAVES 4
+#define UMAX_SLOTS 64   // umax accumulators of the x-row kernel, folded by k_scalar
 
 // padded z position (0..Nz-1) of lane l, value m -> kz index, or -1 (gap)
 __device__ __forceinline__ int wz_k(int pos, int nz, int mz, int Nz) {
@@ -259,7 +260,9 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
       double m = 0.0;
 #pragma unroll
       for (int w = 0; w < WB_ROWS; ++w) m = fmax(m, um[w][threadIdx.x]);
-      gstd(upart + ((long)j * gridDim.x + zblk) * 4 + threadIdx.x, m);
+      // maxima of non-negative doubles order like their bit patterns
+      atomicMax(reinterpret_cast<unsigned long long*>(upart) + (lb % UMAX_SLOTS) * 4 + threadIdx.x,
+                (unsigned long long)__double_as_longlong(m));
     }
   }
   if (ihist) {
