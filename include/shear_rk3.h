@@ -134,6 +134,14 @@ int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n);
  * column-major like the reference: out[s + 23*j]                          */
 int snf_statistics(snf_handle* h, double* stats, double* stats2,
                    int32_t* nacum, double* total_dt);
+/* spectra of the statistics sp, sp2 (cross.f90:1970-2016): out[q + 8*(i +
+ * kxn*k)] = sp(q+1, i, k), the sums over samples and planes of |u|^2,
+ * |v|^2, |w|^2, Re(u v*), |o1|^2, |o2|^2, |o3|^2, Re(o1 o2*) with kx > 0
+ * counted twice (sp2: each sample times its step)                        */
+int snf_spectra(snf_handle* h, double* sp, double* sp2);
+/* zero stats, stats2, sp, sp2, nacum and total_dt, as escru does after
+ * writing them (readwrite.f90:1322-1334)                                 */
+int snf_reset_statistics(snf_handle* h);
 
 /* ---- operator entry points for parity tests (host arrays, [y][line]) --
  * derivyc (diffy_shear_hepta.f90:333-450): order 1 or 2 on nl lines of

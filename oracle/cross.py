@@ -24,6 +24,7 @@ ZETA = (-17.0 / 60.0, -5.0 / 12.0, 0.0)
 CA = (8.0 / 15.0, 2.0 / 3.0, 1.0)
 TINY = 1e-13
 NSTAT = 23
+NSPEC = 8
 
 
 def _div(a, b):
@@ -88,6 +89,8 @@ class Cross:
         self.history = []       # 24-double UNER records (the .cfbin rows)
         self.stats = np.zeros((NSTAT, g.my))
         self.stats2 = np.zeros((NSTAT, g.my))
+        self.spec = np.zeros((NSPEC, g.kxn, g.mz))     # sp(nspec, 0:mx1, 0:mz1)
+        self.spec2 = np.zeros((NSPEC, g.kxn, g.mz))
         self.nacum = 0
         self.total_dt = 0.0
         self.istep = 0
@@ -129,6 +132,15 @@ class Cross:
         r = stencil(self.ops.pr2, fl, t[None, :], np.repeat(shm, g.mz)[None, :])
         out = solve(fac, r).reshape(g.my, g.kxn, g.mz)
         return out
+
+    def reset_statistics(self):
+        """readwrite.f90:1322-1334: zero the statistics and spectra after a dump"""
+        self.stats[:] = 0.0
+        self.stats2[:] = 0.0
+        self.spec[:] = 0.0
+        self.spec2[:] = 0.0
+        self.nacum = 0
+        self.total_dt = 0.0
 
     def getv(self, f, iopt):
         """getv (operators.f90:735-793) with the twist of the current offsets:
@@ -266,6 +278,14 @@ class Cross:
             st2[idx] += f[:, 0, 0].real * dt
         ener[0:9] += [up.sum(), vp.sum(), wp.sum(), uvr.sum(), uwr.sum(), vwr.sum(),
                       o1p.sum(), o2p.sum(), o3p.sum()]
+        # spectra of the same products, summed over the planes
+        # (cross.f90:1970-2016; o3 without the shear correction)
+        c2 = lambda a, b: (a * np.conj(b)).real
+        spec = np.stack([c2(u1, u1), c2(u2, u2), c2(u3, u3), c2(u1, u2),
+                         c2(o1, o1), c2(o2, o2), c2(o3, o3), c2(o1, o2)])
+        spec = np.sum(spec * wt[None], axis=1)
+        self.spec += spec
+        self.spec2 += spec * dt
         o3s = o3.copy()
         o3s[:, 0, 0] -= g.s
         o300 = o3s[:, 0, 0].real

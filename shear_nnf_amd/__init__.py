@@ -32,6 +32,7 @@ EXPORTS = (
     "snf_state_download_ykxkz", "snf_advance", "snf_history", "snf_statistics",
     "snf_op_derivy", "snf_op_helmholtz", "snf_op_fourxz", "snf_op_hvhg",
     "snf_set_timing", "snf_get_timing", "snf_copy_probe", "snf_op_getv", "snf_energy",
+    "snf_spectra", "snf_reset_statistics",
 )
 KERNEL_CLASSES = ("xz_A", "xz_B", "xz_C", "y_lines", "small")
 
@@ -96,6 +97,8 @@ def load_library(path=LIB_PATH):
     lib.snf_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, dp]
     lib.snf_op_getv.argtypes = [H, dp, dp, ctypes.c_int32]
     lib.snf_energy.argtypes = [H, dp]
+    lib.snf_spectra.argtypes = [H, dp, dp]
+    lib.snf_reset_statistics.argtypes = [H]
     _lib = lib
     return lib
 
@@ -235,6 +238,18 @@ class ShearBox:
         self._check(self.lib.snf_statistics(self.h, _ptr(st), _ptr(st2), ctypes.byref(nacum),
                                             ctypes.byref(tdt)))
         return st, st2, nacum.value, tdt.value
+
+    def spectra(self):
+        """(sp, sp2), each (8, kxn, mz): the accumulated spectra of the
+        statistics (cross.f90:1970-2016)"""
+        sp = np.empty((self.mz, self.kxn, 8))
+        sp2 = np.empty((self.mz, self.kxn, 8))
+        self._check(self.lib.snf_spectra(self.h, _ptr(sp), _ptr(sp2)))
+        return sp.transpose(2, 1, 0).copy(), sp2.transpose(2, 1, 0).copy()
+
+    def reset_statistics(self):
+        """zero the statistics, spectra and their counters (readwrite.f90:1322-1334)"""
+        self._check(self.lib.snf_reset_statistics(self.h))
 
     def set_timing(self, on):
         self._check(self.lib.snf_set_timing(self.h, 1 if on else 0))

@@ -102,6 +102,7 @@ struct snf_handle {
   bool mixed = false;     // SNF_PREC_MIXED: FP32 x-z transforms and hybrid planes
   int rx = 0, rz = 0;
   size_t ldsWA = 0, ldsWB = 0, ldsWC = 0;
+  double *spec = nullptr, *spec2 = nullptr;   // spectra sp, sp2: [SNF_NSPEC][P]
   double* upart = nullptr;   // umax accumulators of the wave x-row kernel ([UMAX_SLOTS][4])
   // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
   bool yfft = false;
@@ -297,10 +298,14 @@ int launch_xz(snf_handle* h, int rk) {
                                             h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->pz,
                                             h->twz, h->sc);
   }
-  if (rk == 1)
+  if (rk == 1) {
     k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                           h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->stat_plane,
                                           h->sc, h->p.nhist);
+    k_hist_spec<<<(unsigned)((g.P + 255) / 256), 256, 0, h->stream>>>(
+        g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk,
+        h->spec, h->spec2, h->sc, h->p.nhist);
+  }
   if (h->wave_xz) {
     { TimedLaunch tl(h, SNF_K_XZ_B);
       launch_xzw_any(h, 1, rk); }
@@ -549,6 +554,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
   for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * YS));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
+  TRY(A(&h->spec, (size_t)SNF_NSPEC * h->P));
+  TRY(A(&h->spec2, (size_t)SNF_NSPEC * h->P));
   TRY(A(&h->ring, (size_t)h->cap * 24));
   // base factors -> device
   {
@@ -682,7 +689,7 @@ int snf_destroy(snf_handle* h) {
                   h->u00r, h->w00r, h->rf0u, h->rf0w, h->dudy, h->dwdy, h->base, h->tw[0],
                   h->tw[1], h->shp[0], h->shp[1], h->shm[0], h->shm[1], h->twx, h->twz, h->sc,
                   h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring,
-                  h->upart, h->twy, h->ysym[0], h->ysym[1]};
+                  h->upart, h->twy, h->ysym[0], h->ysym[1], h->spec, h->spec2};
   for (void* q : ptrs)
     if (q) hipFree(q);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
@@ -904,6 +911,44 @@ int snf_statistics(snf_handle* h, double* stats, double* stats2, int32_t* nacum,
   HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
   if (nacum) *nacum = s.nacum;
   if (total_dt) *total_dt = s.total_dt;
+  return SNF_OK;
+}
+
+int snf_spectra(snf_handle* h, double* sp, double* sp2) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const long P = h->g.P;
+  std::vector<double> buf((size_t)SNF_NSPEC * P);
+  double* outs[2] = {sp, sp2};
+  const double* srcs[2] = {h->spec, h->spec2};
+  for (int f = 0; f < 2; ++f) {
+    if (!outs[f]) continue;
+    HIPCHK(hipMemcpy(buf.data(), srcs[f], buf.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // sp(nspec, 0:mx1, 0:mz1), column-major
+    for (long L = 0; L < P; ++L) {
+      const long i = L / h->g.mz, k = L % h->g.mz;
+      for (int q = 0; q < SNF_NSPEC; ++q)
+        outs[f][q + SNF_NSPEC * (i + (long)h->g.kxn * k)] = buf[(size_t)q * P + L];
+    }
+  }
+  return SNF_OK;
+}
+
+int snf_reset_statistics(snf_handle* h) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t ns = (size_t)h->g.my * SNF_NSTAT, nsp = (size_t)SNF_NSPEC * h->P;
+  HIPCHK(hipMemset(h->stats, 0, ns * sizeof(double)));
+  HIPCHK(hipMemset(h->stats2, 0, ns * sizeof(double)));
+  HIPCHK(hipMemset(h->spec, 0, nsp * sizeof(double)));
+  HIPCHK(hipMemset(h->spec2, 0, nsp * sizeof(double)));
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  s.nacum = 0;
+  s.total_dt = 0.0;
+  HIPCHK(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice));
   return SNF_OK;
 }
 

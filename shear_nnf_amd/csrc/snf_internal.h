@@ -9,6 +9,7 @@ typedef float2 cplxf;   // FP32 complex: x-z stage of the mixed precision mode
 #define SNF_MAXN 1032      // largest my supported (line length)
 #define SNF_MAXFFT 1024    // largest mgalx / mgalz
 #define SNF_NSTAT 23
+#define SNF_NSPEC 8        // spectra of the statistics (nspec)
 
 __host__ __device__ __forceinline__ cplx mkc(double r, double i) { return make_double2(r, i); }
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return mkc(a.x + b.x, a.y + b.y); }

@@ -339,6 +339,48 @@ k_xz_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
 //  0 up 1 vp 2 wp 3 uvr 4 uwr 5 vwr 6 o1p 7 o2p 8 o3p 9 o12p 10 eps
 //  11 um 12 vm 13 wm 14 o1m 15 o2m 16 o3m
 // ---------------------------------------------------------------------------
+// Spectra of the 'f' statistics (cross.f90:1970-2016): for every (kx, kz)
+// line, sums over the planes of |u|^2, |v|^2, |w|^2, Re(u v*), |o1|^2,
+// |o2|^2, |o3|^2, Re(o1 o2*) (kx > 0 counted twice), added to sp and, times
+// the step, to sp2 ([8][P]).  One thread per line walks the planes in order,
+// so the sums do not depend on the launch.
+__global__ void __launch_bounds__(256)
+k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+            const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+            const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+            const double* __restrict__ w00, const double* __restrict__ dudy,
+            const double* __restrict__ dwdy, double vbulk, double* __restrict__ sp,
+            double* __restrict__ sp2, const DevScalars* sc, int nhist) {
+  if (sc->halted) return;
+  if ((sc->steps_local - 1) % nhist != 0) return;
+  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= g.P) return;
+  const int i = (int)(L / g.mz), k = (int)(L - (long)i * g.mz);
+  const double wt = (i == 0) ? 1.0 : 2.0;
+  double acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+  for (int j = 0; j < g.my; ++j) {
+    PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
+    cplx d[6];
+    derived6(g, q, i, k, u00[j], w00[j], dudy[j], dwdy[j], vbulk, d);
+    acc[0] += wt * cabs2(d[0]);
+    acc[1] += wt * cabs2(d[1]);
+    acc[2] += wt * cabs2(d[2]);
+    acc[3] += wt * cmulc(d[0], d[1]).x;
+    acc[4] += wt * cabs2(d[3]);
+    acc[5] += wt * cabs2(d[4]);
+    acc[6] += wt * cabs2(d[5]);
+    acc[7] += wt * cmulc(d[3], d[4]).x;
+  }
+  const double dt = sc->Deltat;   // the step of the sample, as in the reference's 'f' pass
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    sp[q * g.P + L] += acc[q];
+    sp2[q * g.P + L] += acc[q] * dt;
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_hist_f(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
          const cplx* __restrict__ hg, const cplx* __restrict__ hv,

@@ -243,3 +243,30 @@ def test_jik_roundtrip(lib):
     a, b, c2, d, t, xb, xt = box.get_state_jik()
     assert np.array_equal(a, jv) and np.array_equal(b, jp)
     assert (t, xb, xt) == (1.5, -0.25, 0.25)
+
+
+def test_spectra_and_reset(lib):
+    """the spectra of the statistics (cross.f90:1970-2016) and the reset after
+    a dump (readwrite.f90:1322-1334) against the oracle"""
+    grid = (48, 49, 48)
+    vor, phi, u00, w00 = box_ic(*grid)
+    box = _box(*grid, explicit=1, nhist=1)
+    box.set_state(vor, phi, u00, w00)
+    c = _oracle(*grid, explicit=1, nhist=1)
+    c.set_state(vor, phi, u00, w00)
+    for n in (4, 2):
+        box.cross(n)
+        c.run(n)
+        sp, sp2 = box.spectra()
+        st, st2, nacum, tdt = box.statistics()
+        assert nacum == c.nacum == n
+        assert rel_l2(sp, c.spec) < 1e-11 and rel_l2(sp2, c.spec2) < 1e-11
+        assert rel_l2(st.T, c.stats) < 1e-11 and rel_l2(st2.T, c.stats2) < 1e-11
+        assert tdt == pytest.approx(c.total_dt, rel=1e-12)
+        # Parseval: the spectrum of u'^2 summed over (kx, kz) is the plane sum
+        assert np.sum(sp[0]) == pytest.approx(np.sum(st[:, 3]), rel=1e-12)
+        box.reset_statistics()
+        c.reset_statistics()
+        sp, _ = box.spectra()
+        st, _, nacum, tdt = box.statistics()
+        assert nacum == 0 and tdt == 0.0 and not np.any(sp) and not np.any(st)

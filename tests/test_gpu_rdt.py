@@ -28,9 +28,10 @@ def gpu_kelvin_run(CFL, explicit, grid=(16, 33, 16), Re=300.0):
     assert st.stop == STOP and st.time == pytest.approx(T_END, abs=1e-14)
     vor, phi, u00, w00 = box.get_state()
     stats = box.statistics()[0].T
+    spec = box.spectra()[0]
     history = list(box.history)
     box.close()
-    return mode.error(vor, phi, st.time), (vor, phi, u00, w00), (mode, history, stats)
+    return mode.error(vor, phi, st.time), (vor, phi, u00, w00), (mode, history, stats, spec)
 
 
 @pytest.mark.parametrize("explicit,order", [(1, 3), (0, 2)])

@@ -100,16 +100,25 @@ def kelvin_expected_record(mode, t, s=1.0):
     return uner, uner2
 
 
-def check_kelvin_history(mode, history, stats):
+def check_kelvin_history(mode, history, stats, spec=None):
     """UNER records (time, Deltat, uner(10), CFL, CFLv, umax(1:3), xoffb,
-    xofft, umax(0), uner2(4)) and per-plane statistics stats(23, my) of a
-    Kelvin-mode run against the closed form at each record's time."""
+    xofft, umax(0), uner2(4)), per-plane statistics stats(23, my) and the
+    spectra sp(8, kxn, mz) of a Kelvin-mode run against the closed form at
+    each record's time."""
     g = mode.g
     assert len(history) >= 80
     acc = np.zeros(12)
+    sp = np.zeros(8)
     for rec in history:
         t = rec[0]
         uner, uner2 = kelvin_expected_record(mode, t)
+        (u1, u2, u3), k = rdt_velocity(mode.u0, mode.v0, mode.w0, mode.k0x, mode.k0y, mode.k0z,
+                                       1.0 / g.Re, t)
+        u = np.array([u1, u2, u3])
+        om = 1j * np.cross(np.array(k), u)
+        c = lambda a, b: 2.0 * (a * np.conj(b)).real * g.my     # kx > 0: twice, all planes
+        sp += [c(u[0], u[0]), c(u[1], u[1]), c(u[2], u[2]), c(u[0], u[1]),
+               c(om[0], om[0]), c(om[1], om[1]), c(om[2], om[2]), c(om[0], om[1])]
         got = rec[2:12]
         scale = np.array([1, 1, 1, 0, 0, 0, 1, 1, 1, 1]) * np.abs(uner) + \
             np.array([0, 0, 0, 1, 1, 1, 0, 0, 0, 0]) * uner[0] ** 2
@@ -134,6 +143,16 @@ def check_kelvin_history(mode, history, stats):
     cube = len(history) * g.mgalx * g.mgalz * (np.sqrt(2.0) * abs(mode.v0)) ** 3
     for r in (16, 17, 18):
         assert np.max(np.abs(stats[r])) < 1e-6 * cube
+    if spec is not None:
+        got = spec[:, mode.i, mode.k]
+        scale = np.array([1, 1, 1, 0, 1, 1, 1, 0]) * np.abs(sp) + \
+            np.array([0, 0, 0, 1, 0, 0, 0, 1]) * max(sp[0], sp[1])
+        scale[3] = max(sp[0], sp[1])
+        scale[7] = max(sp[4], sp[5])
+        assert np.all(np.abs(got - sp) <= 1e-7 * scale), (got, sp)
+        rest = spec.copy()
+        rest[:, mode.i, mode.k] = 0.0
+        assert np.max(np.abs(rest[[0, 1, 2, 4, 5, 6]])) < 1e-6 * np.max(sp)
 
 
 def test_kelvin_mode_history_records():
@@ -144,7 +163,9 @@ def test_kelvin_mode_history_records():
     c.etime = T_END
     c.run(100000)
     assert len(c.history) == c.nacum
-    check_kelvin_history(mode, c.history, c.stats)
+    check_kelvin_history(mode, c.history, c.stats, c.spec)
+    c.reset_statistics()
+    assert c.nacum == 0 and not np.any(c.stats) and not np.any(c.spec)
 
 
 def test_liftup_closed_form_solves_the_kelvin_equations():
