@@ -38,6 +38,20 @@ int fail(int code, const std::string& msg) {
       return fail(SNF_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device scratch of one operator call, freed on every return path.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipMalloc((void**)&p, n * sizeof(T)); }
+  operator T*() const { return p; }
+};
+
 bool plan_of(int n, FFTPlan& p) {
   p.n = n;
   p.nr = 0;
@@ -682,7 +696,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
 
 int snf_destroy(snf_handle* h) {
   if (!h) return SNF_OK;
-  hipSetDevice(h->dev);
+  (void)hipSetDevice(h->dev);   // frees proceed whatever the device state
   void* ptrs[] = {h->vor, h->phi, h->v, h->dome, h->dv, h->rdome, h->rhg, h->rhv, h->s3,
                   h->vorwk, h->phiwk, h->s1, h->s2,
                   h->hyb_in, h->hyb_out, h->hr, h->hc, h->u00c, h->w00c, h->u00wk, h->w00wk,
@@ -691,7 +705,7 @@ int snf_destroy(snf_handle* h) {
                   h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring,
                   h->upart, h->twy, h->ysym[0], h->ysym[1], h->spec, h->spec2};
   for (void* q : ptrs)
-    if (q) hipFree(q);
+    if (q) (void)hipFree(q);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -709,7 +723,7 @@ int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc) {
 static int upload_common(snf_handle* h, const std::vector<cplx>& hv, const std::vector<cplx>& hp,
                          const double* u00, const double* w00, double time, double xoffb,
                          double xofft) {
-  hipSetDevice(h->dev);
+  HIPCHK(hipSetDevice(h->dev));
   const int my = h->g.my;
   HIPCHK(hipMemcpy(h->vor, hv.data(), h->NC * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->phi, hp.data(), h->NC * sizeof(cplx), hipMemcpyHostToDevice));
@@ -761,7 +775,7 @@ int snf_state_upload(snf_handle* h, const double* vor, const double* phi, const 
 
 static int download_common(snf_handle* h, std::vector<cplx>& a, std::vector<cplx>& b, double* u00,
                            double* w00) {
-  hipSetDevice(h->dev);
+  HIPCHK(hipSetDevice(h->dev));
   HIPCHK(hipStreamSynchronize(h->stream));
   const int my = h->g.my;
   a.resize(h->NC);
@@ -960,25 +974,24 @@ int snf_op_derivy(snf_handle* h, const double* u, double* du, int32_t nl, const 
   HIPCHK(hipSetDevice(h->dev));
   const int N = h->g.my;
   const size_t n = (size_t)N * nl;
-  cplx *du_, *dd, *ys, *sp, *sm;
-  TwistTail* tw;
-  HIPCHK(hipMalloc(&du_, n * sizeof(cplx)));
-  HIPCHK(hipMalloc(&dd, n * sizeof(cplx)));
-  HIPCHK(hipMalloc(&ys, n * sizeof(cplx)));
-  HIPCHK(hipMalloc(&sp, nl * sizeof(cplx)));
-  HIPCHK(hipMalloc(&sm, nl * sizeof(cplx)));
-  HIPCHK(hipMalloc(&tw, 2 * (size_t)nl * sizeof(TwistTail)));
+  DevBuf<cplx> du_, dd, ys, sp, sm;
+  DevBuf<TwistTail> tw;
+  HIPCHK(du_.alloc(n));
+  HIPCHK(dd.alloc(n));
+  HIPCHK(ys.alloc(n));
+  HIPCHK(sp.alloc(nl));
+  HIPCHK(sm.alloc(nl));
+  HIPCHK(tw.alloc(2 * (size_t)nl));
   HIPCHK(hipMemcpy(du_, u, n * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(sp, shp, nl * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(sm, shm, nl * sizeof(cplx), hipMemcpyHostToDevice));
   k_twist<<<dim3(nl, 2), 64, 0, h->stream>>>(N, h->b1, h->b2, h->c, sp, tw, nl, nullptr);
   k_op_deriv<<<(nl + 127) / 128, 128, 0, h->stream>>>(N, nl, h->c, order == 1 ? h->b1 : h->b2,
-                                                     tw + (order == 1 ? 0 : nl), sp, sm, du_, dd,
+                                                     tw.p + (order == 1 ? 0 : nl), sp, sm, du_, dd,
                                                      ys, order);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(du, dd, n * sizeof(cplx), hipMemcpyDeviceToHost));
-  hipFree(du_); hipFree(dd); hipFree(ys); hipFree(sp); hipFree(sm); hipFree(tw);
   return SNF_OK;
 }
 
@@ -988,16 +1001,16 @@ int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl, cons
   HIPCHK(hipSetDevice(h->dev));
   const int N = h->g.my;
   const size_t n = (size_t)N * nl;
-  cplx *ud, *pd, *ys, *sp, *sm, *hc;
-  double *k2d, *hr;
-  HIPCHK(hipMalloc(&ud, n * sizeof(cplx)));
-  HIPCHK(hipMalloc(&pd, n * sizeof(cplx)));
-  HIPCHK(hipMalloc(&ys, n * sizeof(cplx)));
-  HIPCHK(hipMalloc(&sp, nl * sizeof(cplx)));
-  HIPCHK(hipMalloc(&sm, nl * sizeof(cplx)));
-  HIPCHK(hipMalloc(&k2d, nl * sizeof(double)));
-  HIPCHK(hipMalloc(&hr, 4 * n * sizeof(double)));
-  HIPCHK(hipMalloc(&hc, 3 * n * sizeof(cplx)));
+  DevBuf<cplx> ud, pd, ys, sp, sm, hc;
+  DevBuf<double> k2d, hr;
+  HIPCHK(ud.alloc(n));
+  HIPCHK(pd.alloc(n));
+  HIPCHK(ys.alloc(n));
+  HIPCHK(sp.alloc(nl));
+  HIPCHK(sm.alloc(nl));
+  HIPCHK(k2d.alloc(nl));
+  HIPCHK(hr.alloc(4 * n));
+  HIPCHK(hc.alloc(3 * n));
   HIPCHK(hipMemcpy(ud, u, n * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(sp, shp, nl * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(sm, shm, nl * sizeof(cplx), hipMemcpyHostToDevice));
@@ -1007,7 +1020,6 @@ int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl, cons
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(p, pd, n * sizeof(cplx), hipMemcpyDeviceToHost));
-  hipFree(ud); hipFree(pd); hipFree(ys); hipFree(sp); hipFree(sm); hipFree(k2d); hipFree(hr); hipFree(hc);
   return SNF_OK;
 }
 
@@ -1015,16 +1027,14 @@ int snf_energy(snf_handle* h, double* out) {
   if (!h || !out) return fail(SNF_BAD_PARAM, "null argument");
   HIPCHK(hipSetDevice(h->dev));
   const int nb = 1024;
-  double *part, *res;
-  HIPCHK(hipMalloc(&part, 2 * (size_t)nb * sizeof(double)));
-  HIPCHK(hipMalloc(&res, 4 * sizeof(double)));
+  DevBuf<double> part, res;
+  HIPCHK(part.alloc(2 * (size_t)nb));
+  HIPCHK(res.alloc(4));
   k_energy_part<<<nb, 256, 0, h->stream>>>(h->g, h->vor, h->phi, part);
   k_energy_final<<<1, 256, 0, h->stream>>>(part, nb, h->u00c, h->w00c, h->g.my, res);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost));
-  hipFree(part);
-  hipFree(res);
   return SNF_OK;
 }
 
@@ -1063,19 +1073,16 @@ int snf_op_getv(snf_handle* h, const double* in, double* out, int32_t dir) {
                        (const double*)sm.data(), 2);
     if (rc == SNF_OK) {
       const size_t n = (size_t)N * P;
-      cplx *vd, *dd, *pd;
-      HIPCHK(hipMalloc(&vd, n * sizeof(cplx)));
-      HIPCHK(hipMalloc(&dd, n * sizeof(cplx)));
-      HIPCHK(hipMalloc(&pd, n * sizeof(cplx)));
+      DevBuf<cplx> vd, dd, pd;
+      HIPCHK(vd.alloc(n));
+      HIPCHK(dd.alloc(n));
+      HIPCHK(pd.alloc(n));
       HIPCHK(hipMemcpy(vd, in, n * sizeof(cplx), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(dd, d2.data(), n * sizeof(cplx), hipMemcpyHostToDevice));
       k_lap_combine<<<1024, 256, 0, h->stream>>>(g, vd, dd, pd);
       HIPCHK(hipGetLastError());
       HIPCHK(hipStreamSynchronize(h->stream));
       HIPCHK(hipMemcpy(out, pd, n * sizeof(cplx), hipMemcpyDeviceToHost));
-      hipFree(vd);
-      hipFree(dd);
-      hipFree(pd);
     }
   }
   if (rc != SNF_OK) return rc;
@@ -1092,11 +1099,11 @@ int snf_op_fourxz(snf_handle* h, const double* in, double* out, int32_t nplanes,
   const Geom& g = h->g;
   const size_t ns = (size_t)nplanes * g.kxn * g.mz, nh = (size_t)nplanes * g.kxn * g.mgalz,
                np = (size_t)nplanes * g.mgalx * g.mgalz;
-  cplx *spec, *hyb;
-  double* phys;
-  HIPCHK(hipMalloc(&spec, ns * sizeof(cplx)));
-  HIPCHK(hipMalloc(&hyb, nh * sizeof(cplx)));
-  HIPCHK(hipMalloc(&phys, np * sizeof(double)));
+  DevBuf<cplx> spec, hyb;
+  DevBuf<double> phys;
+  HIPCHK(spec.alloc(ns));
+  HIPCHK(hyb.alloc(nh));
+  HIPCHK(phys.alloc(np));
   const size_t lz = 2 * (size_t)g.mgalz * sizeof(cplx), lx = 2 * (size_t)g.mgalx * sizeof(cplx);
   if (dir == 1) {
     HIPCHK(hipMemcpy(spec, in, ns * sizeof(cplx), hipMemcpyHostToDevice));
@@ -1113,7 +1120,6 @@ int snf_op_fourxz(snf_handle* h, const double* in, double* out, int32_t nplanes,
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemcpy(out, spec, ns * sizeof(cplx), hipMemcpyDeviceToHost));
   }
-  hipFree(spec); hipFree(hyb); hipFree(phys);
   return SNF_OK;
 }
 
@@ -1163,9 +1169,9 @@ int snf_copy_probe(int32_t device, int64_t nbytes, int32_t iters, double* ms) {
   if (nbytes < 4096 || iters < 1 || !ms) return fail(SNF_BAD_PARAM, "bad argument");
   HIPCHK(hipSetDevice(device));
   const long n = (long)(nbytes / (long)sizeof(double2));
-  double2 *a, *b;
-  HIPCHK(hipMalloc(&a, n * sizeof(double2)));
-  HIPCHK(hipMalloc(&b, n * sizeof(double2)));
+  DevBuf<double2> a, b;
+  HIPCHK(a.alloc(n));
+  HIPCHK(b.alloc(n));
   HIPCHK(hipMemset(a, 0, n * sizeof(double2)));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
@@ -1181,8 +1187,6 @@ int snf_copy_probe(int32_t device, int64_t nbytes, int32_t iters, double* ms) {
   *ms = t / iters;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
   return SNF_OK;
 }
 
