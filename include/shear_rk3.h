@@ -127,6 +127,23 @@ int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi,
  * continues after writing the dump.  snf_state_upload* starts a new run.
  * UNER records are kept on the device; fetch them with snf_history.      */
 int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st);
+/* ---- shooting (the Newton/Arnoldi glue; the Krylov packing stays with
+ * the caller) -------------------------------------------------------------
+ * A state vector is vor, phi in the layout [y][kx][kz] (interleaved complex,
+ * 2*my*kxn*mz doubles each) followed by u00, w00 (my doubles each).      */
+int snf_state_len(snf_handle* h, int64_t* n);
+/* the shooting map of calc_dns_disk3d (gmres_shear.f90:2951-3120): from x
+ * at time t0 with offsets xoffb0, xofft0, restart the statistics and run one
+ * cross call to t0 + T (the etime stop); y receives the final state.    */
+int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double xofft0,
+              double T, double* y, snf_run_status* st);
+/* forward-difference Jacobian-vector products of the shooting map
+ * (matvec_dns_disk3d with iopt_matvec=1, gmres_shear.f90:2666-2730):
+ * out[d] = (Phi_T(x0 + eps[d] dirs[d]) - y0) / eps[d] for ndir directions
+ * (state vectors, contiguous); y0 = Phi_T(x0), computed here if null.   */
+int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
+                  int32_t ndir, const double* eps, double t0, double xoffb0, double xofft0,
+                  double T, double* out);
 /* copies up to max_records records (SNF_UNER_LEN doubles each) produced
  * since the last call; *n receives the count                             */
 int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n);

@@ -32,7 +32,7 @@ EXPORTS = (
     "snf_state_download_ykxkz", "snf_advance", "snf_history", "snf_statistics",
     "snf_op_derivy", "snf_op_helmholtz", "snf_op_fourxz", "snf_op_hvhg",
     "snf_set_timing", "snf_get_timing", "snf_copy_probe", "snf_op_getv", "snf_energy",
-    "snf_spectra", "snf_reset_statistics",
+    "snf_spectra", "snf_reset_statistics", "snf_state_len", "snf_shoot", "snf_jvp_batch",
 )
 KERNEL_CLASSES = ("xz_A", "xz_B", "xz_C", "y_lines", "small")
 
@@ -99,6 +99,10 @@ def load_library(path=LIB_PATH):
     lib.snf_energy.argtypes = [H, dp]
     lib.snf_spectra.argtypes = [H, dp, dp]
     lib.snf_reset_statistics.argtypes = [H]
+    lib.snf_state_len.argtypes = [H, P(ctypes.c_int64)]
+    d = ctypes.c_double
+    lib.snf_shoot.argtypes = [H, dp, d, d, d, d, dp, P(SnfRunStatus)]
+    lib.snf_jvp_batch.argtypes = [H, dp, dp, dp, ctypes.c_int32, dp, d, d, d, d, dp]
     _lib = lib
     return lib
 
@@ -250,6 +254,41 @@ class ShearBox:
     def reset_statistics(self):
         """zero the statistics, spectra and their counters (readwrite.f90:1322-1334)"""
         self._check(self.lib.snf_reset_statistics(self.h))
+
+    # ---- shooting: state vectors [vor | phi | u00 | w00] ([y][kx][kz] complex)
+    def pack_state(self, vor, phi, u00, w00):
+        return np.concatenate([_c128(vor).view(np.float64).ravel(),
+                               _c128(phi).view(np.float64).ravel(),
+                               np.asarray(u00, np.float64), np.asarray(w00, np.float64)])
+
+    def unpack_state(self, x):
+        nc2 = 2 * self.my * self.kxn * self.mz
+        shp = (self.my, self.kxn, self.mz)
+        vor = x[:nc2].copy().view(np.complex128).reshape(shp)
+        phi = x[nc2:2 * nc2].copy().view(np.complex128).reshape(shp)
+        return vor, phi, x[2 * nc2:2 * nc2 + self.my].copy(), x[2 * nc2 + self.my:].copy()
+
+    def shoot(self, x, T, t0=0.0, xoffb0=0.0, xofft0=0.0):
+        """Phi_T(x): the shooting map of calc_dns_disk3d (gmres_shear.f90:2951-3120)"""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        st = SnfRunStatus()
+        self._check(self.lib.snf_shoot(self.h, _ptr(x), t0, xoffb0, xofft0, T, _ptr(y),
+                                       ctypes.byref(st)))
+        return y, st
+
+    def jvp_batch(self, x0, dirs, eps, T, y0=None, t0=0.0, xoffb0=0.0, xofft0=0.0):
+        """(Phi_T(x0 + eps_d d) - Phi_T(x0)) / eps_d for the rows d of dirs
+        (matvec_dns_disk3d, gmres_shear.f90:2666-2730)"""
+        x0 = np.ascontiguousarray(x0, np.float64)
+        dirs = np.ascontiguousarray(np.atleast_2d(dirs), np.float64)
+        eps = np.ascontiguousarray(np.broadcast_to(eps, (dirs.shape[0],)), np.float64)
+        out = np.empty_like(dirs)
+        y0a = np.ascontiguousarray(y0, np.float64) if y0 is not None else None
+        self._check(self.lib.snf_jvp_batch(self.h, _ptr(x0), _ptr(y0a) if y0a is not None else None,
+                                           _ptr(dirs), dirs.shape[0], _ptr(eps), t0, xoffb0,
+                                           xofft0, T, _ptr(out)))
+        return out
 
     def set_timing(self, on):
         self._check(self.lib.snf_set_timing(self.h, 1 if on else 0))

@@ -825,10 +825,9 @@ int snf_state_download(snf_handle* h, double* vor, double* phi, double* u00, dou
   return SNF_OK;
 }
 
-int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
-  if (!h) return fail(SNF_BAD_PARAM, "null handle");
-  if (!h->uploaded) return fail(SNF_NOT_READY, "snf_advance before snf_state_upload");
-  if (nstep < 0) return fail(SNF_BAD_PARAM, "nstep < 0");
+// continue_call: this chunk continues the cross call of the previous one
+// (the shooting map runs one cross call in chunks of steps)
+static int advance_impl(snf_handle* h, int32_t nstep, snf_run_status* st, bool continue_call) {
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
   int rc;
@@ -837,7 +836,7 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
   const int done0 = s0.steps_done, nrec0 = s0.nrec;
   // a call that follows a return at a dump continues that run; any other
   // call is a new cross call
-  const int resume = s0.halted && s0.idump && !s0.istop;
+  const int resume = continue_call || (s0.halted && s0.idump && !s0.istop);
   // start of cross: kx=0 Hermitian fix, twist of the current offsets,
   // v = lapcdy(phi) and the first D1(vor), D1(v)
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], resume);
@@ -887,6 +886,72 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
     for (int q = 0; q < 4; ++q) st->umax[q] = s.umax[q];
     st->CFLv = s.CFLv;
     st->nhist_records = s.nrec - nrec0;
+  }
+  return SNF_OK;
+}
+
+int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
+  if (!h) return fail(SNF_BAD_PARAM, "null handle");
+  if (!h->uploaded) return fail(SNF_NOT_READY, "snf_advance before snf_state_upload");
+  if (nstep < 0) return fail(SNF_BAD_PARAM, "nstep < 0");
+  return advance_impl(h, nstep, st, false);
+}
+
+// ------------------------------------------------------------- shooting
+int snf_state_len(snf_handle* h, int64_t* n) {
+  if (!h || !n) return fail(SNF_BAD_PARAM, "null argument");
+  *n = 4 * (int64_t)h->NC + 2 * (int64_t)h->g.my;
+  return SNF_OK;
+}
+
+int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double xofft0, double T,
+              double* y, snf_run_status* st) {
+  if (!h || !x || !y || !(T > 0)) return fail(SNF_BAD_PARAM, "bad argument");
+  const size_t nc2 = 2 * (size_t)h->NC;
+  const int my = h->g.my;
+  int rc = snf_state_upload_ykxkz(h, x, x + nc2, x + 2 * nc2, x + 2 * nc2 + my, t0, xoffb0, xofft0);
+  if (rc) return rc;
+  // calc_dns_disk3d (gmres_shear.f90:2951-3120): statistics restart, one
+  // cross call that stops at etime = t0 + T (cross.f90:1680-1712)
+  if ((rc = snf_reset_statistics(h))) return rc;
+  DevScalars s{};
+  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+  s.etime = t0 + T;
+  s.dumptime = 1e300;
+  s.istop = s.idump = s.halted = 0;
+  HIPCHK(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+  snf_run_status r{};
+  const int chunk = 256;
+  for (int c = 0;; ++c) {
+    if ((rc = advance_impl(h, chunk, &r, c > 0))) return rc;
+    if (r.stop == SNF_STOP) break;
+    if (c > (1 << 22) / chunk) return fail(SNF_DEVICE_ERROR, "shooting did not reach t0 + T");
+  }
+  if (st) *st = r;
+  return snf_state_download_ykxkz(h, y, y + nc2, y + 2 * nc2, y + 2 * nc2 + my);
+}
+
+int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
+                  int32_t ndir, const double* eps, double t0, double xoffb0, double xofft0,
+                  double T, double* out) {
+  if (!h || !x0 || !dirs || !eps || !out || ndir < 1) return fail(SNF_BAD_PARAM, "bad argument");
+  int64_t n = 0;
+  snf_state_len(h, &n);
+  std::vector<double> base((size_t)n), x((size_t)n), y((size_t)n);
+  int rc;
+  if (y0) {
+    std::memcpy(base.data(), y0, (size_t)n * sizeof(double));
+  } else if ((rc = snf_shoot(h, x0, t0, xoffb0, xofft0, T, base.data(), nullptr))) {
+    return rc;
+  }
+  for (int d = 0; d < ndir; ++d) {
+    const double e = eps[d];
+    if (!(e != 0.0)) return fail(SNF_BAD_PARAM, "eps must be nonzero");
+    const double* dd = dirs + (size_t)d * n;
+    for (int64_t q = 0; q < n; ++q) x[q] = x0[q] + e * dd[q];
+    if ((rc = snf_shoot(h, x.data(), t0, xoffb0, xofft0, T, y.data(), nullptr))) return rc;
+    double* o = out + (size_t)d * n;
+    for (int64_t q = 0; q < n; ++q) o[q] = (y[q] - base[q]) / e;
   }
   return SNF_OK;
 }
