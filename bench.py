@@ -30,6 +30,9 @@ def parse():
     ap.add_argument("--explicit", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed precision leg")
+    ap.add_argument("--batch", type=int, default=32,
+                    help="boxes of the ensemble leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="precision of the main timed leg (mixed: no separate mixed leg)")
     ap.add_argument("--cmp-steps", type=int, default=500,
@@ -156,6 +159,48 @@ def mixed_error(snf, ic, grid, Re, explicit, nstep):
     return err
 
 
+def batch_leg(snf, make_ic, nbox, steps, seed):
+    """Ensemble of independent boxes (grids n^3 and n x (n+1) x n, n cycling
+    over 32, 48, 64, 96; Re_z uniform in [500, 3000]; seeded fields): box-steps
+    per second with every box's steps in flight at once (snf_advance_batch),
+    against the same boxes advanced one after another."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = (32, 48, 64, 96)
+    specs = []
+    for q in range(nbox):
+        n = ns[q % 4]
+        my = n + 1 if (q // 4) % 2 == 0 else n
+        specs.append(((n, my, n), float(rng.uniform(500.0, 3000.0)), seed + 1 + q))
+    ics = [make_ic(*g, math.pi, 2 * math.pi, 1.0, seed=sd) for g, _, sd in specs]
+
+    def boxes():
+        out = []
+        for (g, Re, _), ic in zip(specs, ics):
+            b = snf.ShearBox(*g, Re, nhist=10 ** 9)
+            b.set_state(*ic)
+            out.append(b)
+        return out
+
+    rates = {}
+    for mode in ("sequential", "batched"):
+        bs = boxes()
+        snf.advance_batch(bs, 2)                        # warm-up
+        t0 = time.perf_counter()
+        if mode == "batched":
+            st = snf.advance_batch(bs, steps)
+        else:
+            st = [b.advance_async_status(steps) for b in bs]
+        t1 = time.perf_counter()
+        rates[mode] = sum(s.steps_done for s in st) / (t1 - t0)
+        for b in bs:
+            b.close()
+    return {"boxes": nbox, "steps": steps, "grids": sorted({"%dx%dx%d" % g for g, _, _ in specs}),
+            "unit": "box-steps/s", "dtype": "f64",
+            "batched": rates["batched"], "sequential": rates["sequential"],
+            "speedup": rates["batched"] / rates["sequential"]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -234,6 +279,8 @@ def main():
             "rel_err_vs_fp64": mixed_error(snf, ic, grid, a.Re, a.explicit, a.cmp_steps)
             if a.cmp_steps > 0 else None,
         }
+    if a.batch > 0 and a.precision == "fp64":
+        out["ensemble"] = batch_leg(snf, make_ic, a.batch, a.batch_steps, seed)
     if not a.no_cpu:
         if a.cpu_grid:
             cg = tuple(int(x) for x in a.cpu_grid.split(","))

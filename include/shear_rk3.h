@@ -144,6 +144,11 @@ int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double x
 int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
                   int32_t ndir, const double* eps, double t0, double xoffb0, double xofft0,
                   double T, double* out);
+/* ensembles: advances n independent handles by up to nstep steps each, as
+ * n snf_advance calls would, with every handle's steps in flight on its own
+ * stream before any is waited for (no counterpart in the reference, which
+ * runs one box per MPI job).  st: n statuses (or null).                  */
+int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_status* st);
 /* copies up to max_records records (SNF_UNER_LEN doubles each) produced
  * since the last call; *n receives the count                             */
 int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n);

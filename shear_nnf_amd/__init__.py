@@ -33,6 +33,7 @@ EXPORTS = (
     "snf_op_derivy", "snf_op_helmholtz", "snf_op_fourxz", "snf_op_hvhg",
     "snf_set_timing", "snf_get_timing", "snf_copy_probe", "snf_op_getv", "snf_energy",
     "snf_spectra", "snf_reset_statistics", "snf_state_len", "snf_shoot", "snf_jvp_batch",
+    "snf_advance_batch",
 )
 KERNEL_CLASSES = ("xz_A", "xz_B", "xz_C", "y_lines", "small")
 
@@ -100,6 +101,7 @@ def load_library(path=LIB_PATH):
     lib.snf_spectra.argtypes = [H, dp, dp]
     lib.snf_reset_statistics.argtypes = [H]
     lib.snf_state_len.argtypes = [H, P(ctypes.c_int64)]
+    lib.snf_advance_batch.argtypes = [P(H), ctypes.c_int32, ctypes.c_int32, P(SnfRunStatus)]
     d = ctypes.c_double
     lib.snf_shoot.argtypes = [H, dp, d, d, d, d, dp, P(SnfRunStatus)]
     lib.snf_jvp_batch.argtypes = [H, dp, dp, dp, ctypes.c_int32, dp, d, d, d, d, dp]
@@ -123,6 +125,22 @@ def _ptr(a):
 
 def _c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def advance_batch(boxes, nstep):
+    """Advances every ShearBox of `boxes` by up to nstep steps with all their
+    work in flight at once (one stream each); returns the statuses and
+    appends the UNER records to each box's history."""
+    lib = load_library()
+    n = len(boxes)
+    hs = (ctypes.c_void_p * n)(*[b.h.value for b in boxes])
+    st = (SnfRunStatus * n)()
+    rc = lib.snf_advance_batch(hs, n, int(nstep), st)
+    if rc != SNF_OK:
+        raise SnfError(f"snf error {rc}: {lib.snf_last_error().decode()}")
+    for b in boxes:
+        b.fetch_history()
+    return list(st)
 
 
 class SnfError(RuntimeError):

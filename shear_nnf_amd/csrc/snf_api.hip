@@ -825,18 +825,32 @@ int snf_state_download(snf_handle* h, double* vor, double* phi, double* u00, dou
   return SNF_OK;
 }
 
+// An advance is split in three so that several handles can have their steps
+// in flight on their own streams at once (snf_advance_batch): begin reads
+// the run control, enqueue issues every launch of nstep steps without a
+// host synchronisation, finish waits and reports.
+struct AdvanceMark {
+  int done0, nrec0, resume;
+};
+
 // continue_call: this chunk continues the cross call of the previous one
 // (the shooting map runs one cross call in chunks of steps)
-static int advance_impl(snf_handle* h, int32_t nstep, snf_run_status* st, bool continue_call) {
+static int advance_begin(snf_handle* h, bool continue_call, AdvanceMark& m) {
+  HIPCHK(hipSetDevice(h->dev));
+  DevScalars s0{};
+  HIPCHK(hipMemcpy(&s0, h->sc, sizeof(s0), hipMemcpyDeviceToHost));
+  m.done0 = s0.steps_done;
+  m.nrec0 = s0.nrec;
+  // a call that follows a return at a dump continues that run; any other
+  // call is a new cross call
+  m.resume = continue_call || (s0.halted && s0.idump && !s0.istop);
+  return SNF_OK;
+}
+
+static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
   int rc;
-  DevScalars s0{};
-  HIPCHK(hipMemcpy(&s0, h->sc, sizeof(s0), hipMemcpyDeviceToHost));
-  const int done0 = s0.steps_done, nrec0 = s0.nrec;
-  // a call that follows a return at a dump continues that run; any other
-  // call is a new cross call
-  const int resume = continue_call || (s0.halted && s0.idump && !s0.istop);
   // start of cross: kx=0 Hermitian fix, twist of the current offsets,
   // v = lapcdy(phi) and the first D1(vor), D1(v)
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], resume);
@@ -872,12 +886,17 @@ static int advance_impl(snf_handle* h, int32_t nstep, snf_run_status* st, bool c
       h->cur = 1 - h->cur;
     }
   }
+  return SNF_OK;
+}
+
+static int advance_finish(snf_handle* h, const AdvanceMark& m, snf_run_status* st) {
+  HIPCHK(hipSetDevice(h->dev));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (h->timing) collect_timing(h);
   DevScalars s{};
   HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
   if (st) {
-    st->steps_done = s.steps_done - done0;
+    st->steps_done = s.steps_done - m.done0;
     st->stop = s.istop ? SNF_STOP : (s.idump ? SNF_DUMP : SNF_RUNNING);
     st->time = s.time;
     st->Deltat = s.Deltat;
@@ -885,9 +904,17 @@ static int advance_impl(snf_handle* h, int32_t nstep, snf_run_status* st, bool c
     st->xofft = s.xofft;
     for (int q = 0; q < 4; ++q) st->umax[q] = s.umax[q];
     st->CFLv = s.CFLv;
-    st->nhist_records = s.nrec - nrec0;
+    st->nhist_records = s.nrec - m.nrec0;
   }
   return SNF_OK;
+}
+
+static int advance_impl(snf_handle* h, int32_t nstep, snf_run_status* st, bool continue_call) {
+  AdvanceMark m{};
+  int rc;
+  if ((rc = advance_begin(h, continue_call, m))) return rc;
+  if ((rc = advance_enqueue(h, nstep, m.resume))) return rc;
+  return advance_finish(h, m, st);
 }
 
 int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
@@ -895,6 +922,25 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
   if (!h->uploaded) return fail(SNF_NOT_READY, "snf_advance before snf_state_upload");
   if (nstep < 0) return fail(SNF_BAD_PARAM, "nstep < 0");
   return advance_impl(h, nstep, st, false);
+}
+
+int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_status* st) {
+  if (!hs || n < 1 || nstep < 0) return fail(SNF_BAD_PARAM, "bad argument");
+  for (int i = 0; i < n; ++i) {
+    if (!hs[i]) return fail(SNF_BAD_PARAM, "null handle");
+    if (!hs[i]->uploaded) return fail(SNF_NOT_READY, "snf_advance_batch before snf_state_upload");
+    for (int q = 0; q < i; ++q)
+      if (hs[q] == hs[i]) return fail(SNF_BAD_PARAM, "a handle appears twice");
+  }
+  std::vector<AdvanceMark> m((size_t)n);
+  int rc;
+  for (int i = 0; i < n; ++i)
+    if ((rc = advance_begin(hs[i], false, m[i]))) return rc;
+  for (int i = 0; i < n; ++i)
+    if ((rc = advance_enqueue(hs[i], nstep, m[i].resume))) return rc;
+  for (int i = 0; i < n; ++i)
+    if ((rc = advance_finish(hs[i], m[i], st ? st + i : nullptr))) return rc;
+  return SNF_OK;
 }
 
 // ------------------------------------------------------------- shooting

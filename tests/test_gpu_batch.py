@@ -1,0 +1,43 @@
+"""Ensembles: snf_advance_batch advances independent boxes of different
+grids, Reynolds numbers and modes with all their work in flight at once;
+every box must end exactly where its own snf_advance takes it (bitwise:
+the same kernels on the same data, only the interleaving differs)."""
+import numpy as np
+import pytest
+
+from tests.boxes import ALP, GAM, LY, box_ic
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((32, 33, 32), 500.0, {}), ((48, 49, 48), 3000.0, {"explicit": 0}),
+         ((64, 65, 64), 1200.0, {}), ((96, 97, 96), 2000.0, {}),
+         ((128, 49, 128), 1000.0, {"precision": "mixed"}), ((48, 49, 48), 800.0, {"ydealias": 3})]
+
+
+def _boxes():
+    import shear_nnf_amd as snf
+    out = []
+    for q, (grid, Re, kw) in enumerate(CASES):
+        b = snf.ShearBox(*grid, Re, alp=ALP, gam=GAM, Ly=LY, **kw)
+        b.set_state(*box_ic(*grid, seed=20260001 + q))
+        out.append(b)
+    return out
+
+
+def test_batch_equals_separate_advances():
+    import shear_nnf_amd as snf
+    batch, single = _boxes(), _boxes()
+    st = snf.advance_batch(batch, 6)
+    for b, s, r in zip(batch, single, st):
+        r1 = s.cross(6)
+        assert r.steps_done == r1.steps_done == 6 and r.time == r1.time
+        for x, y in zip(b.get_state(), s.get_state()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(np.array(b.history), np.array(s.history))
+
+
+def test_batch_rejects_a_repeated_handle():
+    import shear_nnf_amd as snf
+    b = _boxes()[:1]
+    with pytest.raises(snf.SnfError):
+        snf.advance_batch([b[0], b[0]], 1)
