@@ -280,14 +280,17 @@ void launch_xzw(snf_handle* h, int which, int rk) {
     case 3: launch_xzw_A<3, H>(h); break;
     case 4: launch_xzw_A<4, H>(h); break;
     case 6: launch_xzw_A<6, H>(h); break;
+    case 8: launch_xzw_A<8, H>(h); break;
     case 10: launch_xzw_B<2, H>(h, rk); break;
     case 11: launch_xzw_B<3, H>(h, rk); break;
     case 12: launch_xzw_B<4, H>(h, rk); break;
     case 14: launch_xzw_B<6, H>(h, rk); break;
+    case 16: launch_xzw_B<8, H>(h, rk); break;
     case 18: launch_xzw_C<2, H>(h); break;
     case 19: launch_xzw_C<3, H>(h); break;
     case 20: launch_xzw_C<4, H>(h); break;
-    default: launch_xzw_C<6, H>(h); break;
+    case 22: launch_xzw_C<6, H>(h); break;
+    default: launch_xzw_C<8, H>(h); break;
   }
 }
 // which: 0 = A, 1 = B, 2 = C
@@ -647,12 +650,14 @@ int snf_create(const snf_params* p, snf_handle** out) {
     return fail(SNF_BAD_PARAM, "x-z transform sizes exceed the LDS plan of this build");
   }
   {
-    auto wave_size = [](int n) { return n % 64 == 0 && (n / 64 == 2 || n / 64 == 3 || n / 64 == 4 || n / 64 == 6); };
+    auto wave_size = [](int n) {
+      return n % 64 == 0 && (n / 64 == 2 || n / 64 == 3 || n / 64 == 4 || n / 64 == 6 || n / 64 == 8);
+    };
     const char* e = getenv("SNF_XZ_PATH");
     h->wave_xz = wave_size(mgalx) && wave_size(mgalz) && !(e && std::string(e) == "generic");
     if (h->mixed && !h->wave_xz) {
       snf_destroy(h);
-      return fail(SNF_BAD_PARAM, "SNF_PREC_MIXED needs mgalx, mgalz in {128, 192, 256, 384}");
+      return fail(SNF_BAD_PARAM, "SNF_PREC_MIXED needs mgalx, mgalz in {128, 192, 256, 384, 512}");
     }
     if (h->wave_xz) {
       h->rx = mgalx / 64;
@@ -661,19 +666,22 @@ int snf_create(const snf_params* p, snf_handle** out) {
       h->ldsWA = xzw_lds_A(h->rz, es);
       h->ldsWC = xzw_lds_C(h->rz, es);
       h->ldsWB = xzw_lds_B(h->rx, g.kxn, es);
-      const void* ka[8] = {(const void*)k_xzw_A<2, cplx>, (const void*)k_xzw_A<3, cplx>,
+      const void* ka[10] = {(const void*)k_xzw_A<2, cplx>, (const void*)k_xzw_A<3, cplx>,
                            (const void*)k_xzw_A<4, cplx>, (const void*)k_xzw_A<6, cplx>,
                            (const void*)k_xzw_A<2, cplxf>, (const void*)k_xzw_A<3, cplxf>,
-                           (const void*)k_xzw_A<4, cplxf>, (const void*)k_xzw_A<6, cplxf>};
-      const void* kb[8] = {(const void*)k_xzw_B<2, cplx>, (const void*)k_xzw_B<3, cplx>,
+                           (const void*)k_xzw_A<4, cplxf>, (const void*)k_xzw_A<6, cplxf>,
+                            (const void*)k_xzw_A<8, cplx>, (const void*)k_xzw_A<8, cplxf>};
+      const void* kb[10] = {(const void*)k_xzw_B<2, cplx>, (const void*)k_xzw_B<3, cplx>,
                            (const void*)k_xzw_B<4, cplx>, (const void*)k_xzw_B<6, cplx>,
                            (const void*)k_xzw_B<2, cplxf>, (const void*)k_xzw_B<3, cplxf>,
-                           (const void*)k_xzw_B<4, cplxf>, (const void*)k_xzw_B<6, cplxf>};
-      const void* kc[8] = {(const void*)k_xzw_C<2, cplx>, (const void*)k_xzw_C<3, cplx>,
+                           (const void*)k_xzw_B<4, cplxf>, (const void*)k_xzw_B<6, cplxf>,
+                            (const void*)k_xzw_B<8, cplx>, (const void*)k_xzw_B<8, cplxf>};
+      const void* kc[10] = {(const void*)k_xzw_C<2, cplx>, (const void*)k_xzw_C<3, cplx>,
                            (const void*)k_xzw_C<4, cplx>, (const void*)k_xzw_C<6, cplx>,
                            (const void*)k_xzw_C<2, cplxf>, (const void*)k_xzw_C<3, cplxf>,
-                           (const void*)k_xzw_C<4, cplxf>, (const void*)k_xzw_C<6, cplxf>};
-      for (int q = 0; q < 8; ++q) {
+                           (const void*)k_xzw_C<4, cplxf>, (const void*)k_xzw_C<6, cplxf>,
+                            (const void*)k_xzw_C<8, cplx>, (const void*)k_xzw_C<8, cplxf>};
+      for (int q = 0; q < 10; ++q) {
         (void)hipFuncSetAttribute(ka[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWA);
         (void)hipFuncSetAttribute(kb[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWB);
         (void)hipFuncSetAttribute(kc[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWC);

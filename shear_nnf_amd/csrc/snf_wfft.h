@@ -1,5 +1,5 @@
-// Wave-level FFT of N = 64*R0 points (R0 = 2, 3, 4, 6: N = 128, 192, 256,
-// 384).  One wave transforms one sequence held in its registers; the two
+// Wave-level FFT of N = 64*R0 points (R0 = 2, 3, 4, 6, 8: N = 128, 192, 256,
+// 384, 512).  One wave transforms one sequence held in its registers; the two
 // exchanges between stages go through a wave-private LDS region, so no
 // block barrier is needed.  Index split N = R0 x 8 x 8:
 //

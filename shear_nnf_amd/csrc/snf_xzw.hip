@@ -1,5 +1,5 @@
 // Nonlinear stage of one RK3 substep (hvhg, cross.f90:1127-1765) for planes
-// of 64*R points in x and z (R = 2, 3, 4, 6), built on the wave-level FFT of
+// of 64*R points in x and z (R = 2, 3, 4, 6, 8), built on the wave-level FFT of
 // snf_wfft.h: every transform runs inside one wave, in registers, with two
 // wave-private LDS exchanges and no block barrier.
 //

@@ -128,11 +128,15 @@ def _hvhg_pair(grid, seed=11):
 
 # planes of 64*R points take the wave-FFT kernels (R = 2, 3, 4, 6)
 @pytest.mark.parametrize("grid", [(128, 17, 128), (192, 17, 192), (256, 17, 256),
-                                  (384, 17, 384), (192, 17, 384), (384, 17, 128)])
+                                  (384, 17, 384), (192, 17, 384), (384, 17, 128), (512, 17, 512),
+                                  (512, 17, 192)])
 def test_hvhg_wave_kernels(lib, grid):
     got, ref = _hvhg_pair(grid)
+    # 512-point planes: longer transforms (9 FFT stages each way) round a
+    # little more, ~1.1e-13 against numpy's FFT
+    tol = 2e-13 if max(grid) >= 512 else OP_TOL
     for a, b in zip(got, ref):
-        assert rel_l2(a, b) < OP_TOL
+        assert rel_l2(a, b) < tol
 
 
 def test_wave_and_generic_xz_paths_agree(lib, monkeypatch):
