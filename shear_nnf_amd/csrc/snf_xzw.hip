@@ -124,20 +124,21 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   const long fstride = (long)g.my * kxn * Nz;
   const long rowbase = (long)j * kxn * Nz + z0;   // + kx*Nz + r
   // --- load the 3 field pairs of row r in lane layout (bins l + 64 m) ---
-  // every load of the block is issued up front (runs of 8 z values, 128 B);
+  // every load of the block is issued up front (runs of 4 z values, 64 B),
+  // field by field: element i of a field is (kx, z) = (i / WB_ROWS, i % WB_ROWS);
   // the tile then passes through LDS one field pair at a time
   constexpr int KXM = 64 * RX / 3;                                  // kxn bound
-  constexpr int NE = (6 * KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);
+  constexpr int NEF = (KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);   // per field
   const int tile = kxn * WB_ROWS;                                   // per field
-  H pre[NE];
+  const int i0 = threadIdx.x, kx0 = i0 / WB_ROWS, rr0 = i0 % WB_ROWS;
+  constexpr int KXS = 64;                                           // kx step per q
+  H pre[6][NEF];
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
-    const int e = threadIdx.x + q * 64 * WB_ROWS;
-    if (e < 6 * tile) {
-      const int f = e / tile, rem = e - f * tile;
-      const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
-      pre[q] = gld(hin + f * fstride + rowbase + (long)kx * Nz + rr);
-    }
+  for (int f = 0; f < 6; ++f) {
+    const H* src = hin + f * fstride + rowbase + rr0;
+#pragma unroll
+    for (int q = 0; q < NEF; ++q)
+      if (i0 + q * 64 * WB_ROWS < tile) pre[f][q] = gld(src + (long)(kx0 + q * KXS) * Nz);
   }
   H zin[3][RX];
   H* S = work;                           // [2][kxn][WB_ROWS + 1]
@@ -146,15 +147,10 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   for (int p = 0; p < 3; ++p) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NE; ++q) {
-      const int e = threadIdx.x + q * 64 * WB_ROWS;
-      const int f = e / tile;
-      if (e < 6 * tile && (f >> 1) == p) {
-        const int rem = e - f * tile;
-        const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
-        S[((f & 1) * kxn + kx) * SP + rr] = pre[q];
-      }
-    }
+    for (int h2 = 0; h2 < 2; ++h2)
+#pragma unroll
+      for (int q = 0; q < NEF; ++q)
+        if (i0 + q * 64 * WB_ROWS < tile) S[(h2 * kxn + kx0 + q * KXS) * SP + rr0] = pre[2 * p + h2][q];
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < RX; ++m) {
@@ -234,11 +230,13 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 3 * kxn * WB_ROWS; e += blockDim.x) {
-    const int f = e / (kxn * WB_ROWS);
-    const int rem = e - f * kxn * WB_ROWS;
-    const int kx = rem / WB_ROWS, rr = rem - kx * WB_ROWS;
-    gst(hout + f * fstride + rowbase + (long)kx * Nz + rr, O[(f * kxn + kx) * SP + rr]);
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    H* dst = hout + f * fstride + rowbase + rr0;
+#pragma unroll
+    for (int q = 0; q < NEF; ++q)
+      if (i0 + q * 64 * WB_ROWS < tile)
+        gst(dst + (long)(kx0 + q * KXS) * Nz, O[(f * kxn + kx0 + q * KXS) * SP + rr0]);
   }
   // --- reductions: one atomic per wave for umax, one partial per row ---
   if (rk == 1) {
