@@ -616,8 +616,9 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
     TRY(hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y twiddles"));
-    // tile, a column per wave, roots, two multiplier rows
-    h->ldsY = (size_t)my * (2 * YF_LINES + 5) * sizeof(cplx);
+    // tile, a column per wave, roots, two multiplier rows, four phase tables
+    // (1472 B at my = 385: 81552 B, two blocks still fit one CU's 160 KiB)
+    h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
     for (const void* ky : {(const void*)k_yf_explicit<3, 11>, (const void*)k_yf_explicit<7, 7>,
                            (const void*)k_yf_explicit<5, 7, 11>, (const void*)k_yf_filter<3, 11>,
                            (const void*)k_yf_filter<7, 7>, (const void*)k_yf_filter<5, 7, 11>})

@@ -42,6 +42,31 @@ struct YPhase {
   }
 };
 
+// Phases exp(+-i (a j + a0)) of a block at the rows j = lane + 64 q, from a
+// table of YPH_N(Q) values that the block fills once:
+//   [exp(i a l), l < 8] [exp(i 8 a l), l < 8] [exp(i (64 a q + a0)), q < Q]
+// so that a phase costs two table reads per lane and one product per row.
+#define YPH_N(Q) (16 + (Q))
+template <bool NEG>
+struct YPh {
+  cplx p;
+  const cplx* c;
+  __device__ YPh(const cplx* tab, int lane) : c(tab + 16) {
+    p = cmul(tab[lane & 7], tab[8 + (lane >> 3)]);
+  }
+  __device__ cplx at(int q) const {
+    const cplx e = cmul(p, c[q]);
+    return NEG ? cconj(e) : e;
+  }
+};
+// entry t of the tables of nb phases (a[b], a0[b]), filled by the block
+__device__ __forceinline__ cplx yph_entry(int t, int Q, const double* a, const double* a0) {
+  const int b = t / YPH_N(Q), r = t - b * YPH_N(Q);
+  const double ang = r < 8 ? a[b] * (double)r
+                   : (r < 16 ? a[b] * (double)(8 * (r - 8)) : a[b] * (double)(64 * (r - 16)) + a0[b]);
+  return yf_expi(ang);
+}
+
 // block-collective transfers of field F (stride P) for lines L0..L0+nv-1
 __device__ __forceinline__ void yf_fill(cplx* T, const cplx* F, long P, long L0, int nv, int N) {
   __syncthreads();
@@ -211,6 +236,16 @@ k_yf_explicit(YArgs a, int rk) {
     syo[t] = gld(a.ysym_old + src);
     syn[t] = gld(a.ysym_new + src);
   }
+  // phase tables: old twist, new twist, the two moving-frame maps
+  cplx* pho = syn + 2 * N;
+  cplx* phn = pho + YPH_N(Q);
+  cplx* pm1 = phn + YPH_N(Q);
+  cplx* pm2 = pm1 + YPH_N(Q);
+  {
+    const double pa[4] = {tho, thn, kx * f1 * g.dy, kx * f2 * g.dy};
+    const double pa0[4] = {0.0, 0.0, -0.5 * kx * f1 * g.Ly, -0.5 * kx * f2 * g.Ly};
+    for (int t = threadIdx.x; t < 4 * YPH_N(Q); t += blockDim.x) pho[t] = yph_entry(t, Q, pa, pa0);
+  }
 
   // ===== omega_y chain =====
   // one field's loads stay in flight in pf while the block transforms
@@ -218,11 +253,11 @@ k_yf_explicit(YArgs a, int rk) {
   yf_fill(T, a.vor, P, L0, nv, N);
   yf_issue(pf, a.rhg, P, L0, nv, N);
   {
-    YPhase ph(-tho, 0.0, lane);
+    YPh<true> ph(pho, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) {
         x[q] = T[tc(j)];
         T[ac(j)] = cmul(x[q], e);
@@ -243,22 +278,22 @@ k_yf_explicit(YArgs a, int rk) {
   yf_commit(T, pf, N);                     // rhg
   if (rk > 1) yf_issue(pf, a.vorwk, P, L0, nv, N);
   {
-    YPhase ph(tho, 0.0, lane);
+    YPh<false> ph(pho, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hg
     }
   }
   if (rk > 1) yf_commit(T, pf, N);        // vorwk
   {
-    YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
-    YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
+    YPh<false> m1(pm1, lane);
+    YPh<false> m2(pm2, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx p1 = m1.next(), p2 = m2.next();
+      const cplx p1 = m1.at(q), p2 = m2.at(q);
       if (j < N) {
         const cplx hgj = T[ac(j)];
         cplx vwk = (rk == 1) ? crmul(gam, hgj) : cadd(T[tc(j)], crmul(gam, hgj));
@@ -271,11 +306,11 @@ k_yf_explicit(YArgs a, int rk) {
   }
   yf_flush(T, a.vorwk, P, L0, nv, N);
   {
-    YPhase ph(-thn, 0.0, lane);
+    YPh<true> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) {
         T[tc(j)] = x[q];
         T[ac(j)] = cmul(x[q], e);
@@ -295,11 +330,11 @@ k_yf_explicit(YArgs a, int rk) {
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
   {
-    YPhase ph(thn, 0.0, lane);
+    YPh<false> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
     }
   }
@@ -309,11 +344,11 @@ k_yf_explicit(YArgs a, int rk) {
   yf_commit(T, pf, N);                     // rdome
   yf_issue(pf, a.phi, P, L0, nv, N);
   {
-    YPhase ph(-tho, 0.0, lane);
+    YPh<true> ph(pho, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) T[ac(j)] = cmul(T[tc(j)], e);
     }
   }
@@ -329,11 +364,11 @@ k_yf_explicit(YArgs a, int rk) {
   yf_commit(T, pf, N);                     // phi
   yf_issue(pf, a.rhv, P, L0, nv, N);
   {
-    YPhase ph(-tho, 0.0, lane);
+    YPh<true> ph(pho, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) {
         x[q] = T[tc(j)];
         T[tc(j)] = cmul(x[q], e);
@@ -354,22 +389,22 @@ k_yf_explicit(YArgs a, int rk) {
   yf_commit(T, pf, N);                     // rhv
   if (rk > 1) yf_issue(pf, a.phiwk, P, L0, nv, N);
   {
-    YPhase ph(tho, 0.0, lane);
+    YPh<false> ph(pho, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hv
     }
   }
   if (rk > 1) yf_commit(T, pf, N);        // phiwk
   {
-    YPhase m1(kx * f1 * g.dy, -0.5 * kx * f1 * g.Ly, lane);
-    YPhase m2(kx * f2 * g.dy, -0.5 * kx * f2 * g.Ly, lane);
+    YPh<false> m1(pm1, lane);
+    YPh<false> m2(pm2, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx p1 = m1.next(), p2 = m2.next();
+      const cplx p1 = m1.at(q), p2 = m2.at(q);
       if (j < N) {
         const cplx hvj = T[ac(j)];
         cplx pwk = (rk == 1) ? crmul(gam, hvj) : cadd(T[tc(j)], crmul(gam, hvj));
@@ -392,11 +427,11 @@ k_yf_explicit(YArgs a, int rk) {
   if (L0 == 0) yf_fill(T, a.v, P, L0, nv, N);   // block of the (0,0) mode
   if (!is00) {
     {
-      YPhase ph(-thn, 0.0, lane);
+      YPh<true> ph(phn, lane);
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int j = lane + 64 * q;
-        const cplx e = ph.next();
+        const cplx e = ph.at(q);
         if (j < N) T[ac(j)] = cmul(x[q], e);
       }
     }
@@ -414,11 +449,11 @@ k_yf_explicit(YArgs a, int rk) {
     }
     yf_fft<1, OwnCol, Rs...>(T, ac, rt);
     yf_fft<1, TileCol, Rs...>(T, tc, rt);
-    YPhase ph(thn, 0.0, lane);
+    YPh<false> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) {
         x[q] = cmul(T[ac(j)], e);              // v
         T[tc(j)] = cmul(T[tc(j)], e);          // dv
@@ -426,11 +461,11 @@ k_yf_explicit(YArgs a, int rk) {
     }
   } else {
     {
-      YPhase ph(-thn, 0.0, lane);
+      YPh<true> ph(phn, lane);
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int j = lane + 64 * q;
-        const cplx e = ph.next();
+        const cplx e = ph.at(q);
         if (j < N) {
           x[q] = T[tc(j)];                     // v of the (0,0) mode, unchanged
           T[ac(j)] = cmul(x[q], e);
@@ -447,11 +482,11 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
     yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-    YPhase ph(thn, 0.0, lane);
+    YPh<false> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
-      const cplx e = ph.next();
+      const cplx e = ph.at(q);
       if (j < N) T[tc(j)] = cmul(T[ac(j)], e);   // dv
     }
   }
