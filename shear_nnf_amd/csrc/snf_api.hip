@@ -122,6 +122,7 @@ struct snf_handle {
   bool yfft = false;
   bool ysym_on = false;   // multiplier tables maintained (spectral y or filter)
   cplx* twy = nullptr;
+  cplx* tpy = nullptr;                  // pass-major twiddles of the y plan
   cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
   size_t ldsY = 0;
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
@@ -201,6 +202,7 @@ YArgs yargs(snf_handle* h) {
   a.sc = h->sc;
   a.PS = h->PS;
   a.twy = h->twy;
+  a.tpy = h->tpy;
   a.ysym_old = h->ysym[h->cur];
   a.ysym_new = h->ysym[1 - h->cur];
   return a;
@@ -377,9 +379,9 @@ void launch_yf(snf_handle* h, int rk) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   switch (h->g.my) {
-    case 33: k_yf_explicit<3, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    case 33: k_yf_explicit<11, 3><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
     case 49: k_yf_explicit<7, 7><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-    default: k_yf_explicit<5, 7, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    default: k_yf_explicit<11, 7, 5><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
   }
 }
 
@@ -410,9 +412,9 @@ void launch_yfilter(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   switch (h->g.my) {
-    case 33: k_yf_filter<3, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    case 33: k_yf_filter<11, 3><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
     case 49: k_yf_filter<7, 7><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-    default: k_yf_filter<5, 7, 11><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    default: k_yf_filter<11, 7, 5><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
   }
 }
 
@@ -439,6 +441,16 @@ const char* snf_last_error(void) { return g_err.c_str(); }
 
 // line lengths my with a compiled spectral y plan (radices of the passes)
 static bool y_plan(int my) { return my == 33 || my == 49 || my == 385; }
+// radices of the compiled y-transform plans in pass order (the template
+// arguments of k_yf_explicit / k_yf_filter); returns the number of passes
+static int y_radices(int my, int* R) {
+  switch (my) {
+    case 33: R[0] = 11; R[1] = 3; return 2;
+    case 49: R[0] = 7; R[1] = 7; return 2;
+    case 385: R[0] = 11; R[1] = 7; R[2] = 5; return 3;
+  }
+  return 0;
+}
 
 int snf_create(const snf_params* p, snf_handle** out) {
   if (!p || !out) return fail(SNF_BAD_PARAM, "null argument");
@@ -568,6 +580,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
   TRY(A(&h->upart, (size_t)UMAX_SLOTS * 4));
   TRY(A(&h->twy, (size_t)my));
+  TRY(A(&h->tpy, (size_t)my));
   for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * YS));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
@@ -616,12 +629,32 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
     TRY(hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y twiddles"));
+    // pass s > 0 of the plan: exp(-2 pi i r k / (Ns R)) at [r - 1][k]
+    std::vector<cplx> tp(my, mkc(0.0, 0.0));
+    int R[4];
+    const int np = y_radices(my, R);
+    if (np > 0) {
+      if (yf_twiddle_count(R, np) > my) return fail(SNF_BAD_PARAM, "y twiddle table larger than my");
+      int Ns = R[0], off = 0;
+      for (int s = 1; s < np; ++s) {
+        for (int r = 1; r < R[s]; ++r)
+          for (int k = 0; k < Ns; ++k) {
+            const long double a =
+                -2.0L * 3.14159265358979323846264338327950288L * r * k / ((long double)Ns * R[s]);
+            tp[off + (r - 1) * Ns + k] = mkc((double)cosl(a), (double)sinl(a));
+          }
+        off += (R[s] - 1) * Ns;
+        Ns *= R[s];
+      }
+    }
+    TRY(hipMemcpy(h->tpy, tp.data(), tp.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
+            ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y pass twiddles"));
     // tile, a column per wave, roots, two multiplier rows, four phase tables
     // (1472 B at my = 385: 81552 B, two blocks still fit one CU's 160 KiB)
     h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
-    for (const void* ky : {(const void*)k_yf_explicit<3, 11>, (const void*)k_yf_explicit<7, 7>,
-                           (const void*)k_yf_explicit<5, 7, 11>, (const void*)k_yf_filter<3, 11>,
-                           (const void*)k_yf_filter<7, 7>, (const void*)k_yf_filter<5, 7, 11>})
+    for (const void* ky : {(const void*)k_yf_explicit<11, 3>, (const void*)k_yf_explicit<7, 7>,
+                           (const void*)k_yf_explicit<11, 7, 5>, (const void*)k_yf_filter<11, 3>,
+                           (const void*)k_yf_filter<7, 7>, (const void*)k_yf_filter<11, 7, 5>})
       (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
@@ -712,7 +745,7 @@ int snf_destroy(snf_handle* h) {
                   h->u00r, h->w00r, h->rf0u, h->rf0w, h->dudy, h->dwdy, h->base, h->tw[0],
                   h->tw[1], h->shp[0], h->shp[1], h->shm[0], h->shm[1], h->twx, h->twz, h->sc,
                   h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring,
-                  h->upart, h->twy, h->ysym[0], h->ysym[1], h->spec, h->spec2};
+                  h->upart, h->twy, h->tpy, h->ysym[0], h->ysym[1], h->spec, h->spec2};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);

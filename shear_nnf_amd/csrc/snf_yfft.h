@@ -16,7 +16,10 @@
 // Transforms: one wave per line, mixed-radix Stockham passes (radices 2, 3,
 // 4, 5, 7, 8, 11) in place on the line's column of an LDS tile, every pass a
 // read-all / write-all of the wave's own column (no block barrier).
-// Twiddles from the table rt[t] = exp(-2 pi i t / N), staged in LDS.
+// Twiddles from a pass-major table staged in LDS: pass s > 0 (stride Ns,
+// radix R) holds exp(-2 pi i r k / (Ns R)) at [r - 1][k], r < R, k < Ns, so
+// that the lanes of a pass read consecutive entries (no bank conflicts);
+// the first pass needs none.  yf_twiddle_table builds it on the host.
 #pragma once
 #include "snf_internal.h"
 #include "snf_wfft.h"
@@ -24,7 +27,6 @@
 #ifndef YF_LINES
 #define YF_LINES 4        // lines (waves) per block; tile rows of 4 lines = 64 B
 #endif
-#define YF_SH (YF_LINES == 2 ? 3 : (YF_LINES == 4 ? 2 : 1))   // log2(16 / YF_LINES)
 
 // Compile-time transform plan: the radices of the passes, in order.  A plan
 // fixes N = prod(Rs) and the number of rows per lane Q = ceil(N / 64).
@@ -36,14 +38,32 @@ template <int R, int... Rs> struct YPlanT<R, Rs...> {
   static constexpr int N = R * YPlanT<Rs...>::N;
 };
 
+// Entries of the pass-major twiddle table of the plan R[0..np-1]:
+// sum over the passes s > 0 of (R[s] - 1) * (R[0] ... R[s-1]); never above N.
+inline int yf_twiddle_count(const int* R, int np) {
+  int n = 0, Ns = R[0];
+  for (int s = 1; s < np; ++s) {
+    n += (R[s] - 1) * Ns;
+    Ns *= R[s];
+  }
+  return n;
+}
+
 // Column accessors.  TileCol: line c of the block's transfer tile, rows of
 // YF_LINES lines with the line index XORed with a function of j so that a
-// wave walking one line down the rows touches distinct LDS banks.  OwnCol: a wave's
+// wave walking one line down the rows touches distinct LDS banks: 16-byte
+// writes are served 8 lanes (128 B) a cycle, reads 16 lanes (256 B), and the
+// swizzle keeps both groups on distinct 16-byte slots.  OwnCol: a wave's
 // private contiguous column.
+__device__ __forceinline__ int yf_swz(int j) {
+  if (YF_LINES == 2) return ((j >> 2) ^ (j >> 3)) & 1;
+  if (YF_LINES == 4) return ((j >> 1) ^ (j >> 3)) & 3;
+  return (j ^ (j >> 3)) & (YF_LINES - 1);
+}
 struct TileCol {
   int c;
   __device__ __forceinline__ int operator()(int j) const {
-    return j * YF_LINES + (c ^ ((j >> YF_SH) & (YF_LINES - 1)));
+    return j * YF_LINES + (c ^ yf_swz(j));
   }
 };
 struct OwnCol {
@@ -120,12 +140,12 @@ __host__ __device__ __forceinline__ void yf_dft(cplx* v) {
 }
 
 // one Stockham pass on column w of the tile: inputs x[b + r nb], outputs
-// (b/Ns) Ns R + b%Ns + r Ns, twiddle exp(SIGN 2 pi i r (b%Ns) / (Ns R))
-template <int R, int SIGN, int N, int Ns, class Col>
-__device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* rt) {
+// (b/Ns) Ns R + b%Ns + r Ns, twiddle exp(SIGN 2 pi i r (b%Ns) / (Ns R)) read
+// from tw[OFF + (r - 1) Ns + b%Ns]
+template <int R, int SIGN, int N, int Ns, int OFF, class Col>
+__device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
   constexpr int nb = N / R;
   constexpr int MB = (nb + 63) / 64;       // butterflies per lane
-  constexpr int stw = N / (Ns * R);
   const int lane = threadIdx.x & 63;
   cplx v[MB][R];
 #pragma unroll
@@ -136,7 +156,7 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* rt) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         cplx x = T[col(b + r * nb)];
-        if (r > 0) x = wf_tw<SIGN>(x, rt[r * k * stw]);
+        if (Ns > 1 && r > 0) x = wf_tw<SIGN>(x, tw[OFF + (r - 1) * Ns + k]);
         v[q][r] = x;
       }
     }
@@ -157,17 +177,17 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* rt) {
 }
 
 // transform of column w in place (natural order in and out), unnormalised
-template <int SIGN, int N, int Ns, class Col>
+template <int SIGN, int N, int Ns, int OFF, class Col>
 __device__ __forceinline__ void yf_passes(cplx*, Col, const cplx*) {}
-template <int SIGN, int N, int Ns, class Col, int R, int... Rs>
-__device__ __forceinline__ void yf_passes(cplx* T, Col col, const cplx* rt) {
-  yf_pass<R, SIGN, N, Ns>(T, col, rt);
-  yf_passes<SIGN, N, Ns * R, Col, Rs...>(T, col, rt);
+template <int SIGN, int N, int Ns, int OFF, class Col, int R, int... Rs>
+__device__ __forceinline__ void yf_passes(cplx* T, Col col, const cplx* tw) {
+  yf_pass<R, SIGN, N, Ns, OFF>(T, col, tw);
+  yf_passes<SIGN, N, Ns * R, (Ns > 1 ? OFF + (R - 1) * Ns : 0), Col, Rs...>(T, col, tw);
 }
 template <int SIGN, class Col, int... Rs>
-__device__ __forceinline__ void yf_fft(cplx* T, Col col, const cplx* rt) {
+__device__ __forceinline__ void yf_fft(cplx* T, Col col, const cplx* tw) {
   wf_sync();   // the column was just written lane by lane
-  yf_passes<SIGN, YPlanT<Rs...>::N, 1, Col, Rs...>(T, col, rt);
+  yf_passes<SIGN, YPlanT<Rs...>::N, 1, 0, Col, Rs...>(T, col, tw);
 }
 
 // seven stencil coefficients held in registers

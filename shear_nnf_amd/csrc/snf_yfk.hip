@@ -149,7 +149,7 @@ k_yf_explicit(YArgs a, int rk) {
   // LDS: tile [N][YF_LINES], one column per wave, the N-th roots, the
   // multiplier rows of this block's kx at the old and new twist
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.twy + t);
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.tpy + t);
   __syncthreads();
   const int ir = rk - 1;
   double dt, f1, f2;
@@ -229,12 +229,12 @@ k_yf_explicit(YArgs a, int rk) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [m][2]
+  cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [2][m]
   cplx* syn = syo + 2 * N;     // ... at the new twist
   for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
     const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
-    syo[t] = gld(a.ysym_old + src);
-    syn[t] = gld(a.ysym_new + src);
+    syo[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
+    syn[(t & 1) * N + (t >> 1)] = gld(a.ysym_new + src);
   }
   // phase tables: old twist, new twist, the two moving-frame maps
   cplx* pho = syn + 2 * N;
@@ -269,7 +269,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = syo[2 * m + 1];
+      const cplx s2 = syo[N + m];
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
@@ -324,7 +324,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = syn[2 * m];
+      const cplx s1 = syn[m];
       T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
     }
   }
@@ -357,7 +357,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = syo[2 * m];
+      const cplx s1 = syo[m];
       T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
     }
   }
@@ -380,7 +380,7 @@ k_yf_explicit(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = syo[2 * m + 1];
+      const cplx s2 = syo[N + m];
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cadd(T[ac(m)], cmul(T[tc(m)], mul));
     }
@@ -440,7 +440,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx s1 = syn[2 * m], s2 = syn[2 * m + 1];
+        const cplx s1 = syn[m], s2 = syn[N + m];
         const cplx H = yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y));   // lapcdy
         const cplx vh = cmul(T[ac(m)], H);
         T[tc(m)] = cmul(vh, s1);
@@ -477,7 +477,7 @@ k_yf_explicit(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx s1 = syn[2 * m];
+        const cplx s1 = syn[m];
         T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
       }
     }
@@ -516,7 +516,7 @@ k_yf_filter(YArgs a) {
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.twy + t);
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.tpy + t);
   __syncthreads();
   const double invN = 1.0 / (double)N;
   const TileCol tc{w};

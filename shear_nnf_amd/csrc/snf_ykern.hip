@@ -172,6 +172,7 @@ struct YArgs {
   DevScalars* sc;
   long PS;
   const cplx* twy;                          // spectral y path: exp(-2 pi i t/my), t < my
+  const cplx* tpy;                          // ... pass-major twiddles of the transform plan
   const cplx *ysym_old, *ysym_new;          // [kxn][my][2] multipliers of d/dy, d2/dy2
 };
 
