@@ -248,10 +248,12 @@ k_yf_explicit(YArgs a, int rk) {
   }
 
   // ===== omega_y chain =====
-  // one field's loads stay in flight in pf while the block transforms
-  cplx pf[Q];
+  // two fields' loads stay in flight (pfa, pfb) while the block transforms;
+  // each is issued about two stages before the tile needs it
+  cplx pfa[Q], pfb[Q];
   yf_fill(T, a.vor, P, L0, nv, N);
-  yf_issue(pf, a.rhg, P, L0, nv, N);
+  yf_issue(pfa, a.rhg, P, L0, nv, N);
+  if (rk > 1) yf_issue(pfb, a.vorwk, P, L0, nv, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -275,8 +277,8 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-  yf_commit(T, pf, N);                     // rhg
-  if (rk > 1) yf_issue(pf, a.vorwk, P, L0, nv, N);
+  yf_commit(T, pfa, N);                    // rhg
+  yf_issue(pfa, a.rdome, P, L0, nv, N);
   {
     YPh<false> ph(pho, lane);
 #pragma unroll
@@ -286,7 +288,8 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hg
     }
   }
-  if (rk > 1) yf_commit(T, pf, N);        // vorwk
+  if (rk > 1) yf_commit(T, pfb, N);       // vorwk
+  yf_issue(pfb, a.phi, P, L0, nv, N);
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -318,7 +321,6 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_flush(T, a.vor, P, L0, nv, N);
-  yf_issue(pf, a.rdome, P, L0, nv, N);
   yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -341,8 +343,8 @@ k_yf_explicit(YArgs a, int rk) {
   yf_flush(T, a.dome, P, L0, nv, N);
 
   // ===== phi chain =====
-  yf_commit(T, pf, N);                     // rdome
-  yf_issue(pf, a.phi, P, L0, nv, N);
+  yf_commit(T, pfa, N);                    // rdome
+  yf_issue(pfa, a.rhv, P, L0, nv, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -361,8 +363,8 @@ k_yf_explicit(YArgs a, int rk) {
       T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
     }
   }
-  yf_commit(T, pf, N);                     // phi
-  yf_issue(pf, a.rhv, P, L0, nv, N);
+  yf_commit(T, pfb, N);                    // phi
+  if (rk > 1) yf_issue(pfb, a.phiwk, P, L0, nv, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -386,8 +388,7 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-  yf_commit(T, pf, N);                     // rhv
-  if (rk > 1) yf_issue(pf, a.phiwk, P, L0, nv, N);
+  yf_commit(T, pfa, N);                    // rhv
   {
     YPh<false> ph(pho, lane);
 #pragma unroll
@@ -397,7 +398,7 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hv
     }
   }
-  if (rk > 1) yf_commit(T, pf, N);        // phiwk
+  if (rk > 1) yf_commit(T, pfb, N);       // phiwk
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
