@@ -547,8 +547,12 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->rhg, NC));
   TRY(A(&h->rhv, NC));
   TRY(A(&h->s3, NS));
-  TRY(A(&h->vorwk, NC));
-  TRY(A(&h->phiwk, NC));
+  // the spectral y path keeps the accumulators block-major: kxn * nch
+  // blocks of YF_LINES lines (nch = ceil(mz / YF_LINES)), at least NC
+  const size_t NWK = std::max((size_t)NC, (size_t)g.kxn * ((g.mz + YF_LINES - 1) / YF_LINES) *
+                                              YF_LINES * (size_t)my);
+  TRY(A(&h->vorwk, NWK));
+  TRY(A(&h->phiwk, NWK));
   TRY(A(&h->s1, NS));
   TRY(A(&h->s2, NS));
   const size_t NH = (size_t)my * g.kxn * mgalz;

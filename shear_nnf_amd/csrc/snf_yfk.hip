@@ -110,6 +110,27 @@ __device__ __forceinline__ void yf_commit(cplx* T, const cplx (&r)[Q], int N) {
   __syncthreads();
 }
 
+// The RK accumulators vorwk, phiwk are private to this path and kept
+// block-major: block lb owns the contiguous chunk W + lb N YF_LINES, in the
+// element order of the tile transfers (e = j YF_LINES + c).  The chunk moves
+// as one 24.6 KB run at my = 385, instead of my rows of 64 bytes.
+template <int Q>
+__device__ __forceinline__ void yf_issue_blk(cplx (&r)[Q], const cplx* W, int N) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = threadIdx.x + q * 64 * YF_LINES;
+    r[q] = e < N * YF_LINES ? gld(W + e) : mkc(0.0, 0.0);
+  }
+}
+__device__ __forceinline__ void yf_flush_blk(const cplx* T, cplx* W, int N) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    gst(W + e, T[TileCol{c}(j)]);
+  }
+  __syncthreads();
+}
+
 // exp(i w_m) of mode m: exp(2 pi i m/N) exp(i theta/N)
 __device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
   return cmul(cconj(gld(rt + m)), eth);
@@ -223,6 +244,8 @@ k_yf_explicit(YArgs a, int rk) {
   const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
   const long L0 = (long)i * g.mz + YF_LINES * ch;
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
+  cplx* vwk_blk = a.vorwk + lb * (long)N * YF_LINES;    // block-major accumulators
+  cplx* pwk_blk = a.phiwk + lb * (long)N * YF_LINES;
   const int k = min(YF_LINES * ch + w, g.mz - 1);
   const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
   const double k2 = kx * kx + kzv * kzv;
@@ -253,7 +276,7 @@ k_yf_explicit(YArgs a, int rk) {
   cplx pfa[Q], pfb[Q];
   yf_fill(T, a.vor, P, L0, nv, N);
   yf_issue(pfa, a.rhg, P, L0, nv, N);
-  if (rk > 1) yf_issue(pfb, a.vorwk, P, L0, nv, N);
+  if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -307,7 +330,7 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.vorwk, P, L0, nv, N);
+  yf_flush_blk(T, vwk_blk, N);
   {
     YPh<true> ph(phn, lane);
 #pragma unroll
@@ -364,7 +387,7 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_commit(T, pfb, N);                    // phi
-  if (rk > 1) yf_issue(pfb, a.phiwk, P, L0, nv, N);
+  if (rk > 1) yf_issue_blk(pfb, pwk_blk, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -416,7 +439,7 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.phiwk, P, L0, nv, N);
+  yf_flush_blk(T, pwk_blk, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
