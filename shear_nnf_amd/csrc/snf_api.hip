@@ -1049,7 +1049,8 @@ int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const doubl
 }
 
 int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n) {
-  if (!h || !n) return fail(SNF_BAD_PARAM, "null argument");
+  if (!h || !n || max_records < 0 || (!out && max_records > 0))
+    return fail(SNF_BAD_PARAM, "bad argument");
   HIPCHK(hipSetDevice(h->dev));
   HIPCHK(hipStreamSynchronize(h->stream));
   DevScalars s{};
