@@ -35,8 +35,9 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="precision of the main timed leg (mixed: no separate mixed leg)")
-    ap.add_argument("--cmp-steps", type=int, default=500,
-                    help="steps of the FP64-vs-mixed statistics comparison")
+    ap.add_argument("--cmp-steps", type=int, default=2000,
+                    help="steps of the FP64-vs-mixed statistics comparison (2000: the "
+                         "length the mixed-precision workload is specified over)")
     ap.add_argument("--cpu-grid", type=str, default="",
                     help="grid of the CPU baseline sample (default: the workload grid)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
