@@ -205,8 +205,8 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU (sharded) path "
-              "is design-only (see DESIGN.md)", file=sys.stderr)
+        print("bench.py: one GPU only; the sharded multi-GPU path is designed, not built "
+              "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import shear_nnf_amd as snf
