@@ -685,7 +685,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   }
   if (!b_fits(h->bthreads, h->zb)) {
     snf_destroy(h);
-    return fail(SNF_BAD_PARAM, "x-z transform sizes exceed the LDS plan of this build");
+    return fail(SNF_BAD_PARAM, "x-z transform sizes exceed the LDS plan of the library");
   }
   {
     auto wave_size = [](int n) {
