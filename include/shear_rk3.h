@@ -64,7 +64,7 @@ typedef struct snf_params {
                                  FP32 x-z transforms, hybrid planes and
                                  u x omega; FP64 state, KMM reconstruction,
                                  y operators, RK update, statistics sums;
-                                 needs mgalx, mgalz in 64*{2,3,4,6,8} */
+                                 any grid (wave or Stockham x-z kernels) */
   int32_t device;             /* HIP device ordinal                       */
   int32_t ydealias;           /* iydealiasing (cross.f90:459-491), applied
                                  to vor, phi, U, W every substep: 0 none

@@ -303,6 +303,35 @@ void launch_xzw_any(snf_handle* h, int which, int rk) {
     launch_xzw<cplx>(h, which, rk);
 }
 
+// Stockham x-z kernels (sizes without a wave plan); which: 0 = A, 1 = B, 2 = C
+template <class H>
+void launch_xz_generic(snf_handle* h, int which, int rk) {
+  const Geom& g = h->g;
+  dim3 gA(g.kxn, g.my);
+  H* hin = reinterpret_cast<H*>(h->hyb_in);
+  H* hout = reinterpret_cast<H*>(h->hyb_out);
+  if (which == 0) {
+    k_xz_A<H><<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
+                                               h->w00r, h->dudy, h->dwdy, h->p.vbulk, hin, h->pz,
+                                               h->twz, h->sc);
+  } else if (which == 1) {
+    dim3 gB(g.mgalz / h->zb, g.my);
+    auto kb = h->bthreads == 256 ? k_xz_B<256, H>
+                                 : (h->bthreads == 512 ? k_xz_B<512, H> : k_xz_B<1024, H>);
+    kb<<<gB, h->bthreads, h->ldsB, h->stream>>>(g, hin, hout, h->px, h->twx, rk, h->umax_acc,
+                                                 h->phist, h->sc, h->p.nhist, h->zb);
+  } else {
+    k_xz_C<H><<<gA, 256, h->ldsC, h->stream>>>(g, hout, h->v, h->rdome, h->rhg, h->rhv, h->rf0u,
+                                               h->rf0w, h->pz, h->twz, h->sc);
+  }
+}
+void launch_xz_generic_any(snf_handle* h, int which, int rk) {
+  if (h->mixed)
+    launch_xz_generic<cplxf>(h, which, rk);
+  else
+    launch_xz_generic<cplx>(h, which, rk);
+}
+
 int launch_xz(snf_handle* h, int rk) {
   const Geom& g = h->g;
   // the x-z stage reads U, W as real arrays
@@ -311,11 +340,8 @@ int launch_xz(snf_handle* h, int rk) {
     TimedLaunch tl(h, SNF_K_XZ_A);
     launch_xzw_any(h, 0, rk);
   } else {
-    dim3 gA(g.kxn, g.my);
     TimedLaunch tl(h, SNF_K_XZ_A);
-    k_xz_A<<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
-                                            h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->hyb_in, h->pz,
-                                            h->twz, h->sc);
+    launch_xz_generic_any(h, 0, rk);
   }
   if (rk == 1) {
     k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
@@ -331,16 +357,10 @@ int launch_xz(snf_handle* h, int rk) {
     { TimedLaunch tl(h, SNF_K_XZ_C);
       launch_xzw_any(h, 2, rk); }
   } else {
-    dim3 gA(g.kxn, g.my);
-    dim3 gB(g.mgalz / h->zb, g.my);
     { TimedLaunch tl(h, SNF_K_XZ_B);
-      auto kb = h->bthreads == 256 ? k_xz_B<256> : (h->bthreads == 512 ? k_xz_B<512> : k_xz_B<1024>);
-      kb<<<gB, h->bthreads, h->ldsB, h->stream>>>(g, h->hyb_in, h->hyb_out, h->px, h->twx, rk,
-                                                   h->umax_acc, h->phist, h->sc, h->p.nhist,
-                                                   h->zb); }
+      launch_xz_generic_any(h, 1, rk); }
     { TimedLaunch tl(h, SNF_K_XZ_C);
-      k_xz_C<<<gA, 256, h->ldsC, h->stream>>>(g, h->hyb_out, h->v, h->rdome, h->rhg, h->rhv,
-                                              h->rf0u, h->rf0w, h->pz, h->twz, h->sc); }
+      launch_xz_generic_any(h, 2, rk); }
   }
   HIPCHK(hipGetLastError());
   return SNF_OK;
@@ -693,10 +713,6 @@ int snf_create(const snf_params* p, snf_handle** out) {
     };
     const char* e = getenv("SNF_XZ_PATH");
     h->wave_xz = wave_size(mgalx) && wave_size(mgalz) && !(e && std::string(e) == "generic");
-    if (h->mixed && !h->wave_xz) {
-      snf_destroy(h);
-      return fail(SNF_BAD_PARAM, "SNF_PREC_MIXED needs mgalx, mgalz in {128, 192, 256, 384, 512}");
-    }
     if (h->wave_xz) {
       h->rx = mgalx / 64;
       h->rz = mgalz / 64;
@@ -726,15 +742,21 @@ int snf_create(const snf_params* p, snf_handle** out) {
       }
     }
   }
-  h->ldsA = 2 * 3 * (size_t)mgalz * sizeof(cplx);   // ping-pong, 3 sequences
-  h->ldsB = 3 * (size_t)h->zb * mgalx * sizeof(cplx);
-  h->ldsC = 2 * 3 * (size_t)mgalz * sizeof(cplx);
+  // Stockham x-z kernels: elements of es bytes (FP32 in the mixed mode)
+  const size_t es = h->mixed ? sizeof(cplxf) : sizeof(cplx);
+  h->ldsA = 2 * 3 * (size_t)mgalz * es;   // ping-pong, 3 sequences
+  h->ldsB = 3 * (size_t)h->zb * mgalx * es;
+  h->ldsC = 2 * 3 * (size_t)mgalz * es;
   (void)hipFuncSetAttribute((const void*)k_ydealias, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(3 * (size_t)my * sizeof(cplx)));
-  (void)hipFuncSetAttribute((const void*)k_xz_A, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
-  for (const void* kb : {(const void*)k_xz_B<256>, (const void*)k_xz_B<512>, (const void*)k_xz_B<1024>})
+  for (const void* k : {(const void*)k_xz_A<cplx>, (const void*)k_xz_A<cplxf>})
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
+  for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
+                         (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
+                         (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>})
     (void)hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
-  (void)hipFuncSetAttribute((const void*)k_xz_C, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
+  for (const void* k : {(const void*)k_xz_C<cplx>, (const void*)k_xz_C<cplxf>})
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
 #undef TRY
   *out = h;
   return SNF_OK;

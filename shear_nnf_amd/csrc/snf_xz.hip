@@ -93,23 +93,27 @@ __device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i,
 #define XZ_B_MAXW 2      // (row pair, x) product items per thread
 
 
+// H: element type of the transforms and hybrid planes (cplx, or cplxf in the
+// mixed precision mode); the spectral fields and the reconstruction are FP64
+template <class H>
 __global__ void __launch_bounds__(256)
 k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
        const cplx* __restrict__ hg, const cplx* __restrict__ hv,
        const cplx* __restrict__ dvordy, const double* __restrict__ u00,
        const double* __restrict__ w00, const double* __restrict__ dudy,
-       const double* __restrict__ dwdy, double vbulk, cplx* __restrict__ hyb, FFTPlan pz,
+       const double* __restrict__ dwdy, double vbulk, H* __restrict__ hyb, FFTPlan pz,
        const cplx* __restrict__ twz, const DevScalars* sc) {
+  typedef typename wf_real<H>::T R;
   if (sc->halted) return;
-  extern __shared__ cplx lds[];
+  extern __shared__ double4 xz_smem[];
   const int Nz = g.mgalz;
-  cplx* a = lds;
-  cplx* b = lds + 3 * Nz;
+  H* a = reinterpret_cast<H*>(xz_smem);
+  H* b = a + 3 * Nz;
   const int i = blockIdx.x, j = blockIdx.y;
   const double u00j = u00[j], w00j = w00[j], dudyj = dudy[j], dwdyj = dwdy[j];
   const long fstride = (long)g.my * g.kxn * Nz;
   for (int grp = 0; grp < 2; ++grp) {
-    for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) a[t] = mkc(0.0, 0.0);
+    for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) a[t] = wf_mk((R)0, (R)0);
     __syncthreads();
     for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
       PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
@@ -117,10 +121,10 @@ k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
       derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, d);
       const int pos = kz_pos(k, g.nz, g.mz, Nz);
 #pragma unroll
-      for (int f = 0; f < 3; ++f) a[f * Nz + pos] = d[grp * 3 + f];
+      for (int f = 0; f < 3; ++f) a[f * Nz + pos] = to_h<H>(d[grp * 3 + f]);
     }
     __syncthreads();
-    cplx* r = fft_lds(a, b, 3, pz, twz, +1);
+    H* r = fft_lds(a, b, 3, pz, twz, +1);
     for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
       const int f = t / Nz, z = t - f * Nz;
       hyb[(grp * 3 + f) * fstride + ((long)j * g.kxn + i) * Nz + z] = r[t];
@@ -139,13 +143,15 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
 }
 
 // NT threads per block (256, 512 or 1024), ZB z-rows per block
-template <int NT>
+template <int NT, class H>
 __global__ void __launch_bounds__(NT)
-k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px,
+k_xz_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout, FFTPlan px,
        const cplx* __restrict__ twx, int rk, unsigned long long* umax_acc,
        double* __restrict__ phist, const DevScalars* sc, int nhist, int ZB) {
+  typedef typename wf_real<H>::T R;
   if (sc->halted) return;
-  extern __shared__ cplx lds[];
+  extern __shared__ double4 xz_smem[];
+  H* lds = reinterpret_cast<H*>(xz_smem);
   const int Mx = g.mgalx, Nz = g.mgalz, kxn = g.kxn;
   // neighbouring z-blocks of one plane share 128-byte lines of the hybrid
   // field: keep them on one XCD
@@ -154,7 +160,7 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
   const long fstride = (long)g.my * kxn * Nz;
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
   const int nseq = 3 * ZB;    // sequence s = p*ZB + r (pair p, row r)
-  for (int t = threadIdx.x; t < nseq * Mx; t += blockDim.x) lds[t] = mkc(0.0, 0.0);
+  for (int t = threadIdx.x; t < nseq * Mx; t += blockDim.x) lds[t] = wf_mk((R)0, (R)0);
   __syncthreads();
   // Z_pair = G_a + i G_b with the Hermitian extension in kx
   for (int t = threadIdx.x; t < 3 * kxn * ZB; t += blockDim.x) {
@@ -162,28 +168,29 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
     const int pi_ = t / ZB;
     const int i = pi_ % kxn, p = pi_ / kxn;
     const long off = ((long)j * kxn + i) * Nz + z0 + r;
-    const cplx Ga = hin[(2 * p) * fstride + off];
-    const cplx Gb = hin[(2 * p + 1) * fstride + off];
-    cplx* s = lds + (p * ZB + r) * Mx;
+    const H Ga = hin[(2 * p) * fstride + off];
+    const H Gb = hin[(2 * p + 1) * fstride + off];
+    H* s = lds + (p * ZB + r) * Mx;
     if (i == 0) {
-      s[0] = mkc(Ga.x, Gb.x);
+      s[0] = wf_mk(Ga.x, Gb.x);
     } else {
-      s[i] = mkc(Ga.x - Gb.y, Ga.y + Gb.x);
-      s[Mx - i] = mkc(Ga.x + Gb.y, Gb.x - Ga.y);
+      s[i] = wf_mk(Ga.x - Gb.y, Ga.y + Gb.x);
+      s[Mx - i] = wf_mk(Ga.x + Gb.y, Gb.x - Ga.y);
     }
   }
   __syncthreads();
   fft_lds_ip<XZ_MAXV>(lds, nseq, px, twx, +1);
   // u x omega: each (row pair, x) handled by one thread; results go to
   // sequences 0..ZB-1 (H1 + i H2 per row) and ZB..3ZB/2-1 (H3 of two rows)
-  const double c2 = 2.0 * g.chi;
-  const double sy = g.s * ((double)j * g.dy - 0.5 * g.Ly);
-  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  // products and maxima in the element type, the 'p' sums in FP64
+  const R c2 = (R)(2.0 * g.chi);
+  const R sy = (R)(g.s * ((double)j * g.dy - 0.5 * g.Ly));
+  R m0 = 0, m1 = 0, m2 = 0, m3 = 0;
   double p0 = 0.0, p1 = 0.0, p2 = 0.0;
   const int npair = ZB / 2;
   const int nwork = npair * Mx;
   constexpr int MAXW = XZ_B_MAXW;   // host-checked
-  double h1[MAXW][2], h2[MAXW][2], h3[MAXW][2];
+  R h1[MAXW][2], h2[MAXW][2], h3[MAXW][2];
 #pragma unroll
   for (int q = 0; q < MAXW; ++q) {
     const int t = threadIdx.x + q * blockDim.x;
@@ -192,10 +199,10 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int r = 2 * rp + h;
-        const cplx z0v = lds[(0 * ZB + r) * Mx + x];
-        const cplx z1v = lds[(1 * ZB + r) * Mx + x];
-        const cplx z2v = lds[(2 * ZB + r) * Mx + x];
-        const double u = z0v.x, v = z0v.y, w = z1v.x, o1 = z1v.y, o2 = z2v.x, o3 = z2v.y;
+        const H z0v = lds[(0 * ZB + r) * Mx + x];
+        const H z1v = lds[(1 * ZB + r) * Mx + x];
+        const H z2v = lds[(2 * ZB + r) * Mx + x];
+        const R u = z0v.x, v = z0v.y, w = z1v.x, o1 = z1v.y, o2 = z2v.x, o3 = z2v.y;
         if (rk == 1) {
           m0 = fmax(m0, fabs(u));
           m1 = fmax(m1, fabs(u + sy));
@@ -203,9 +210,9 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
           m3 = fmax(m3, fabs(w));
         }
         if (ihist) {
-          p0 += v * (u * u);
-          p1 += v * (w * w);
-          p2 += v * v * v;
+          p0 += (double)(v * (u * u));
+          p1 += (double)(v * (w * w));
+          p2 += (double)(v * v * v);
         }
         h1[q][h] = v * (o3 + c2) - w * o2;
         h2[q][h] = w * o1 - u * (o3 + c2);
@@ -219,9 +226,9 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
     const int t = threadIdx.x + q * blockDim.x;
     if (t < nwork) {
       const int rp = t / Mx, x = t - rp * Mx;
-      lds[(2 * rp) * Mx + x] = mkc(h1[q][0], h2[q][0]);
-      lds[(2 * rp + 1) * Mx + x] = mkc(h1[q][1], h2[q][1]);
-      lds[(ZB + rp) * Mx + x] = mkc(h3[q][0], h3[q][1]);
+      lds[(2 * rp) * Mx + x] = wf_mk(h1[q][0], h2[q][0]);
+      lds[(2 * rp + 1) * Mx + x] = wf_mk(h1[q][1], h2[q][1]);
+      lds[(ZB + rp) * Mx + x] = wf_mk(h3[q][0], h3[q][1]);
     }
   }
   // block reductions: wave shuffles, then one value per wave (fixed order)
@@ -235,10 +242,10 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
       m3 = fmax(m3, __shfl_xor(m3, o, 64));
     }
     if (lane == 0) {
-      red[0][wv] = m0;
-      red[1][wv] = m1;
-      red[2][wv] = m2;
-      red[3][wv] = m3;
+      red[0][wv] = (double)m0;
+      red[1][wv] = (double)m1;
+      red[2][wv] = (double)m2;
+      red[3][wv] = (double)m3;
     }
     __syncthreads();
     if (threadIdx.x < 4) {
@@ -272,13 +279,14 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
   for (int t = threadIdx.x; t < kxn * ZB; t += blockDim.x) {
     const int r = t % ZB, k = t / ZB;
     const int km = (k == 0) ? 0 : Mx - k;
-    const cplx A = lds[r * Mx + k], Am = lds[r * Mx + km];
-    const cplx* Bs = lds + (ZB + r / 2) * Mx;
-    const cplx B = Bs[k], Bm = Bs[km];
-    const cplx H1 = mkc(0.5 * (A.x + Am.x), 0.5 * (A.y - Am.y));
-    const cplx H2 = mkc(0.5 * (A.y + Am.y), -0.5 * (A.x - Am.x));
-    const cplx H3 = (r % 2 == 0) ? mkc(0.5 * (B.x + Bm.x), 0.5 * (B.y - Bm.y))
-                                 : mkc(0.5 * (B.y + Bm.y), -0.5 * (B.x - Bm.x));
+    const H A = lds[r * Mx + k], Am = lds[r * Mx + km];
+    const H* Bs = lds + (ZB + r / 2) * Mx;
+    const H B = Bs[k], Bm = Bs[km];
+    const R hf = (R)0.5;
+    const H H1 = wf_mk(hf * (A.x + Am.x), hf * (A.y - Am.y));
+    const H H2 = wf_mk(hf * (A.y + Am.y), -hf * (A.x - Am.x));
+    const H H3 = (r % 2 == 0) ? wf_mk(hf * (B.x + Bm.x), hf * (B.y - Bm.y))
+                              : wf_mk(hf * (B.y + Bm.y), -hf * (B.x - Bm.x));
     const long off = ((long)j * kxn + k) * Nz + z0 + r;
     hout[off] = H1;
     hout[fstride + off] = H2;
@@ -290,17 +298,18 @@ k_xz_B(Geom g, const cplx* __restrict__ hin, cplx* __restrict__ hout, FFTPlan px
 // C: hybrid -> spectral (forward z DFT), output combinations (iadd_mode=5,
 // cross.f90:1533-1544) and the 00-mode terms rf0u, rf0w (cross.f90:1489-1490)
 // ---------------------------------------------------------------------------
+template <class H>
 __global__ void __launch_bounds__(256)
-k_xz_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
+k_xz_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
        cplx* __restrict__ dvordy, cplx* __restrict__ hg, cplx* __restrict__ hv,
        double* __restrict__ rf0u,
        double* __restrict__ rf0w, FFTPlan pz, const cplx* __restrict__ twz,
        const DevScalars* sc) {
   if (sc->halted) return;
-  extern __shared__ cplx lds[];
+  extern __shared__ double4 xz_smem[];
   const int Nz = g.mgalz;
-  cplx* a = lds;
-  cplx* b = lds + 3 * Nz;
+  H* a = reinterpret_cast<H*>(xz_smem);
+  H* b = a + 3 * Nz;
   const int i = blockIdx.x, j = blockIdx.y;
   const long fstride = (long)g.my * g.kxn * Nz;
   for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
@@ -308,16 +317,16 @@ k_xz_C(Geom g, const cplx* __restrict__ hout, const cplx* __restrict__ vel,
     a[t] = hout[f * fstride + ((long)j * g.kxn + i) * Nz + z];
   }
   __syncthreads();
-  cplx* r = fft_lds(a, b, 3, pz, twz, -1);
+  H* r = fft_lds(a, b, 3, pz, twz, -1);
   const double norm = 1.0 / ((double)g.mgalx * (double)g.mgalz);
   const double kx = g.alp * (double)i;
   const double kx2 = kx * kx;
   const long base = (long)j * g.P + (long)i * g.mz;
   for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
     const int pos = kz_pos(k, g.nz, g.mz, Nz);
-    const cplx h1 = crmul(norm, r[pos]);
-    const cplx h2 = crmul(norm, r[Nz + pos]);
-    const cplx h3 = crmul(norm, r[2 * Nz + pos]);
+    const cplx h1 = crmul(norm, to_d(r[pos]));
+    const cplx h2 = crmul(norm, to_d(r[Nz + pos]));
+    const cplx h3 = crmul(norm, to_d(r[2 * Nz + pos]));
     const double kz = kz_of(k, g.nz, g.mz, g.gam);
     const long idx = base + k;
     const cplx v = vel[idx];

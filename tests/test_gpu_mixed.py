@@ -83,7 +83,29 @@ def test_mixed_kelvin_mode():
     assert e[0] < 1e-6 and e[1] < 1e-6 and e[2] < 1e-6
 
 
-def test_mixed_needs_wave_sizes():
-    import shear_nnf_amd as snf
-    with pytest.raises(snf.SnfError):
-        snf.ShearBox(48, 49, 48, 1000.0, alp=ALP, gam=GAM, Ly=LY, precision="mixed")
+# sizes without a wave plan take the Stockham x-z kernels, FP32 there too:
+# the ensemble sizes n^3, n x (n+1) x n (spectral and banded y), explicit and
+# implicit, and a Newton-search box
+@pytest.mark.parametrize("grid,explicit", [((48, 49, 48), 1), ((32, 33, 32), 1),
+                                           ((64, 65, 64), 1), ((96, 96, 96), 0)])
+def test_mixed_stockham_sizes_agree_with_fp64(grid, explicit):
+    f64, h64, st64 = _run(grid, "fp64", 10, explicit=explicit)
+    fmx, hmx, stmx = _run(grid, "mixed", 10, explicit=explicit)
+    assert stmx.steps_done == st64.steps_done == 10
+    for a, b in zip(fmx, f64):
+        assert 0.0 < rel_l2(a, b) < MIX_TOL
+    s64, smx = _stats(h64), _stats(hmx)
+    for k in s64:
+        assert np.max(np.abs(smx[k] - s64[k]) / np.abs(s64[k])) < MIX_TOL, k
+
+
+def test_mixed_stockham_agrees_with_oracle():
+    from oracle.cross import Cross
+    from oracle.grid import Grid
+    grid = (48, 49, 48)
+    (vor, phi, u00, w00), _, _ = _run(grid, "mixed", 3)
+    c = Cross(Grid(*grid, Re=1000.0, alp=ALP, gam=GAM, Ly=LY, s=1.0, CFL=0.5))
+    c.set_state(*box_ic(*grid))
+    c.run(3)
+    for a, b in zip((vor, phi, u00, w00), (c.vor, c.phi, c.u00, c.w00)):
+        assert rel_l2(a, b) < MIX_TOL
