@@ -175,20 +175,20 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
         specs.append(((n, my, n), float(rng.uniform(500.0, 3000.0)), seed + 1 + q))
     ics = [make_ic(*g, math.pi, 2 * math.pi, 1.0, seed=sd) for g, _, sd in specs]
 
-    def boxes():
+    def boxes(precision):
         out = []
         for (g, Re, _), ic in zip(specs, ics):
-            b = snf.ShearBox(*g, Re, nhist=10 ** 9)
+            b = snf.ShearBox(*g, Re, nhist=10 ** 9, precision=precision)
             b.set_state(*ic)
             out.append(b)
         return out
 
     rates = {}
-    for mode in ("sequential", "batched"):
-        bs = boxes()
+    for mode, prec in (("sequential", "fp64"), ("batched", "fp64"), ("batched_mixed", "mixed")):
+        bs = boxes(prec)
         snf.advance_batch(bs, 2)                        # warm-up
         t0 = time.perf_counter()
-        if mode == "batched":
+        if mode.startswith("batched"):
             st = snf.advance_batch(bs, steps)
         else:
             st = [b.advance_async_status(steps) for b in bs]
@@ -197,9 +197,11 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
         for b in bs:
             b.close()
     return {"boxes": nbox, "steps": steps, "grids": sorted({"%dx%dx%d" % g for g, _, _ in specs}),
-            "unit": "box-steps/s", "dtype": "f64",
+            "unit": "box-steps/s", "dtype": "f64 (batched_mixed: the mixed precision mode)",
             "batched": rates["batched"], "sequential": rates["sequential"],
-            "speedup": rates["batched"] / rates["sequential"]}
+            "speedup": rates["batched"] / rates["sequential"],
+            "batched_mixed": rates["batched_mixed"],
+            "mixed_vs_fp64_batched": rates["batched_mixed"] / rates["batched"]}
 
 
 def main():
