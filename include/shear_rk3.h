@@ -140,7 +140,11 @@ int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double x
 /* forward-difference Jacobian-vector products of the shooting map
  * (matvec_dns_disk3d with iopt_matvec=1, gmres_shear.f90:2666-2730):
  * out[d] = (Phi_T(x0 + eps[d] dirs[d]) - y0) / eps[d] for ndir directions
- * (state vectors, contiguous); y0 = Phi_T(x0), computed here if null.   */
+ * (state vectors, contiguous); y0 = Phi_T(x0), computed here if null.
+ * The directions shoot concurrently: h and up to 15 copies of it (made on
+ * first need while half the free device memory holds them, kept until
+ * snf_destroy(h)), each bitwise the same as a separate snf_shoot.  h is left
+ * holding the state of one of the shootings.                             */
 int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
                   int32_t ndir, const double* eps, double t0, double xoffb0, double xofft0,
                   double T, double* out);

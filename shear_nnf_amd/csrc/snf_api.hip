@@ -116,6 +116,12 @@ struct snf_handle {
   bool mixed = false;     // SNF_PREC_MIXED: FP32 x-z transforms and hybrid planes
   int rx = 0, rz = 0;
   size_t ldsWA = 0, ldsWB = 0, ldsWC = 0;
+  size_t dev_bytes = 0;   // device memory the handle allocated
+  std::vector<snf_handle*> clones;   // copies that shoot JVP directions (kept)
+  // two RK3 steps captured as a HIP graph, per parity of cur at their start
+  // (every substep flips cur, so two steps return to the same parity)
+  bool graphs = true;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   double *spec = nullptr, *spec2 = nullptr;   // spectra sp, sp2: [SNF_NSPEC][P]
   double* upart = nullptr;   // umax accumulators of the wave x-row kernel ([UMAX_SLOTS][4])
   // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
@@ -547,7 +553,10 @@ int snf_create(const snf_params* p, snf_handle** out) {
   h->id1 = identity_tail(f1, my);
   h->id2 = identity_tail(f2, my);
   int rc;
-  auto A = [&](auto** ptr, size_t n) { return dalloc(ptr, n); };
+  auto A = [&](auto** ptr, size_t n) {
+    h->dev_bytes += n * sizeof(**ptr) + 256;
+    return dalloc(ptr, n);
+  };
 #define TRY(x)                   \
   do {                           \
     rc = (x);                    \
@@ -713,6 +722,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
     };
     const char* e = getenv("SNF_XZ_PATH");
     h->wave_xz = wave_size(mgalx) && wave_size(mgalz) && !(e && std::string(e) == "generic");
+    const char* eg = getenv("SNF_GRAPHS");
+    h->graphs = !(eg && std::string(eg) == "0");
     if (h->wave_xz) {
       h->rx = mgalx / 64;
       h->rz = mgalz / 64;
@@ -764,7 +775,10 @@ int snf_create(const snf_params* p, snf_handle** out) {
 
 int snf_destroy(snf_handle* h) {
   if (!h) return SNF_OK;
+  for (snf_handle* c : h->clones) snf_destroy(c);
   (void)hipSetDevice(h->dev);   // frees proceed whatever the device state
+  for (hipGraphExec_t& ge : h->gexec)
+    if (ge) (void)hipGraphExecDestroy(ge);
   void* ptrs[] = {h->vor, h->phi, h->v, h->dome, h->dv, h->rdome, h->rhg, h->rhv, h->s3,
                   h->vorwk, h->phiwk, h->s1, h->s2,
                   h->hyb_in, h->hyb_out, h->hr, h->hc, h->u00c, h->w00c, h->u00wk, h->w00wk,
@@ -915,6 +929,8 @@ static int advance_begin(snf_handle* h, bool continue_call, AdvanceMark& m) {
   return SNF_OK;
 }
 
+static int enqueue_steps(snf_handle* h, int nstep);
+
 static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
@@ -928,6 +944,38 @@ static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
   if (h->p.ydealias == 3 && !resume) launch_yfilter(h);
   k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
   HIPCHK(hipGetLastError());
+  // steps in pairs from captured graphs (one launch per pair instead of about
+  // fifty), the remainder launched one by one; with per-kernel timing on,
+  // every step is launched one by one
+  int left = nstep;
+  while (h->graphs && !h->timing && left >= 2) {
+    hipGraphExec_t& ge = h->gexec[h->cur];
+    if (!ge) {
+      const int cur0 = h->cur;
+      hipGraph_t gr = nullptr;
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_steps(h, 2);
+      const hipError_t ec = hipStreamEndCapture(h->stream, &gr);
+      h->cur = cur0;
+      if (rc || ec != hipSuccess || !gr || hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        (void)hipGetLastError();
+        ge = nullptr;
+        h->graphs = false;   // launch one by one from here on
+        break;
+      }
+      (void)hipGraphDestroy(gr);
+    }
+    HIPCHK(hipGraphLaunch(ge, h->stream));
+    left -= 2;
+  }
+  return enqueue_steps(h, left);
+}
+
+// nstep RK3 steps launched kernel by kernel (cross.f90:336-1029)
+static int enqueue_steps(snf_handle* h, int nstep) {
+  const Geom& g = h->g;
+  int rc;
   for (int step = 0; step < nstep; ++step) {
     k_step_begin<<<1, 1, 0, h->stream>>>(h->sc, h->p.ydealias == 2 ? 2.0 * M_PI / g.alp : 0.0);
     for (int rk = 1; rk <= 3; ++rk) {
@@ -1018,54 +1066,123 @@ int snf_state_len(snf_handle* h, int64_t* n) {
   return SNF_OK;
 }
 
+// calc_dns_disk3d (gmres_shear.f90:2951-3120) for n handles at once: each
+// starts a new run from its state x[q] at (t0, xoffb0, xofft0) with its
+// statistics restarted and stops at etime = t0 + T (cross.f90:1680-1712).
+// The cross calls run in chunks of steps that continue them; every chunk of
+// every unfinished handle is in flight before any is waited for.
+static int shoot_many(snf_handle* const* hs, int n, const double* const* x, double t0,
+                      double xoffb0, double xofft0, double T, double* const* y,
+                      snf_run_status* st) {
+  int rc;
+  for (int q = 0; q < n; ++q) {
+    snf_handle* h = hs[q];
+    const size_t nc2 = 2 * (size_t)h->NC;
+    const int my = h->g.my;
+    if ((rc = snf_state_upload_ykxkz(h, x[q], x[q] + nc2, x[q] + 2 * nc2, x[q] + 2 * nc2 + my, t0,
+                                     xoffb0, xofft0)))
+      return rc;
+    if ((rc = snf_reset_statistics(h))) return rc;
+    DevScalars s{};
+    HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+    s.etime = t0 + T;
+    s.dumptime = 1e300;
+    s.istop = s.idump = s.halted = 0;
+    HIPCHK(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice));
+  }
+  const int chunk = 256;
+  std::vector<char> live((size_t)n, 1);
+  std::vector<AdvanceMark> m((size_t)n);
+  std::vector<snf_run_status> r((size_t)n);
+  for (int c = 0, left = n; left > 0; ++c) {
+    if (c > (1 << 22) / chunk) return fail(SNF_DEVICE_ERROR, "shooting did not reach t0 + T");
+    // the begins read device scalars with blocking copies: all of them
+    // before any handle's steps are in flight
+    for (int q = 0; q < n; ++q)
+      if (live[q] && (rc = advance_begin(hs[q], c > 0, m[q]))) return rc;
+    for (int q = 0; q < n; ++q)
+      if (live[q] && (rc = advance_enqueue(hs[q], chunk, m[q].resume))) return rc;
+    for (int q = 0; q < n; ++q)
+      if (live[q]) {
+        if ((rc = advance_finish(hs[q], m[q], &r[q]))) return rc;
+        if (r[q].stop == SNF_STOP) {
+          live[q] = 0;
+          --left;
+        }
+      }
+  }
+  for (int q = 0; q < n; ++q) {
+    snf_handle* h = hs[q];
+    const size_t nc2 = 2 * (size_t)h->NC;
+    const int my = h->g.my;
+    if (st) st[q] = r[q];
+    if ((rc = snf_state_download_ykxkz(h, y[q], y[q] + nc2, y[q] + 2 * nc2, y[q] + 2 * nc2 + my)))
+      return rc;
+  }
+  return SNF_OK;
+}
+
 int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double xofft0, double T,
               double* y, snf_run_status* st) {
   if (!h || !x || !y || !(T > 0)) return fail(SNF_BAD_PARAM, "bad argument");
-  const size_t nc2 = 2 * (size_t)h->NC;
-  const int my = h->g.my;
-  int rc = snf_state_upload_ykxkz(h, x, x + nc2, x + 2 * nc2, x + 2 * nc2 + my, t0, xoffb0, xofft0);
-  if (rc) return rc;
-  // calc_dns_disk3d (gmres_shear.f90:2951-3120): statistics restart, one
-  // cross call that stops at etime = t0 + T (cross.f90:1680-1712)
-  if ((rc = snf_reset_statistics(h))) return rc;
-  DevScalars s{};
-  HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
-  s.etime = t0 + T;
-  s.dumptime = 1e300;
-  s.istop = s.idump = s.halted = 0;
-  HIPCHK(hipMemcpy(h->sc, &s, sizeof(s), hipMemcpyHostToDevice));
-  snf_run_status r{};
-  const int chunk = 256;
-  for (int c = 0;; ++c) {
-    if ((rc = advance_impl(h, chunk, &r, c > 0))) return rc;
-    if (r.stop == SNF_STOP) break;
-    if (c > (1 << 22) / chunk) return fail(SNF_DEVICE_ERROR, "shooting did not reach t0 + T");
-  }
-  if (st) *st = r;
-  return snf_state_download_ykxkz(h, y, y + nc2, y + 2 * nc2, y + 2 * nc2 + my);
+  return shoot_many(&h, 1, &x, t0, xoffb0, xofft0, T, &y, st);
 }
+
 
 int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
                   int32_t ndir, const double* eps, double t0, double xoffb0, double xofft0,
                   double T, double* out) {
-  if (!h || !x0 || !dirs || !eps || !out || ndir < 1) return fail(SNF_BAD_PARAM, "bad argument");
+  if (!h || !x0 || !dirs || !eps || !out || ndir < 1 || !(T > 0))
+    return fail(SNF_BAD_PARAM, "bad argument");
+  for (int d = 0; d < ndir; ++d)
+    if (!(eps[d] != 0.0)) return fail(SNF_BAD_PARAM, "eps must be nonzero");
   int64_t n = 0;
   snf_state_len(h, &n);
-  std::vector<double> base((size_t)n), x((size_t)n), y((size_t)n);
+  std::vector<double> base((size_t)n);
   int rc;
   if (y0) {
     std::memcpy(base.data(), y0, (size_t)n * sizeof(double));
   } else if ((rc = snf_shoot(h, x0, t0, xoffb0, xofft0, T, base.data(), nullptr))) {
     return rc;
   }
-  for (int d = 0; d < ndir; ++d) {
-    const double e = eps[d];
-    if (!(e != 0.0)) return fail(SNF_BAD_PARAM, "eps must be nonzero");
-    const double* dd = dirs + (size_t)d * n;
-    for (int64_t q = 0; q < n; ++q) x[q] = x0[q] + e * dd[q];
-    if ((rc = snf_shoot(h, x.data(), t0, xoffb0, xofft0, T, y.data(), nullptr))) return rc;
-    double* o = out + (size_t)d * n;
-    for (int64_t q = 0; q < n; ++q) o[q] = (y[q] - base[q]) / e;
+  // the caller's handle and up to 15 copies of it shoot concurrently; the
+  // copies are made on first need (while half the free device memory holds
+  // them) and kept with the handle for later calls
+  HIPCHK(hipSetDevice(h->dev));
+  size_t freeb = 0, totalb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  const size_t per = std::max<size_t>(h->dev_bytes, 1);
+  const size_t want = std::min<size_t>((size_t)ndir - 1, 15);
+  size_t room = freeb / 2 / per;
+  while (h->clones.size() < want && room > 0) {
+    snf_handle* c = nullptr;
+    if ((rc = snf_create(&h->p, &c))) return rc;
+    h->clones.push_back(c);
+    --room;
+  }
+  std::vector<snf_handle*> hs(1, h);
+  hs.insert(hs.end(), h->clones.begin(), h->clones.begin() + std::min(h->clones.size(), want));
+  const int G = (int)hs.size();
+  std::vector<std::vector<double>> xs((size_t)G, std::vector<double>((size_t)n)),
+      ys((size_t)G, std::vector<double>((size_t)n));
+  for (int d0 = 0; d0 < ndir; d0 += G) {
+    const int g = std::min(G, ndir - d0);
+    std::vector<const double*> xp((size_t)g);
+    std::vector<double*> yp((size_t)g);
+    for (int q = 0; q < g; ++q) {
+      const double e = eps[d0 + q];
+      const double* dd = dirs + (size_t)(d0 + q) * n;
+      for (int64_t k = 0; k < n; ++k) xs[q][k] = x0[k] + e * dd[k];
+      xp[q] = xs[q].data();
+      yp[q] = ys[q].data();
+    }
+    if ((rc = shoot_many(hs.data(), g, xp.data(), t0, xoffb0, xofft0, T, yp.data(), nullptr)))
+      return rc;
+    for (int q = 0; q < g; ++q) {
+      const double e = eps[d0 + q];
+      double* o = out + (size_t)(d0 + q) * n;
+      for (int64_t k = 0; k < n; ++k) o[k] = (ys[q][k] - base[k]) / e;
+    }
   }
   return SNF_OK;
 }

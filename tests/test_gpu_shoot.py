@@ -89,3 +89,20 @@ def test_jvp_batch_equals_separate_shootings():
     # with y0 given, the same result
     out2 = box.jvp_batch(x0, dirs, eps, T, y0=y0)
     assert np.array_equal(out, out2)
+
+
+def test_jvp_batch_groups_of_handles():
+    """20 directions: the caller's handle and 15 copies shoot the first 16
+    together, then 4 more; each product is bitwise that of its own shootings"""
+    grid, T = (48, 49, 48), 0.005
+    box = _box(grid)
+    x0 = box.pack_state(*box_ic(*grid))
+    rng = np.random.default_rng(9)
+    nd = 20
+    dirs = rng.normal(size=(nd, x0.size)) * np.abs(x0).max()
+    eps = 1e-7 * (1.0 + np.arange(nd) / nd)
+    y0, _ = box.shoot(x0, T)
+    out = box.jvp_batch(x0, dirs, eps, T, y0=y0)
+    for q in (0, 15, 16, 19):
+        yq, _ = box.shoot(x0 + eps[q] * dirs[q], T)
+        assert np.array_equal(out[q], (yq - y0) / eps[q])
