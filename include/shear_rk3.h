@@ -43,6 +43,14 @@ enum snf_stop_code {     /* why an advance returned (cross.f90 istop/idump) */
   SNF_DUMP = 2           /* idump: dumptime reached, caller should write     */
 };
 
+enum snf_stop_reason {   /* which istop condition ended the run (the first   */
+  SNF_STOP_NONE = 0,     /* raised in it); the reference raises one flag     */
+  SNF_STOP_ETIME = 1,    /* time reached etime (cross.f90:1699, 1709)        */
+  SNF_STOP_DT = 2,       /* diverged: Deltat < 1e-10 (cross.f90:1682-1690)   */
+  SNF_STOP_NAN = 3,      /* NaN in the UNER record (cross.f90:2102-2105)     */
+  SNF_STOP_LAMINAR = 4   /* v' / Lz < 1e-5, laminarised (cross.f90:2106-2111)*/
+};
+
 /* Parameters of modules ctes/running/bcs (mod.f90) and hre2.dat/hre3.dat
  * (initcr.f90:54-135, 1950-1971) that the time advance reads.            */
 typedef struct snf_params {
@@ -85,6 +93,7 @@ typedef struct snf_run_status {
   double umax[4];             /* umax(0:3) of the last CFL evaluation      */
   double CFLv;
   int32_t nhist_records;      /* UNER records produced by this call        */
+  int32_t stop_reason;        /* snf_stop_reason when stop == SNF_STOP     */
 } snf_run_status;
 
 /* Number of doubles in one UNER record (.cfbin row, cross.f90:2087):

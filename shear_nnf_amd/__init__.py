@@ -59,7 +59,7 @@ class SnfRunStatus(ctypes.Structure):
         ("time", ctypes.c_double), ("Deltat", ctypes.c_double),
         ("xoffb", ctypes.c_double), ("xofft", ctypes.c_double),
         ("umax", ctypes.c_double * 4), ("CFLv", ctypes.c_double),
-        ("nhist_records", ctypes.c_int32),
+        ("nhist_records", ctypes.c_int32), ("stop_reason", ctypes.c_int32),
     ]
 
 

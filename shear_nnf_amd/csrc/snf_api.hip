@@ -1014,6 +1014,7 @@ static int advance_finish(snf_handle* h, const AdvanceMark& m, snf_run_status* s
   if (st) {
     st->steps_done = s.steps_done - m.done0;
     st->stop = s.istop ? SNF_STOP : (s.idump ? SNF_DUMP : SNF_RUNNING);
+    st->stop_reason = s.istop ? s.stop_reason : SNF_STOP_NONE;
     st->time = s.time;
     st->Deltat = s.Deltat;
     st->xoffb = s.xoffb;

@@ -109,6 +109,7 @@ struct DevScalars {
   int nrec;           // UNER records written (ring position)
   int nacum;
   double total_dt;
+  int stop_reason;    // snf_stop_reason of the first istop of the run
 };
 
 // Derived per-plane statistics produced by the hist kernels:

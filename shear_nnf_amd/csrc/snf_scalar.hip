@@ -29,6 +29,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// istop = 1 with the condition that raised it (snf_stop_reason), the first
+// one of the run kept
+__device__ __forceinline__ void raise_stop(DevScalars* sc, int reason) {
+  if (!sc->istop) sc->stop_reason = reason;
+  sc->istop = 1;
+}
+
 __global__ void __launch_bounds__(64)
 k_twist(int N, BaseFactor b1, BaseFactor b2, YCoef c, const cplx* __restrict__ shp_arr,
         TwistTail* __restrict__ out, int nk, const DevScalars* sc) {
@@ -202,17 +209,17 @@ k_scalar(ScalarArgs a, int rk) {
       if (sc->CFLv > a.CFL && a.explicit_visc == 1) dt = 1.0 / a.cflvv;
       double Deltat = dt;
       double dtr = g.Re / Deltat;
-      if (Deltat < 1e-10) sc->istop = 1;
+      if (Deltat < 1e-10) raise_stop(sc, 2);
       if (sc->time + Deltat > sc->dumptime) {
         Deltat = fabs(sc->dumptime - sc->time);
         dtr = g.Re / Deltat;
-        if (sc->dumptime > sc->etime - 1e-13) sc->istop = 1;
+        if (sc->dumptime > sc->etime - 1e-13) raise_stop(sc, 1);
         sc->dumptime = sc->dumptime + sc->dtimag;
         sc->idump = 1;
       } else if (sc->time + Deltat > sc->etime) {
         Deltat = fabs(sc->etime - sc->time);
         dtr = g.Re / Deltat;
-        sc->istop = 1;
+        raise_stop(sc, 1);
       }
       sc->Deltat = Deltat;
       sc->dtr = dtr;
@@ -248,8 +255,8 @@ k_scalar(ScalarArgs a, int rk) {
       r[19] = sc->umax[0];
       for (int q = 0; q < 4; ++q) r[20 + q] = ener2[q] / my;
       sc->nrec += 1;
-      if (isnan(uner[0])) sc->istop = 1;
-      if (uner[1] / (2.0 * M_PI / g.gam) < 1e-5) sc->istop = 1;
+      if (isnan(uner[0])) raise_stop(sc, 3);
+      if (uner[1] / (2.0 * M_PI / g.gam) < 1e-5) raise_stop(sc, 4);
     }
     sh_dt = sc->Deltat;
   }
@@ -300,6 +307,7 @@ __global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm, int r
     if (!resume) {
       sc->steps_local = 0;
       sc->istop = 0;
+      sc->stop_reason = 0;
     }
     sc->idump = 0;
     sc->xwkb = sc->xoffb;
