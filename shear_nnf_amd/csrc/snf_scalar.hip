@@ -154,11 +154,16 @@ k_scalar(ScalarArgs a, int rk) {
   const double dt_old = sc->Deltat;
   const int nzb = g.mgalz / a.zb_rows;
   if (ihist) {
-    for (int j = tid; j < g.my; j += blockDim.x) {
-      const double* sp = a.stat_plane + (long)j * STAT_W;
+    // one wave per plane: lanes sum the plane's 'p' partials in a fixed order
+    const int lane = tid & 63, nw = blockDim.x >> 6;
+    for (int j = tid >> 6; j < g.my; j += nw) {
       double ph[3] = {0.0, 0.0, 0.0};
-      for (int zb = 0; zb < nzb; ++zb)
+      for (int zb = lane; zb < nzb; zb += 64)
         for (int q = 0; q < 3; ++q) ph[q] += a.phist[((long)j * nzb + zb) * 3 + q];
+      for (int o = 32; o > 0; o >>= 1)
+        for (int q = 0; q < 3; ++q) ph[q] += __shfl_xor(ph[q], o, 64);
+      if (lane != 0) continue;
+      const double* sp = a.stat_plane + (long)j * STAT_W;
       const double o3s = sp[16] - g.s;
       const double pk = -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);
       const double v[SNF_NSTAT] = {sp[11], sp[12], sp[13], sp[0], sp[1], sp[2], sp[3], sp[4],
