@@ -405,9 +405,15 @@ void launch_yf(snf_handle* h, int rk) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   switch (h->g.my) {
-    case 33: k_yf_explicit<11, 3><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-    case 49: k_yf_explicit<7, 7><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-    default: k_yf_explicit<11, 7, 5><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    case 33: k_yf_explicit<YFFT<11, 3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    case 49: k_yf_explicit<YFFT<7, 7>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    case 385: k_yf_explicit<YFFT<11, 7, 5>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    default:   // direct DFT, my <= Y_DIRECT_MAX
+      switch ((h->g.my + 63) / 64) {
+        case 1: k_yf_explicit<YDirect<1>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+        case 2: k_yf_explicit<YDirect<2>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+        default: k_yf_explicit<YDirect<3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+      }
   }
 }
 
@@ -438,9 +444,15 @@ void launch_yfilter(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   switch (h->g.my) {
-    case 33: k_yf_filter<11, 3><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-    case 49: k_yf_filter<7, 7><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-    default: k_yf_filter<11, 7, 5><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    case 33: k_yf_filter<YFFT<11, 3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    case 49: k_yf_filter<YFFT<7, 7>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    case 385: k_yf_filter<YFFT<11, 7, 5>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+    default:
+      switch ((h->g.my + 63) / 64) {
+        case 1: k_yf_filter<YDirect<1>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+        case 2: k_yf_filter<YDirect<2>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+        default: k_yf_filter<YDirect<3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
+      }
   }
 }
 
@@ -467,6 +479,11 @@ const char* snf_last_error(void) { return g_err.c_str(); }
 
 // line lengths my with a compiled spectral y plan (radices of the passes)
 static bool y_plan(int my) { return my == 33 || my == 49 || my == 385; }
+// line lengths the spectral y stage serves: the radix plans, and the direct
+// DFT where its my^2 work per line still beats the banded sweeps (measured:
+// 3.2x faster at my = 32, 1.8x at 129, 2.2x slower at 193)
+#define Y_DIRECT_MAX 144
+static bool y_spectral(int my) { return y_plan(my) || my <= Y_DIRECT_MAX; }
 // radices of the compiled y-transform plans in pass order (the template
 // arguments of k_yf_explicit / k_yf_filter); returns the number of passes
 static int y_radices(int my, int* R) {
@@ -493,8 +510,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
     return fail(SNF_BAD_PARAM, "precision: SNF_PREC_FP64 or SNF_PREC_MIXED");
   if (p->ydealias < 0 || p->ydealias > 3)
     return fail(SNF_BAD_PARAM, "ydealias: 0 (none), 1, 2 (y-Fourier cuts) or 3 (compact filter)");
-  if (p->ydealias == 3 && !y_plan(p->my))
-    return fail(SNF_BAD_PARAM, "ydealias=3 needs a compiled y-transform plan (my = 33, 49, 385)");
+  if (p->ydealias == 3 && !y_spectral(p->my))
+    return fail(SNF_BAD_PARAM, "ydealias=3 needs my <= 144 or a y-transform plan (my = 385)");
   snf_handle* h = new snf_handle();
   h->p = *p;
   h->mixed = p->precision == SNF_PREC_MIXED;
@@ -653,7 +670,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   }
   {
     const char* e = getenv("SNF_Y_PATH");
-    h->yfft = p->explicit_visc == 1 && y_plan(my) && !(e && std::string(e) == "banded");
+    h->yfft = p->explicit_visc == 1 && y_spectral(my) && !(e && std::string(e) == "banded");
     h->ysym_on = h->yfft || p->ydealias == 3;
     std::vector<cplx> ty(my);
     for (int t = 0; t < my; ++t) {
@@ -685,9 +702,14 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // tile, a column per wave, roots, two multiplier rows, four phase tables
     // (1472 B at my = 385: 81552 B, two blocks still fit one CU's 160 KiB)
     h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
-    for (const void* ky : {(const void*)k_yf_explicit<11, 3>, (const void*)k_yf_explicit<7, 7>,
-                           (const void*)k_yf_explicit<11, 7, 5>, (const void*)k_yf_filter<11, 3>,
-                           (const void*)k_yf_filter<7, 7>, (const void*)k_yf_filter<11, 7, 5>})
+    for (const void* ky :
+         {(const void*)k_yf_explicit<YFFT<11, 3>>, (const void*)k_yf_explicit<YFFT<7, 7>>,
+          (const void*)k_yf_explicit<YFFT<11, 7, 5>>, (const void*)k_yf_explicit<YDirect<1>>,
+          (const void*)k_yf_explicit<YDirect<2>>, (const void*)k_yf_explicit<YDirect<3>>,
+          (const void*)k_yf_filter<YFFT<11, 3>>,
+          (const void*)k_yf_filter<YFFT<7, 7>>, (const void*)k_yf_filter<YFFT<11, 7, 5>>,
+          (const void*)k_yf_filter<YDirect<1>>, (const void*)k_yf_filter<YDirect<2>>,
+          (const void*)k_yf_filter<YDirect<3>>})
       (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).

@@ -154,7 +154,9 @@ k_scalar(ScalarArgs a, int rk) {
   const double dt_old = sc->Deltat;
   const int nzb = g.mgalz / a.zb_rows;
   if (ihist) {
-    // one wave per plane: lanes sum the plane's 'p' partials in a fixed order
+    // one wave per plane sums the plane's 'p' partials (lanes in a fixed
+    // order); then one thread per plane accumulates the plane's statistics
+    __shared__ double sh_ph[SNF_MAXN][3];
     const int lane = tid & 63, nw = blockDim.x >> 6;
     for (int j = tid >> 6; j < g.my; j += nw) {
       double ph[3] = {0.0, 0.0, 0.0};
@@ -162,7 +164,12 @@ k_scalar(ScalarArgs a, int rk) {
         for (int q = 0; q < 3; ++q) ph[q] += a.phist[((long)j * nzb + zb) * 3 + q];
       for (int o = 32; o > 0; o >>= 1)
         for (int q = 0; q < 3; ++q) ph[q] += __shfl_xor(ph[q], o, 64);
-      if (lane != 0) continue;
+      if (lane == 0)
+        for (int q = 0; q < 3; ++q) sh_ph[j][q] = ph[q];
+    }
+    __syncthreads();
+    for (int j = tid; j < g.my; j += blockDim.x) {
+      const double* ph = sh_ph[j];
       const double* sp = a.stat_plane + (long)j * STAT_W;
       const double o3s = sp[16] - g.s;
       const double pk = -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);

@@ -190,6 +190,66 @@ __device__ __forceinline__ void yf_fft(cplx* T, Col col, const cplx* tw) {
   yf_passes<SIGN, YPlanT<Rs...>::N, 1, 0, Col, Rs...>(T, col, tw);
 }
 
+// Direct DFT of column col in place (natural order in and out,
+// unnormalised) for any N <= 64 QM: lane l computes X[l + 64 q] as a sum over
+// the column (each x_j one LDS broadcast) against rt[t] = exp(-2 pi i t/N).
+// Line lengths without a radix plan use it; N^2 multiply-adds per transform
+// cost less than the banded sweeps up to a few hundred points.
+template <int SIGN, int QM, class Col>
+__device__ __forceinline__ void yf_dft_direct(cplx* T, Col col, const cplx* rt, int N) {
+  const int lane = threadIdx.x & 63;
+  wf_sync();   // the column was just written lane by lane
+  cplx acc[QM];
+  int idx[QM], stp[QM];
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    acc[q] = mkc(0.0, 0.0);
+    idx[q] = 0;
+    stp[q] = (lane + 64 * q) % N;
+  }
+  for (int j = 0; j < N; ++j) {
+    const cplx x = T[col(j)];
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
+      const cplx w = rt[idx[q]];
+      acc[q] = cadd(acc[q], SIGN < 0 ? cmul(x, w) : cmulc(x, w));
+      idx[q] += stp[q];
+      if (idx[q] >= N) idx[q] -= N;
+    }
+  }
+  wf_sync();
+#pragma unroll
+  for (int q = 0; q < QM; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) T[col(m)] = acc[q];
+  }
+  wf_sync();
+}
+
+// Plans of the y-line transforms, the template argument of the y kernels.
+//   YFFT<Rs...>  mixed-radix Stockham passes, N = prod Rs at compile time,
+//                twiddles from the pass-major table;
+//   YDirect<QM>  the direct DFT, N = my <= 64 QM at run time, twiddles from
+//                the root table exp(-2 pi i t/N).
+template <int... Rs> struct YFFT {
+  static constexpr int Q = (YPlanT<Rs...>::N + 63) / 64;
+  static constexpr bool direct = false;
+  static constexpr int NC = YPlanT<Rs...>::N;   // line length (compile time)
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* tw, int) {
+    yf_fft<SIGN, Col, Rs...>(T, c, tw);
+  }
+};
+template <int QM> struct YDirect {
+  static constexpr int Q = QM;
+  static constexpr bool direct = true;
+  static constexpr int NC = 0;                  // line length: my, at run time
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N) {
+    yf_dft_direct<SIGN, QM>(T, c, rt, N);
+  }
+};
+
 // seven stencil coefficients held in registers
 struct C7 {
   double c[7];

@@ -155,10 +155,11 @@ k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt
   gst(o + 2, mkc(yf_filter_symbol(e1), 0.0));
 }
 
-template <int... Rs>
+template <class PL>
 __global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
 k_yf_explicit(YArgs a, int rk) {
-  constexpr int N = YPlanT<Rs...>::N, Q = (N + 63) / 64;
+  constexpr int Q = PL::Q;
+  const int N = PL::direct ? a.g.my : PL::NC;
   if (a.sc->halted) return;
   extern __shared__ cplx T[];
   const Geom& g = a.g;
@@ -170,7 +171,7 @@ k_yf_explicit(YArgs a, int rk) {
   // LDS: tile [N][YF_LINES], one column per wave, the N-th roots, the
   // multiplier rows of this block's kx at the old and new twist
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.tpy + t);
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
   const int ir = rk - 1;
   double dt, f1, f2;
@@ -198,7 +199,7 @@ k_yf_explicit(YArgs a, int rk) {
         T[ac(j)] = x[q];
       }
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -206,7 +207,7 @@ k_yf_explicit(YArgs a, int rk) {
         T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m + 1));
       }
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -220,7 +221,7 @@ k_yf_explicit(YArgs a, int rk) {
         T[ac(j)] = mkc(un, 0.0);
       }
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -228,7 +229,7 @@ k_yf_explicit(YArgs a, int rk) {
         T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m));
       }
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -289,7 +290,7 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+  PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
@@ -299,7 +300,7 @@ k_yf_explicit(YArgs a, int rk) {
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
   }
-  yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+  PL::template fft<1>(T, ac, rt, N);
   yf_commit(T, pfa, N);                    // rhg
   yf_issue(pfa, a.rdome, P, L0, nv, N);
   {
@@ -344,7 +345,7 @@ k_yf_explicit(YArgs a, int rk) {
     }
   }
   yf_flush(T, a.vor, P, L0, nv, N);
-  yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+  PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
@@ -353,7 +354,7 @@ k_yf_explicit(YArgs a, int rk) {
       T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
     }
   }
-  yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+  PL::template fft<1>(T, ac, rt, N);
   {
     YPh<false> ph(phn, lane);
 #pragma unroll
@@ -377,7 +378,7 @@ k_yf_explicit(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cmul(T[tc(j)], e);
     }
   }
-  yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+  PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
@@ -400,7 +401,7 @@ k_yf_explicit(YArgs a, int rk) {
       }
     }
   }
-  yf_fft<-1, TileCol, Rs...>(T, tc, rt);
+  PL::template fft<-1>(T, tc, rt, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
@@ -410,7 +411,7 @@ k_yf_explicit(YArgs a, int rk) {
       T[ac(m)] = cadd(T[ac(m)], cmul(T[tc(m)], mul));
     }
   }
-  yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+  PL::template fft<1>(T, ac, rt, N);
   yf_commit(T, pfa, N);                    // rhv
   {
     YPh<false> ph(pho, lane);
@@ -459,7 +460,7 @@ k_yf_explicit(YArgs a, int rk) {
         if (j < N) T[ac(j)] = cmul(x[q], e);
       }
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -471,8 +472,8 @@ k_yf_explicit(YArgs a, int rk) {
         T[ac(m)] = vh;
       }
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
-    yf_fft<1, TileCol, Rs...>(T, tc, rt);
+    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft<1>(T, tc, rt, N);
     YPh<false> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -496,7 +497,7 @@ k_yf_explicit(YArgs a, int rk) {
         }
       }
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -505,7 +506,7 @@ k_yf_explicit(YArgs a, int rk) {
         T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
       }
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<1>(T, ac, rt, N);
     YPh<false> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -528,10 +529,11 @@ k_yf_explicit(YArgs a, int rk) {
 // (cross.f90:214-225, 476-488), then dU/dy, dW/dy of the filtered means
 // (read by the x-z stage, cross.f90:1198-1205).  Same block layout as
 // k_yf_explicit.
-template <int... Rs>
+template <class PL>
 __global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
 k_yf_filter(YArgs a) {
-  constexpr int N = YPlanT<Rs...>::N, Q = (N + 63) / 64;
+  constexpr int Q = PL::Q;
+  const int N = PL::direct ? a.g.my : PL::NC;
   if (a.sc->halted) return;
   extern __shared__ cplx T[];
   const Geom& g = a.g;
@@ -540,7 +542,7 @@ k_yf_filter(YArgs a) {
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld(a.tpy + t);
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
   const double invN = 1.0 / (double)N;
   const TileCol tc{w};
@@ -555,13 +557,13 @@ k_yf_filter(YArgs a) {
       const int j = lane + 64 * q;
       if (j < N) T[ac(j)] = gld(u + j);
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = crmul(invN * gld(sy + YS * m + 2).x, T[ac(m)]);
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -571,13 +573,13 @@ k_yf_filter(YArgs a) {
         T[ac(j)] = uf;
       }
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, gld(sy + YS * m)));
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -604,13 +606,13 @@ k_yf_filter(YArgs a) {
         if (j < N) T[ac(j)] = cmul(T[tc(j)], e);
       }
     }
-    yf_fft<-1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = crmul(invN * gld(sy + YS * m + 2).x, T[ac(m)]);
     }
-    yf_fft<1, OwnCol, Rs...>(T, ac, rt);
+    PL::template fft<1>(T, ac, rt, N);
     {
       YPhase ph(tho, 0.0, lane);
 #pragma unroll

@@ -274,3 +274,30 @@ def test_spectra_and_reset(lib):
         sp, _ = box.spectra()
         st, _, nacum, tdt = box.statistics()
         assert nacum == 0 and tdt == 0.0 and not np.any(sp) and not np.any(st)
+
+
+# line lengths without a radix plan (my <= 144) take the direct-DFT spectral
+# y stage: against the oracle's compact solves and against the banded sweeps
+@pytest.mark.parametrize("grid", [(32, 32, 32), (48, 40, 48), (64, 65, 64), (32, 97, 32),
+                                  (32, 129, 32), (32, 144, 32), (32, 145, 32)])
+def test_cross_direct_dft_y_3_steps(lib, grid):
+    st, (vor, phi, u00, w00), box, c = _run_pair(1, 3, grid=grid)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("grid", [(64, 65, 64), (96, 96, 96)])
+def test_direct_dft_and_banded_y_paths_agree(lib, monkeypatch, grid):
+    vor, phi, u00, w00 = box_ic(*grid)
+    outs = []
+    for path in ("spectral", "banded"):
+        monkeypatch.setenv("SNF_Y_PATH", path)
+        box = _box(*grid, explicit=1)
+        box.set_state(vor, phi, u00, w00)
+        box.cross(5)
+        outs.append(box.get_state())
+    for a, b in zip(*outs):
+        assert rel_l2(a, b) < STEP_TOL
