@@ -401,18 +401,26 @@ ScalarArgs sargs(snf_handle* h) {
   return a;
 }
 
+// the spectral y stage of one substep: explicit or implicit viscous terms
+template <class PL>
+void launch_yf_plan(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
+  if (h->p.explicit_visc == 1)
+    k_yf_explicit<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+  else
+    k_yf_implicit<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+}
 void launch_yf(snf_handle* h, int rk) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   switch (h->g.my) {
-    case 33: k_yf_explicit<YFFT<11, 3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-    case 49: k_yf_explicit<YFFT<7, 7>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-    case 385: k_yf_explicit<YFFT<11, 7, 5>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+    case 33: launch_yf_plan<YFFT<11, 3>>(h, rk, nb, a); break;
+    case 49: launch_yf_plan<YFFT<7, 7>>(h, rk, nb, a); break;
+    case 385: launch_yf_plan<YFFT<11, 7, 5>>(h, rk, nb, a); break;
     default:   // direct DFT, my <= Y_DIRECT_MAX
       switch ((h->g.my + 63) / 64) {
-        case 1: k_yf_explicit<YDirect<1>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-        case 2: k_yf_explicit<YDirect<2>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
-        default: k_yf_explicit<YDirect<3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk); break;
+        case 1: launch_yf_plan<YDirect<1>>(h, rk, nb, a); break;
+        case 2: launch_yf_plan<YDirect<2>>(h, rk, nb, a); break;
+        default: launch_yf_plan<YDirect<3>>(h, rk, nb, a); break;
       }
   }
 }
@@ -670,7 +678,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   }
   {
     const char* e = getenv("SNF_Y_PATH");
-    h->yfft = p->explicit_visc == 1 && y_spectral(my) && !(e && std::string(e) == "banded");
+    h->yfft = y_spectral(my) && !(e && std::string(e) == "banded");
     h->ysym_on = h->yfft || p->ydealias == 3;
     std::vector<cplx> ty(my);
     for (int t = 0; t < my; ++t) {
@@ -706,6 +714,9 @@ int snf_create(const snf_params* p, snf_handle** out) {
          {(const void*)k_yf_explicit<YFFT<11, 3>>, (const void*)k_yf_explicit<YFFT<7, 7>>,
           (const void*)k_yf_explicit<YFFT<11, 7, 5>>, (const void*)k_yf_explicit<YDirect<1>>,
           (const void*)k_yf_explicit<YDirect<2>>, (const void*)k_yf_explicit<YDirect<3>>,
+          (const void*)k_yf_implicit<YFFT<11, 3>>, (const void*)k_yf_implicit<YFFT<7, 7>>,
+          (const void*)k_yf_implicit<YFFT<11, 7, 5>>, (const void*)k_yf_implicit<YDirect<1>>,
+          (const void*)k_yf_implicit<YDirect<2>>, (const void*)k_yf_implicit<YDirect<3>>,
           (const void*)k_yf_filter<YFFT<11, 3>>,
           (const void*)k_yf_filter<YFFT<7, 7>>, (const void*)k_yf_filter<YFFT<11, 7, 5>>,
           (const void*)k_yf_filter<YDirect<1>>, (const void*)k_yf_filter<YDirect<2>>,
@@ -1013,7 +1024,7 @@ static int enqueue_steps(snf_handle* h, int nstep) {
       }
       {
         TimedLaunch tl(h, SNF_K_Y);
-        if (h->p.explicit_visc == 1 && h->yfft)
+        if (h->yfft)
           launch_yf(h, rk);
         else if (h->p.explicit_visc == 1)
           k_y_explicit<<<(unsigned)((2 * h->P + 2 + 127) / 128), 128, 0, h->stream>>>(yargs(h), rk);

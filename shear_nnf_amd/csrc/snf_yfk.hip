@@ -524,6 +524,413 @@ k_yf_explicit(YArgs a, int rk) {
   yf_flush(T, a.v, P, L0, nv, N);
 }
 
+// Implicit RK3 substep of the y stage (cross.f90:797-990) in the same block
+// layout as k_yf_explicit, with the Helmholtz solves (lapcdy with k2 + dtr1,
+// diffy_shear_hepta.f90:466-525) as twisted-Fourier multipliers
+// 1 / (R2/B2 - k2 - dtr1).  Per line, old twist theta, new twist theta':
+//   omega_y: vwk = facb (gama hg [+ vorwk] + alpha/Re (D2 - k2) vor + vor/dt)
+//            mapped by m2; vor = Helm'(vwk); dome = D1'(vor);
+//            vorwk = zeta hg m1 (rk 1, 2)
+//   phi:     hv = rhv - D1(rdome); pwk as vwk with hv, phi;
+//            phi = Helm'(pwk); v = lapcdy'(phi); dv = D1'(v);
+//            phiwk = zeta hv m1 (rk 1, 2)
+// The (0,0) line keeps vor, phi and v (cross.f90:866, 952) and takes dome,
+// dv = D1 of them.  U, W (m=1 calls, last block) follow cross.f90:829-843,
+// 858-863, 900-918.  13 line transforms; every field read and written once.
+template <class PL>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_implicit(YArgs a, int rk) {
+  constexpr int Q = PL::Q;
+  const int N = PL::direct ? a.g.my : PL::NC;
+  if (a.sc->halted) return;
+  extern __shared__ cplx T[];
+  const Geom& g = a.g;
+  const long P = g.P;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
+  const long nbl = (long)g.kxn * nch;
+  cplx* rt = T + 2 * YF_LINES * N;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
+  __syncthreads();
+  const int ir = rk - 1;
+  double dt, f1, f2;
+  substep_factors(a.sc, g.s, rk, dt, f1, f2);
+  const double fva = c_alpha[ir] * (1.0 / g.Re);   // alpha(rk)/Re
+  const double facb = -g.Re * c_ibeta[ir];
+  const double rdt = 1.0 / dt;
+  const double dtr1 = a.sc->dtr * c_ibeta[ir];
+  const double gam = c_gama[ir], zet = c_zeta[ir];
+  const double invN = 1.0 / (double)N;
+  const TileCol tc{w};
+  const OwnCol ac{YF_LINES * N + w * N};
+  cplx x[Q];
+
+  if (blockIdx.x == nbl) {
+    // ---- 00 lines: U (wave 0) and W (wave 1), real, untwisted
+    if (w >= 2) return;
+    cplx* u = w == 0 ? a.u00c : a.w00c;
+    cplx* uwk = w == 0 ? a.u00wk : a.w00wk;
+    const double* rf = w == 0 ? a.rf0u : a.rf0w;
+    double* dd = w == 0 ? a.dudy : a.dwdy;
+    const cplx* sy = a.ysym_new;   // kx = 0 row: untwisted
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) {
+        x[q] = gld(u + j);
+        T[ac(j)] = x[q];
+      }
+    }
+    PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m + 1));
+    }
+    PL::template fft<1>(T, ac, rt, N);
+    // wk = facb (gama rf [+ wk] + alpha/Re D2 U + U/dt)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) {
+        const double rfj = gldd(rf + j);
+        double wkj = (rk == 1) ? gam * rfj : gld(uwk + j).x + gam * rfj;
+        wkj = wkj + fva * (T[ac(j)].x * invN);
+        wkj = facb * (wkj + rdt * x[q].x);
+        T[ac(j)] = mkc(wkj, 0.0);
+        if (rk != 3) gst(uwk + j, mkc(zet * rfj, 0.0));
+      }
+    }
+    // U = lapcdy(wk, dtr1); dU/dy
+    PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        const cplx s1 = gld(sy + YS * m), s2 = gld(sy + YS * m + 1);
+        const cplx uh = cmul(T[ac(m)], yf_div(mkc(invN, 0.0), mkc(s2.x - dtr1, s2.y)));
+        T[ac(m)] = uh;
+        T[tc(m)] = cmul(uh, s1);
+      }
+    }
+    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft<1>(T, tc, rt, N);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) {
+        gst(u + j, mkc(T[ac(j)].x, 0.0));
+        gstd(dd + j, T[tc(j)].x);
+      }
+    }
+    return;
+  }
+
+  const long lb = xcd_block(blockIdx.x, nbl);
+  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
+  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
+  cplx* vwk_blk = a.vorwk + lb * (long)N * YF_LINES;    // block-major accumulators
+  cplx* pwk_blk = a.phiwk + lb * (long)N * YF_LINES;
+  const int k = min(YF_LINES * ch + w, g.mz - 1);
+  const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
+  const double k2 = kx * kx + kzv * kzv;
+  const bool is00 = (i == 0 && k == 0);
+  const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
+  const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
+  cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [2][m]
+  cplx* syn = syo + 2 * N;     // ... at the new twist
+  for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
+    const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
+    syo[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
+    syn[(t & 1) * N + (t >> 1)] = gld(a.ysym_new + src);
+  }
+  cplx* pho = syn + 2 * N;
+  cplx* phn = pho + YPH_N(Q);
+  cplx* pm1 = phn + YPH_N(Q);
+  cplx* pm2 = pm1 + YPH_N(Q);
+  {
+    const double pa[4] = {tho, thn, kx * f1 * g.dy, kx * f2 * g.dy};
+    const double pa0[4] = {0.0, 0.0, -0.5 * kx * f1 * g.Ly, -0.5 * kx * f2 * g.Ly};
+    for (int t = threadIdx.x; t < 4 * YPH_N(Q); t += blockDim.x) pho[t] = yph_entry(t, Q, pa, pa0);
+  }
+  // Helmholtz multiplier 1/(R2/B2 - k2 - dtr1) / N at the new twist
+  auto helm = [&](int m) {
+    const cplx s2 = syn[N + m];
+    return yf_div(mkc(invN, 0.0), mkc(s2.x - (k2 + dtr1), s2.y));
+  };
+
+  // ===== omega_y chain =====
+  cplx pfa[Q], pfb[Q];
+  yf_fill(T, a.vor, P, L0, nv, N);
+  yf_issue(pfa, a.rhg, P, L0, nv, N);
+  if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
+  {
+    YPh<true> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) {
+        x[q] = T[tc(j)];
+        T[ac(j)] = cmul(x[q], e);
+      }
+    }
+  }
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx s2 = syo[N + m];
+      const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fva * invN, mkc(s2.x - k2, s2.y));
+      T[ac(m)] = cmul(T[ac(m)], mul);
+    }
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  // own: alpha/Re (D2 - k2) vor + vor/dt [+ vorwk]
+  if (rk > 1) yf_commit(T, pfb, N);       // vorwk
+  {
+    YPh<false> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) {
+        cplx o = cadd(cmul(T[ac(j)], e), crmul(rdt, x[q]));
+        if (rk > 1) o = cadd(T[tc(j)], o);
+        T[ac(j)] = o;
+      }
+    }
+  }
+  yf_commit(T, pfa, N);                    // rhg
+  yf_issue(pfa, a.rdome, P, L0, nv, N);
+  yf_issue(pfb, a.phi, P, L0, nv, N);
+  {
+    YPh<false> m1(pm1, lane);
+    YPh<false> m2(pm2, lane);
+    YPh<true> en(phn, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx p1 = m1.at(q), p2 = m2.at(q), e = en.at(q);
+      if (j < N) {
+        const cplx hgj = T[tc(j)];
+        const cplx vwk = cmul(crmul(facb, cadd(T[ac(j)], crmul(gam, hgj))), p2);
+        T[ac(j)] = cmul(is00 ? x[q] : vwk, e);      // to the Helmholtz solve (00: D1 of vor)
+        if (rk != 3) T[tc(j)] = cmul(crmul(zet, hgj), p1);
+      }
+    }
+  }
+  if (rk != 3) yf_flush_blk(T, vwk_blk, N);
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx vh = is00 ? crmul(invN, T[ac(m)]) : cmul(T[ac(m)], helm(m));
+      T[ac(m)] = vh;
+      T[tc(m)] = cmul(vh, syn[m]);
+    }
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  PL::template fft<1>(T, tc, rt, N);
+  {
+    YPh<false> en(phn, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = en.at(q);
+      if (j < N) {
+        if (!is00) x[q] = cmul(T[ac(j)], e);   // vor (the (0,0) line keeps its own)
+        T[tc(j)] = cmul(T[tc(j)], e);          // dome
+      }
+    }
+  }
+  yf_flush(T, a.dome, P, L0, nv, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = lane + 64 * q;
+    if (j < N) T[tc(j)] = x[q];
+  }
+  yf_flush(T, a.vor, P, L0, nv, N);
+
+  // ===== phi chain =====
+  yf_commit(T, pfb, N);                    // phi
+  yf_issue(pfb, a.rhv, P, L0, nv, N);
+  {
+    YPh<true> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) {
+        x[q] = T[tc(j)];
+        T[ac(j)] = cmul(x[q], e);
+      }
+    }
+  }
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx s2 = syo[N + m];
+      const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fva * invN, mkc(s2.x - k2, s2.y));
+      T[ac(m)] = cmul(T[ac(m)], mul);
+    }
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  {
+    YPh<false> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) T[ac(j)] = cadd(cmul(T[ac(j)], e), crmul(rdt, x[q]));
+    }
+  }
+  // the (0,0) line keeps phi: x holds it until the flush
+  yf_commit(T, pfa, N);                    // rdome
+  if (rk > 1) yf_issue_blk(pfa, pwk_blk, N);
+  {
+    YPh<true> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) T[tc(j)] = cmul(T[tc(j)], e);
+    }
+  }
+  PL::template fft<-1>(T, tc, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) T[tc(m)] = cmul(T[tc(m)], crmul(invN, syo[m]));
+  }
+  PL::template fft<1>(T, tc, rt, N);
+  // pfb holds rhv; D1(rdome) goes to registers d, the tile turns over
+  cplx d[Q];
+  {
+    YPh<false> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) d[q] = cmul(T[tc(j)], e);
+    }
+  }
+  if (rk > 1) {
+    yf_commit(T, pfa, N);                  // phiwk
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) T[ac(j)] = cadd(T[tc(j)], T[ac(j)]);
+    }
+  }
+  yf_commit(T, pfb, N);                    // rhv
+  {
+    YPh<false> m1(pm1, lane);
+    YPh<false> m2(pm2, lane);
+    YPh<true> en(phn, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx p1 = m1.at(q), p2 = m2.at(q), e = en.at(q);
+      if (j < N) {
+        const cplx hvj = csub(T[tc(j)], d[q]);                 // hv = rhv - D1(rdome)
+        const cplx pwk = cmul(crmul(facb, cadd(T[ac(j)], crmul(gam, hvj))), p2);
+        T[ac(j)] = cmul(pwk, e);
+        if (rk != 3) T[tc(j)] = cmul(crmul(zet, hvj), p1);
+      }
+    }
+  }
+  if (rk != 3) yf_flush_blk(T, pwk_blk, N);
+  // phi = Helm'(pwk); v = lapcdy'(phi), dv = D1'(v)
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx ph = cmul(T[ac(m)], helm(m));
+      T[ac(m)] = ph;
+      T[tc(m)] = ph;
+    }
+  }
+  PL::template fft<1>(T, tc, rt, N);
+  {
+    YPh<false> en(phn, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = en.at(q);
+      if (j < N && !is00) x[q] = cmul(T[tc(j)], e);   // phi
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = lane + 64 * q;
+    if (j < N) T[tc(j)] = x[q];
+  }
+  yf_flush(T, a.phi, P, L0, nv, N);
+  if (L0 == 0) yf_fill(T, a.v, P, L0, nv, N);   // block of the (0,0) mode
+  if (!is00) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        const cplx s1 = syn[m], s2 = syn[N + m];
+        // own holds Helm'(pwk) / N with the 1/N already applied once: the
+        // lapcdy multiplier B2/(R2 - k2 B2) without a second 1/N
+        const cplx vh = yf_div(T[ac(m)], mkc(s2.x - k2, s2.y));
+        T[tc(m)] = cmul(vh, s1);
+        T[ac(m)] = vh;
+      }
+    }
+  } else {
+    YPh<true> en(phn, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = en.at(q);
+      if (j < N) {
+        x[q] = T[tc(j)];                     // v of the (0,0) mode, unchanged
+        T[ac(j)] = cmul(x[q], e);
+      }
+    }
+    PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) {
+        T[tc(m)] = cmul(T[ac(m)], crmul(invN, syn[m]));
+        T[ac(m)] = crmul(invN, T[ac(m)]);
+      }
+    }
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  PL::template fft<1>(T, tc, rt, N);
+  {
+    YPh<false> en(phn, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = en.at(q);
+      if (j < N) {
+        if (!is00) x[q] = cmul(T[ac(j)], e);   // v
+        T[tc(j)] = cmul(T[tc(j)], e);          // dv
+      }
+    }
+  }
+  yf_flush(T, a.dv, P, L0, nv, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = lane + 64 * q;
+    if (j < N) T[tc(j)] = x[q];
+  }
+  yf_flush(T, a.v, P, L0, nv, N);
+}
+
 // iydealiasing=3: the compact filter (filterc, filter_shear_hepta.f90:128-165)
 // applied in place to vor and phi at the current twist and to U, W
 // (cross.f90:214-225, 476-488), then dU/dy, dW/dy of the filtered means

@@ -301,3 +301,29 @@ def test_direct_dft_and_banded_y_paths_agree(lib, monkeypatch, grid):
         outs.append(box.get_state())
     for a, b in zip(*outs):
         assert rel_l2(a, b) < STEP_TOL
+
+
+# implicit substeps (cross.f90:797-990) on the spectral y stage: the
+# Helmholtz solves with k2 + dtr1 as multipliers, radix and direct plans
+@pytest.mark.parametrize("grid", [(32, 33, 32), (48, 385, 48), (64, 65, 64), (32, 96, 32)])
+def test_cross_spectral_implicit_3_steps(lib, grid):
+    st, (vor, phi, u00, w00), box, c = _run_pair(0, 3, grid=grid)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("grid", [(48, 49, 48), (96, 97, 96)])
+def test_spectral_and_banded_implicit_agree(lib, monkeypatch, grid):
+    vor, phi, u00, w00 = box_ic(*grid)
+    outs = []
+    for path in ("spectral", "banded"):
+        monkeypatch.setenv("SNF_Y_PATH", path)
+        box = _box(*grid, explicit=0)
+        box.set_state(vor, phi, u00, w00)
+        box.cross(5)
+        outs.append(box.get_state())
+    for a, b in zip(*outs):
+        assert rel_l2(a, b) < STEP_TOL
