@@ -464,10 +464,13 @@ void launch_yfilter(snf_handle* h) {
   }
 }
 
-int launch_twist(snf_handle* h, int which) {
+// tables at the twist shp[which]: the banded factors' tails (tails = false
+// when only the spectral y stage reads the twist) and the multipliers
+int launch_twist(snf_handle* h, int which, bool tails = true) {
   dim3 gt(h->g.kxn, 2);
-  k_twist<<<gt, 64, 0, h->stream>>>(h->g.my, h->b1, h->b2, h->c, h->shp[which], h->tw[which],
-                                    h->g.kxn, h->sc);
+  if (tails)
+    k_twist<<<gt, 64, 0, h->stream>>>(h->g.my, h->b1, h->b2, h->c, h->shp[which], h->tw[which],
+                                      h->g.kxn, h->sc);
   if (h->ysym_on) {
     dim3 gs(h->g.kxn, (h->g.my + 255) / 256);
     k_ysym<<<gs, 256, 0, h->stream>>>(h->g.my, h->c, h->shp[which], h->twy, h->ysym[which], h->sc);
@@ -1020,7 +1023,8 @@ static int enqueue_steps(snf_handle* h, int nstep) {
       {
         TimedLaunch tl(h, SNF_K_SMALL);
         k_scalar<<<1, 256, 0, h->stream>>>(sargs(h), rk);
-        if ((rc = launch_twist(h, 1 - h->cur))) return rc;
+        // the banded kernels read the tails; a spectral y stage only the multipliers
+        if ((rc = launch_twist(h, 1 - h->cur, !h->yfft))) return rc;
       }
       {
         TimedLaunch tl(h, SNF_K_Y);
