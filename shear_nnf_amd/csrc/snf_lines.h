@@ -379,7 +379,7 @@ struct HFac {
 // T = 1, 2, 3: rows N-6, N-5, N-4.  Returns the border-row coefficients and
 // writes the backward-sweep row to the scratch.
 template <int T>
-__device__ __forceinline__ void helm_factor_row(HFac& h, int j, int N, cplx shp, HScratch hs,
+__device__ __forceinline__ void helm_factor_row(HFac& h, int j, int /*N*/, cplx shp, HScratch hs,
                                                 long NP, long o, cplx& le, cplx& lf, cplx& lg,
                                                 double& La_out, double& Lb_out, double& Lc_out) {
   const double Dj = h.a0 - (h.Lc_j * h.D3 * h.Lc_j + h.Lb_j * h.D2 * h.Lb_j + h.La_j * h.D1 * h.La_j);

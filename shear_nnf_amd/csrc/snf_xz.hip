@@ -447,7 +447,7 @@ k_hist_f(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
   for (int q = 0; q < 11; ++q) red[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
+    if ((int)threadIdx.x < st) {
 #pragma unroll
       for (int q = 0; q < 11; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
     }

@@ -614,7 +614,7 @@ k_energy_part(Geom g, const cplx* vor, const cplx* phi, double* part) {
   rb[threadIdx.x] = sb;
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) {
+    if ((int)threadIdx.x < st) {
       ra[threadIdx.x] += ra[threadIdx.x + st];
       rb[threadIdx.x] += rb[threadIdx.x + st];
     }
@@ -642,7 +642,7 @@ k_energy_final(const double* part, int nb, const cplx* u00, const cplx* w00, int
   for (int q = 0; q < 4; ++q) r[q][threadIdx.x] = s[q];
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st)
+    if ((int)threadIdx.x < st)
       for (int q = 0; q < 4; ++q) r[q][threadIdx.x] += r[q][threadIdx.x + st];
     __syncthreads();
   }
