@@ -409,6 +409,27 @@ void launch_yf_plan(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
   else
     k_yf_implicit<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
 }
+// start of a cross call: v, dv, dome (and dU/dy, dW/dy) at the current twist
+template <class PL>
+void launch_yf_first_plan(snf_handle* h, unsigned nb, const YArgs& a) {
+  k_yf_first<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
+}
+void launch_yf_first(snf_handle* h) {
+  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
+  const YArgs a = yargs(h);
+  switch (h->g.my) {
+    case 33: launch_yf_first_plan<YFFT<11, 3>>(h, nb, a); break;
+    case 49: launch_yf_first_plan<YFFT<7, 7>>(h, nb, a); break;
+    case 385: launch_yf_first_plan<YFFT<11, 7, 5>>(h, nb, a); break;
+    default:
+      switch ((h->g.my + 63) / 64) {
+        case 1: launch_yf_first_plan<YDirect<1>>(h, nb, a); break;
+        case 2: launch_yf_first_plan<YDirect<2>>(h, nb, a); break;
+        default: launch_yf_first_plan<YDirect<3>>(h, nb, a); break;
+      }
+  }
+}
+
 void launch_yf(snf_handle* h, int rk) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
@@ -720,6 +741,9 @@ int snf_create(const snf_params* p, snf_handle** out) {
           (const void*)k_yf_implicit<YFFT<11, 3>>, (const void*)k_yf_implicit<YFFT<7, 7>>,
           (const void*)k_yf_implicit<YFFT<11, 7, 5>>, (const void*)k_yf_implicit<YDirect<1>>,
           (const void*)k_yf_implicit<YDirect<2>>, (const void*)k_yf_implicit<YDirect<3>>,
+          (const void*)k_yf_first<YFFT<11, 3>>, (const void*)k_yf_first<YFFT<7, 7>>,
+          (const void*)k_yf_first<YFFT<11, 7, 5>>, (const void*)k_yf_first<YDirect<1>>,
+          (const void*)k_yf_first<YDirect<2>>, (const void*)k_yf_first<YDirect<3>>,
           (const void*)k_yf_filter<YFFT<11, 3>>,
           (const void*)k_yf_filter<YFFT<7, 7>>, (const void*)k_yf_filter<YFFT<11, 7, 5>>,
           (const void*)k_yf_filter<YDirect<1>>, (const void*)k_yf_filter<YDirect<2>>,
@@ -978,7 +1002,10 @@ static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
   if ((rc = launch_twist(h, h->cur))) return rc;
   // before the step loop (cross.f90:214-225)
   if (h->p.ydealias == 3 && !resume) launch_yfilter(h);
-  k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
+  if (h->yfft)
+    launch_yf_first(h);
+  else
+    k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
   HIPCHK(hipGetLastError());
   // steps in pairs from captured graphs (one launch per pair instead of about
   // fifty), the remainder launched one by one; with per-kernel timing on,

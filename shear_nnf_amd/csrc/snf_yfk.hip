@@ -931,6 +931,162 @@ k_yf_implicit(YArgs a, int rk) {
   yf_flush(T, a.v, P, L0, nv, N);
 }
 
+// Start of a cross call on the spectral y stage (cross.f90:408-445), the
+// role of k_y_first: at the current twist, v = lapcdy(phi) (the (0,0) line
+// keeps its v), dv = D1(v), dome = D1(vor); dU/dy, dW/dy (cross.f90:1198-1205)
+// in the last block.  Same block layout as k_yf_explicit; 6 line transforms.
+template <class PL>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_first(YArgs a) {
+  constexpr int Q = PL::Q;
+  const int N = PL::direct ? a.g.my : PL::NC;
+  if (a.sc->halted) return;
+  extern __shared__ cplx T[];
+  const Geom& g = a.g;
+  const long P = g.P;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
+  const long nbl = (long)g.kxn * nch;
+  cplx* rt = T + 2 * YF_LINES * N;
+  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
+  __syncthreads();
+  const double invN = 1.0 / (double)N;
+  const TileCol tc{w};
+  const OwnCol ac{YF_LINES * N + w * N};
+  cplx x[Q];
+
+  if (blockIdx.x == nbl) {
+    // ---- dU/dy (wave 0), dW/dy (wave 1): real, untwisted
+    if (w >= 2) return;
+    const cplx* u = w == 0 ? a.u00c : a.w00c;
+    double* dd = w == 0 ? a.dudy : a.dwdy;
+    const cplx* sy = a.ysym_old;   // kx = 0 row: untwisted
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) T[ac(j)] = gld(u + j);
+    }
+    PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int m = lane + 64 * q;
+      if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, gld(sy + YS * m)));
+    }
+    PL::template fft<1>(T, ac, rt, N);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      if (j < N) gstd(dd + j, T[ac(j)].x);
+    }
+    return;
+  }
+
+  const long lb = xcd_block(blockIdx.x, nbl);
+  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
+  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
+  const int k = min(YF_LINES * ch + w, g.mz - 1);
+  const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
+  const double k2 = kx * kx + kzv * kzv;
+  const bool is00 = (i == 0 && k == 0);
+  const cplx so = gld(a.shp_old + i);
+  const double tho = atan2(so.y, so.x) * invN;
+  cplx* sy = rt + N;           // d/dy, d2/dy2 at the current twist: [2][m]
+  for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
+    const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
+    sy[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
+  }
+  cplx* pho = sy + 2 * N;
+  {
+    const double pa[1] = {tho}, pa0[1] = {0.0};
+    for (int t = threadIdx.x; t < YPH_N(Q); t += blockDim.x) pho[t] = yph_entry(t, Q, pa, pa0);
+  }
+
+  // dome = D1(vor)
+  yf_fill(T, a.vor, P, L0, nv, N);
+  {
+    YPh<true> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) T[ac(j)] = cmul(T[tc(j)], e);
+    }
+  }
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, sy[m]));
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  {
+    YPh<false> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
+    }
+  }
+  yf_flush(T, a.dome, P, L0, nv, N);
+
+  // v = lapcdy(phi) (the (0,0) line keeps v), dv = D1(v)
+  if (L0 == 0) {                 // block of the (0,0) mode
+    yf_fill(T, a.v, P, L0, nv, N);
+    if (is00) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int j = lane + 64 * q;
+        if (j < N) x[q] = T[tc(j)];
+      }
+    }
+  }
+  yf_fill(T, a.phi, P, L0, nv, N);
+  {
+    YPh<true> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) T[ac(j)] = cmul(is00 ? x[q] : T[tc(j)], e);
+    }
+  }
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx s1 = sy[m], s2 = sy[N + m];
+      const cplx vh = is00 ? crmul(invN, T[ac(m)])
+                           : cmul(T[ac(m)], yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y)));
+      T[tc(m)] = cmul(vh, s1);
+      T[ac(m)] = vh;
+    }
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  PL::template fft<1>(T, tc, rt, N);
+  {
+    YPh<false> ph(pho, lane);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) {
+        if (!is00) x[q] = cmul(T[ac(j)], e);   // v
+        T[tc(j)] = cmul(T[tc(j)], e);          // dv
+      }
+    }
+  }
+  yf_flush(T, a.dv, P, L0, nv, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = lane + 64 * q;
+    if (j < N) T[tc(j)] = x[q];
+  }
+  yf_flush(T, a.v, P, L0, nv, N);
+}
+
 // iydealiasing=3: the compact filter (filterc, filter_shear_hepta.f90:128-165)
 // applied in place to vor and phi at the current twist and to U, W
 // (cross.f90:214-225, 476-488), then dU/dy, dW/dy of the filtered means
