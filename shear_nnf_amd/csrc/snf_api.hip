@@ -102,6 +102,7 @@ struct snf_handle {
   DevScalars* sc = nullptr;
   unsigned long long* umax_acc = nullptr;
   double *stat_plane = nullptr, *phist = nullptr, *stats = nullptr, *stats2 = nullptr;
+  double* pplane = nullptr;   // [my][3] 'p' sums per plane
   double* ring = nullptr;
   int cap = 4096;
   int nrec_read = 0;
@@ -368,6 +369,11 @@ int launch_xz(snf_handle* h, int rk) {
     { TimedLaunch tl(h, SNF_K_XZ_C);
       launch_xz_generic_any(h, 2, rk); }
   }
+  if (rk == 1) {   // the 'p' sums per plane of a statistics step
+    TimedLaunch tl(h, SNF_K_SMALL);
+    k_phist_plane<<<g.my, 64, 0, h->stream>>>(h->phist, g.mgalz / (h->wave_xz ? 1 : h->zb),
+                                              h->pplane, h->sc, h->p.nhist);
+  }
   HIPCHK(hipGetLastError());
   return SNF_OK;
 }
@@ -379,7 +385,6 @@ ScalarArgs sargs(snf_handle* h) {
   a.nhist = h->p.nhist;
   a.readflag = h->p.readflag;
   a.explicit_visc = h->p.explicit_visc;
-  a.zb_rows = h->wave_xz ? 1 : h->zb;
   a.upart = h->wave_xz ? reinterpret_cast<unsigned long long*>(h->upart) : nullptr;
   a.nupart = h->wave_xz ? UMAX_SLOTS : 0;
   a.cflx = h->cflx;
@@ -390,7 +395,7 @@ ScalarArgs sargs(snf_handle* h) {
   a.uprim = h->p.uprim;
   a.umax_acc = h->umax_acc;
   a.stat_plane = h->stat_plane;
-  a.phist = h->phist;
+  a.pplane = h->pplane;
   a.stats = h->stats;
   a.stats2 = h->stats2;
   a.ring = h->ring;
@@ -660,6 +665,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->umax_acc, 4));
   TRY(A(&h->stat_plane, (size_t)my * STAT_W));
   TRY(A(&h->phist, (size_t)my * mgalz * 3));
+  TRY(A(&h->pplane, (size_t)my * 3));
   TRY(A(&h->upart, (size_t)UMAX_SLOTS * 4));
   TRY(A(&h->twy, (size_t)my));
   TRY(A(&h->tpy, (size_t)my));
@@ -844,7 +850,7 @@ int snf_destroy(snf_handle* h) {
                   h->hyb_in, h->hyb_out, h->hr, h->hc, h->u00c, h->w00c, h->u00wk, h->w00wk,
                   h->u00r, h->w00r, h->rf0u, h->rf0w, h->dudy, h->dwdy, h->base, h->tw[0],
                   h->tw[1], h->shp[0], h->shp[1], h->shm[0], h->shm[1], h->twx, h->twz, h->sc,
-                  h->umax_acc, h->stat_plane, h->phist, h->stats, h->stats2, h->ring,
+                  h->umax_acc, h->stat_plane, h->phist, h->pplane, h->stats, h->stats2, h->ring,
                   h->upart, h->twy, h->tpy, h->ysym[0], h->ysym[1], h->spec, h->spec2};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);

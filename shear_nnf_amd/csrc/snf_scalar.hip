@@ -129,19 +129,37 @@ k_twist(int N, BaseFactor b1, BaseFactor b2, YCoef c, const cplx* __restrict__ s
 struct ScalarArgs {
   Geom g;
   DevScalars* sc;
-  int nhist, readflag, explicit_visc, zb_rows;
+  int nhist, readflag, explicit_visc;
   double cflx, cfly, cflz, cflvv, CFL, uprim;
   unsigned long long* umax_acc;
   unsigned long long* upart;  // [nupart][4] maxima as double bit patterns (or null)
   long nupart;
   const double* stat_plane;   // [my][STAT_W]
-  const double* phist;        // [my][Nz/2][3]
+  const double* pplane;       // [my][3]: 'p' partials summed per plane (k_phist_plane)
   double *stats, *stats2;     // [my][NSTAT]
   double* ring;               // [cap][24]
   int cap;
   cplx *shp_new, *shm_new;
   double wrap_lx;             // Lx when the offsets wrap (iydealiasing=2), else 0
 };
+
+// The 'p' triple products of a statistics step (cross.f90:2030-2045): block
+// j sums the x-row kernel's partials of plane j in a fixed order (lanes in
+// strides of 64, then a shuffle tree).  Launched at rkstep 1 after the x-row
+// kernel; a no-op off the statistics steps.
+__global__ void __launch_bounds__(64)
+k_phist_plane(const double* __restrict__ phist, int nzb, double* __restrict__ pplane,
+              const DevScalars* sc, int nhist) {
+  if (sc->halted || (sc->steps_local - 1) % nhist != 0) return;
+  const int j = blockIdx.x, lane = threadIdx.x;
+  double ph[3] = {0.0, 0.0, 0.0};
+  for (int zb = lane; zb < nzb; zb += 64)
+    for (int q = 0; q < 3; ++q) ph[q] += phist[((long)j * nzb + zb) * 3 + q];
+  for (int o = 32; o > 0; o >>= 1)
+    for (int q = 0; q < 3; ++q) ph[q] += __shfl_xor(ph[q], o, 64);
+  if (lane == 0)
+    for (int q = 0; q < 3; ++q) pplane[(long)j * 3 + q] = ph[q];
+}
 
 __global__ void __launch_bounds__(256)
 k_scalar(ScalarArgs a, int rk) {
@@ -152,24 +170,10 @@ k_scalar(ScalarArgs a, int rk) {
   const int ihist = (rk == 1) && ((sc->steps_local - 1) % a.nhist == 0);
   __shared__ double sh_dt;
   const double dt_old = sc->Deltat;
-  const int nzb = g.mgalz / a.zb_rows;
   if (ihist) {
-    // one wave per plane sums the plane's 'p' partials (lanes in a fixed
-    // order); then one thread per plane accumulates the plane's statistics
-    __shared__ double sh_ph[SNF_MAXN][3];
-    const int lane = tid & 63, nw = blockDim.x >> 6;
-    for (int j = tid >> 6; j < g.my; j += nw) {
-      double ph[3] = {0.0, 0.0, 0.0};
-      for (int zb = lane; zb < nzb; zb += 64)
-        for (int q = 0; q < 3; ++q) ph[q] += a.phist[((long)j * nzb + zb) * 3 + q];
-      for (int o = 32; o > 0; o >>= 1)
-        for (int q = 0; q < 3; ++q) ph[q] += __shfl_xor(ph[q], o, 64);
-      if (lane == 0)
-        for (int q = 0; q < 3; ++q) sh_ph[j][q] = ph[q];
-    }
-    __syncthreads();
+    // one thread per plane accumulates the plane's statistics
     for (int j = tid; j < g.my; j += blockDim.x) {
-      const double* ph = sh_ph[j];
+      const double* ph = a.pplane + (long)j * 3;
       const double* sp = a.stat_plane + (long)j * STAT_W;
       const double o3s = sp[16] - g.s;
       const double pk = -o3s * (-sp[3]) + o3s * o3s * (1.0 / g.Re);
