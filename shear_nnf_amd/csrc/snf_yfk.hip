@@ -168,6 +168,16 @@ k_yf_explicit(YArgs a, int rk) {
   // blocks: YF_LINES consecutive kz lines of one kx (nch chunks per kx)
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
+  // ---- (kx, kz) lines; every wave takes part in every block transfer.
+  // Neighbouring blocks share the 128-byte rows of the field: keep them on
+  // one XCD.  A line block issues its vor loads first, so that they are in
+  // flight while the block stages its tables.
+  const long lb = xcd_block(blockIdx.x, nbl);
+  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
+  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
+  cplx pv[Q];
+  if (blockIdx.x != nbl) yf_issue(pv, a.vor, P, L0, nv, N);
   // LDS: tile [N][YF_LINES], one column per wave, the N-th roots, the
   // multiplier rows of this block's kx at the old and new twist
   cplx* rt = T + 2 * YF_LINES * N;
@@ -238,13 +248,6 @@ k_yf_explicit(YArgs a, int rk) {
     return;
   }
 
-  // ---- (kx, kz) lines; every wave takes part in every block transfer.
-  // Neighbouring blocks share the 128-byte rows of the field: keep them on
-  // one XCD.
-  const long lb = xcd_block(blockIdx.x, nbl);
-  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
-  const long L0 = (long)i * g.mz + YF_LINES * ch;
-  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
   cplx* vwk_blk = a.vorwk + lb * (long)N * YF_LINES;    // block-major accumulators
   cplx* pwk_blk = a.phiwk + lb * (long)N * YF_LINES;
   const int k = min(YF_LINES * ch + w, g.mz - 1);
@@ -275,7 +278,7 @@ k_yf_explicit(YArgs a, int rk) {
   // two fields' loads stay in flight (pfa, pfb) while the block transforms;
   // each is issued about two stages before the tile needs it
   cplx pfa[Q], pfb[Q];
-  yf_fill(T, a.vor, P, L0, nv, N);
+  yf_commit(T, pv, N);                     // vor
   yf_issue(pfa, a.rhg, P, L0, nv, N);
   if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
   {
@@ -549,6 +552,13 @@ k_yf_implicit(YArgs a, int rk) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
+  // a line block's vor loads are in flight while it stages its tables
+  const long lb = xcd_block(blockIdx.x, nbl);
+  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
+  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
+  cplx pv[Q];
+  if (blockIdx.x != nbl) yf_issue(pv, a.vor, P, L0, nv, N);
   cplx* rt = T + 2 * YF_LINES * N;
   for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
@@ -626,10 +636,6 @@ k_yf_implicit(YArgs a, int rk) {
     return;
   }
 
-  const long lb = xcd_block(blockIdx.x, nbl);
-  const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
-  const long L0 = (long)i * g.mz + YF_LINES * ch;
-  const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
   cplx* vwk_blk = a.vorwk + lb * (long)N * YF_LINES;    // block-major accumulators
   cplx* pwk_blk = a.phiwk + lb * (long)N * YF_LINES;
   const int k = min(YF_LINES * ch + w, g.mz - 1);
@@ -662,7 +668,7 @@ k_yf_implicit(YArgs a, int rk) {
 
   // ===== omega_y chain =====
   cplx pfa[Q], pfb[Q];
-  yf_fill(T, a.vor, P, L0, nv, N);
+  yf_commit(T, pv, N);                     // vor
   yf_issue(pfa, a.rhg, P, L0, nv, N);
   if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
   {
