@@ -106,3 +106,46 @@ def test_jvp_batch_groups_of_handles():
     for q in (0, 15, 16, 19):
         yq, _ = box.shoot(x0 + eps[q] * dirs[q], T)
         assert np.array_equal(out[q], (yq - y0) / eps[q])
+
+
+@pytest.mark.parametrize("kw", [{"precision": "mixed"}, {"explicit": 0}])
+def test_jvp_batch_mixed_and_implicit(kw):
+    """the concurrent shootings of other handle kinds: bitwise those of
+    separate shootings.  T spans several steps: a first step that already
+    reaches t0 + T is followed by a zero-length one (see
+    test_implicit_shoot_shorter_than_one_step)"""
+    grid, T = (32, 33, 32), 0.05
+    box = _box(grid, **kw)
+    x0 = box.pack_state(*box_ic(*grid))
+    dirs = np.random.default_rng(2).normal(size=(3, x0.size)) * np.abs(x0).max()
+    eps = np.array([1e-6, 2e-6, 3e-6])
+    y0, _ = box.shoot(x0, T)
+    out = box.jvp_batch(x0, dirs, eps, T, y0=y0)
+    for q in range(3):
+        yq, _ = box.shoot(x0 + eps[q] * dirs[q], T)
+        assert np.array_equal(out[q], (yq - y0) / eps[q])
+
+
+def test_implicit_shoot_shorter_than_one_step():
+    """The stop test is skipped before the second step (cross.f90:356,
+    istep-1 /= 1), so when the first step already lands on t0 + T a second
+    step of length zero runs; the implicit update divides by Deltat
+    (cross.f90:849-865) and the state turns to NaN.  The library reproduces
+    the reference here: the same entries are NaN, the rest agree."""
+    from oracle.cross import Cross
+    from oracle.grid import Grid
+    grid, T = (32, 33, 32), 0.01
+    ic = box_ic(*grid)
+    box = _box(grid, explicit=0)
+    y, st = box.shoot(box.pack_state(*ic), T)
+    c = Cross(Grid(*grid, Re=1000.0, alp=ALP, gam=GAM, Ly=LY, s=1.0, CFL=0.5), explicit=0)
+    c.set_state(*ic)
+    c.etime = T
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        c.run(100000)
+    for a, b in zip(box.unpack_state(y), (c.vor, c.phi, c.u00, c.w00)):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        fin = ~np.isnan(b)
+        np.testing.assert_allclose(a[fin], b[fin], rtol=0, atol=1e-12 * max(1.0, np.abs(b[fin]).max(initial=0.0)))
+    assert np.isnan(c.vor).any()
+    assert st.time == pytest.approx(T, abs=1e-14)
