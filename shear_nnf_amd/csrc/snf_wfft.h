@@ -220,3 +220,59 @@ __device__ __forceinline__ void wfft_t2l(C* v, C* a, C* buf, const C* rt) {
   wf_sync();
   W::t2l_c(lane, buf, a, rt);
 }
+
+// Two independent transforms of one wave, stage by stage: each exchange
+// waits for the LDS once for both sequences, which halves the exposed LDS
+// latency of a kernel that runs at few waves per SIMD.  buf0, buf1: two
+// disjoint regions of WFFT<R0,SIGN>::LDS values.
+template <int R0, int SIGN, class C>
+__device__ __forceinline__ void wfft_l2t2(C* a0, C* a1, C* out0, C* out1, C* buf0, C* buf1,
+                                          const C* rt) {
+  typedef WFFT<R0, SIGN, C> W;
+  const int lane = threadIdx.x & 63;
+  wf_sync();
+  W::l2t_a(lane, a0, buf0, rt);
+  W::l2t_a(lane, a1, buf1, rt);
+  wf_sync();
+  C d0[8], d1[8];
+  if (lane < W::NT) {
+    W::l2t_b_read(lane, buf0, d0);
+    W::l2t_b_read(lane, buf1, d1);
+  }
+  wf_sync();
+  if (lane < W::NT) {
+    W::l2t_b_write(lane, d0, buf0, rt);
+    W::l2t_b_write(lane, d1, buf1, rt);
+  }
+  wf_sync();
+  if (lane < W::NT) {
+    W::l2t_c(lane, buf0, out0);
+    W::l2t_c(lane, buf1, out1);
+  }
+}
+
+template <int R0, int SIGN, class C>
+__device__ __forceinline__ void wfft_t2l2(C* v0, C* v1, C* a0, C* a1, C* buf0, C* buf1,
+                                          const C* rt) {
+  typedef WFFT<R0, SIGN, C> W;
+  const int lane = threadIdx.x & 63;
+  wf_sync();
+  if (lane < W::NT) {
+    W::t2l_a(lane, v0, buf0, rt);
+    W::t2l_a(lane, v1, buf1, rt);
+  }
+  wf_sync();
+  C u0[8], u1[8];
+  if (lane < W::NT) {
+    W::t2l_b_read(lane, buf0, u0);
+    W::t2l_b_read(lane, buf1, u1);
+  }
+  wf_sync();
+  if (lane < W::NT) {
+    W::t2l_b_write(lane, u0, buf0);
+    W::t2l_b_write(lane, u1, buf1);
+  }
+  wf_sync();
+  W::t2l_c(lane, buf0, a0, rt);
+  W::t2l_c(lane, buf1, a1, rt);
+}

@@ -115,8 +115,9 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   const int Mx = 64 * RX, Nz = g.mgalz, kxn = g.kxn;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   H* rt = lds;
-  H* work = lds + Mx;                    // staging, then per-wave FFT buffers
-  H* buf = work + wave * WI::LDS;
+  H* work = lds + Mx;                    // staging, then two FFT buffers per wave
+  H* buf = work + wave * 2 * WI::LDS;
+  H* buf1 = buf + WI::LDS;
   stage_roots(rt, twx, Mx);
   const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
   const int zblk = (int)(lb % gridDim.x), j = (int)(lb / gridDim.x);
@@ -173,8 +174,8 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   __syncthreads();   // staging dead: the region becomes the FFT buffers
   // --- inverse x transforms: task t holds x = t + 8 RX q ---
   H zx[3][8];
-#pragma unroll
-  for (int p = 0; p < 3; ++p) wfft_l2t<RX, 1>(zin[p], zx[p], buf, rt);
+  wfft_l2t2<RX, 1>(zin[0], zin[1], zx[0], zx[1], buf, buf1, rt);
+  wfft_l2t<RX, 1>(zin[2], zx[2], buf, rt);
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
   const R c2 = (R)(2.0 * g.chi);
   const R sy = (R)(g.s * ((double)j * g.dy - 0.5 * g.Ly));
@@ -200,8 +201,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   }
   // --- forward x transforms -> lane layout bins ---
   H a12[RX], a3[RX];
-  wfft_t2l<RX, -1>(z12, a12, buf, rt);
-  wfft_t2l<RX, -1>(z3, a3, buf, rt);
+  wfft_t2l2<RX, -1>(z12, z3, a12, a3, buf, buf1, rt);
   // separate H1, H2 from Z12 = H1 + i H2: needs bin Mx - k from another lane
   wf_sync();
 #pragma unroll
@@ -341,7 +341,7 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
 inline size_t xzw_lds_A(int R, size_t es) { return (size_t)(64 * R + WA_WAVES * (R * WF_EP > 8 * R * WF_FP ? R * WF_EP : 8 * R * WF_FP)) * es; }
 inline size_t xzw_lds_C(int R, size_t es) { return xzw_lds_A(R, es); }
 inline size_t xzw_lds_B(int Rx, int kxn, size_t es) {
-  const size_t fft = (size_t)WB_ROWS * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
+  const size_t fft = (size_t)WB_ROWS * 2 * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
   const size_t stage = (size_t)2 * kxn * (WB_ROWS + 1);
   const size_t out = (size_t)3 * kxn * (WB_ROWS + 1);
   size_t w = fft > stage ? fft : stage;
