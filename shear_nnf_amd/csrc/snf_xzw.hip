@@ -114,10 +114,15 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   H* lds = reinterpret_cast<H*>(xzw_smem);
   const int Mx = 64 * RX, Nz = g.mgalz, kxn = g.kxn;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // planes up to 384 points (WIDE) stage all 6 fields in one round and run
+  // independent transforms in pairs; the LDS this takes (67.5 KB at 384,
+  // FP64) still leaves 2 blocks per CU, which 512-point planes would lose
+  constexpr bool WIDE = RX <= 6;
+  constexpr int NFR = WIDE ? 6 : 2;      // fields staged per round
   H* rt = lds;
-  H* work = lds + Mx;                    // staging, then two FFT buffers per wave
-  H* buf = work + wave * 2 * WI::LDS;
-  H* buf1 = buf + WI::LDS;
+  H* work = lds + Mx;                    // staging, then the FFT buffers of each wave
+  H* buf = work + wave * (WIDE ? 2 : 1) * WI::LDS;
+  H* buf1 = buf + WI::LDS;               // second buffer (WIDE)
   stage_roots(rt, twx, Mx);
   const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
   const int zblk = (int)(lb % gridDim.x), j = (int)(lb / gridDim.x);
@@ -127,7 +132,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   // --- load the 3 field pairs of row r in lane layout (bins l + 64 m) ---
   // every load of the block is issued up front (runs of 4 z values, 64 B),
   // field by field: element i of a field is (kx, z) = (i / WB_ROWS, i % WB_ROWS);
-  // the tile then passes through LDS one field pair at a time
+  // the tile then passes through LDS, NFR fields at a time
   constexpr int KXM = 64 * RX / 3;                                  // kxn bound
   constexpr int NEF = (KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);   // per field
   const int tile = kxn * WB_ROWS;                                   // per field
@@ -142,39 +147,50 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
       if (i0 + q * 64 * WB_ROWS < tile) pre[f][q] = gld(src + (long)(kx0 + q * KXS) * Nz);
   }
   H zin[3][RX];
-  H* S = work;                           // [2][kxn][WB_ROWS + 1]
+  H* S = work;                           // [NFR][kxn][WB_ROWS + 1]
   const int SP = WB_ROWS + 1;
 #pragma unroll
-  for (int p = 0; p < 3; ++p) {
+  for (int f0 = 0; f0 < 6; f0 += NFR) {
     __syncthreads();
 #pragma unroll
-    for (int h2 = 0; h2 < 2; ++h2)
+    for (int h = 0; h < NFR; ++h)
 #pragma unroll
       for (int q = 0; q < NEF; ++q)
-        if (i0 + q * 64 * WB_ROWS < tile) S[(h2 * kxn + kx0 + q * KXS) * SP + rr0] = pre[2 * p + h2][q];
+        if (i0 + q * 64 * WB_ROWS < tile) S[(h * kxn + kx0 + q * KXS) * SP + rr0] = pre[f0 + h][q];
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < RX; ++m) {
-      const int b = lane + 64 * m;
-      H s = wf_mk((R)0, (R)0);
-      if (b == 0) {
-        const H Ga = S[r], Gb = S[kxn * SP + r];
-        s = wf_mk(Ga.x, Gb.x);   // c2r ignores the kx=0 imaginary part
-      } else if (b < kxn) {
-        const H Ga = S[b * SP + r], Gb = S[(kxn + b) * SP + r];
-        s = wf_mk(Ga.x - Gb.y, Ga.y + Gb.x);
-      } else if (b > Mx - kxn) {
-        const int i = Mx - b;
-        const H Ga = S[i * SP + r], Gb = S[(kxn + i) * SP + r];
-        s = wf_mk(Ga.x + Gb.y, Gb.x - Ga.y);
+    for (int pp = 0; pp < NFR / 2; ++pp) {
+      const int p = f0 / 2 + pp;
+      const H* Sa = S + 2 * pp * kxn * SP;
+      const H* Sb = Sa + kxn * SP;
+#pragma unroll
+      for (int m = 0; m < RX; ++m) {
+        const int b = lane + 64 * m;
+        H s = wf_mk((R)0, (R)0);
+        if (b == 0) {
+          const H Ga = Sa[r], Gb = Sb[r];
+          s = wf_mk(Ga.x, Gb.x);   // c2r ignores the kx=0 imaginary part
+        } else if (b < kxn) {
+          const H Ga = Sa[b * SP + r], Gb = Sb[b * SP + r];
+          s = wf_mk(Ga.x - Gb.y, Ga.y + Gb.x);
+        } else if (b > Mx - kxn) {
+          const int i = Mx - b;
+          const H Ga = Sa[i * SP + r], Gb = Sb[i * SP + r];
+          s = wf_mk(Ga.x + Gb.y, Gb.x - Ga.y);
+        }
+        zin[p][m] = s;
       }
-      zin[p][m] = s;
     }
   }
   __syncthreads();   // staging dead: the region becomes the FFT buffers
   // --- inverse x transforms: task t holds x = t + 8 RX q ---
   H zx[3][8];
-  wfft_l2t2<RX, 1>(zin[0], zin[1], zx[0], zx[1], buf, buf1, rt);
+  if (WIDE) {
+    wfft_l2t2<RX, 1>(zin[0], zin[1], zx[0], zx[1], buf, buf1, rt);
+  } else {
+    wfft_l2t<RX, 1>(zin[0], zx[0], buf, rt);
+    wfft_l2t<RX, 1>(zin[1], zx[1], buf, rt);
+  }
   wfft_l2t<RX, 1>(zin[2], zx[2], buf, rt);
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
   const R c2 = (R)(2.0 * g.chi);
@@ -201,7 +217,12 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   }
   // --- forward x transforms -> lane layout bins ---
   H a12[RX], a3[RX];
-  wfft_t2l2<RX, -1>(z12, z3, a12, a3, buf, buf1, rt);
+  if (WIDE) {
+    wfft_t2l2<RX, -1>(z12, z3, a12, a3, buf, buf1, rt);
+  } else {
+    wfft_t2l<RX, -1>(z12, a12, buf, rt);
+    wfft_t2l<RX, -1>(z3, a3, buf, rt);
+  }
   // separate H1, H2 from Z12 = H1 + i H2: needs bin Mx - k from another lane
   wf_sync();
 #pragma unroll
@@ -341,8 +362,9 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
 inline size_t xzw_lds_A(int R, size_t es) { return (size_t)(64 * R + WA_WAVES * (R * WF_EP > 8 * R * WF_FP ? R * WF_EP : 8 * R * WF_FP)) * es; }
 inline size_t xzw_lds_C(int R, size_t es) { return xzw_lds_A(R, es); }
 inline size_t xzw_lds_B(int Rx, int kxn, size_t es) {
-  const size_t fft = (size_t)WB_ROWS * 2 * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
-  const size_t stage = (size_t)2 * kxn * (WB_ROWS + 1);
+  const bool wide = Rx <= 6;   // k_xzw_B's WIDE
+  const size_t fft = (size_t)WB_ROWS * (wide ? 2 : 1) * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
+  const size_t stage = (size_t)(wide ? 6 : 2) * kxn * (WB_ROWS + 1);
   const size_t out = (size_t)3 * kxn * (WB_ROWS + 1);
   size_t w = fft > stage ? fft : stage;
   w = w > out ? w : out;
