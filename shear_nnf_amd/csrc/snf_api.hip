@@ -354,7 +354,7 @@ int launch_xz(snf_handle* h, int rk) {
     k_hist_f<<<g.my, 256, 0, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                           h->w00r, h->dudy, h->dwdy, h->p.vbulk, h->stat_plane,
                                           h->sc, h->p.nhist);
-    k_hist_spec<<<(unsigned)((g.P + 255) / 256), 256, 0, h->stream>>>(
+    k_hist_spec<<<(unsigned)((g.P + 63) / 64), 64 * SPEC_SEG, 0, h->stream>>>(
         g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk,
         h->spec, h->spec2, h->sc, h->p.nhist);
   }

@@ -351,9 +351,11 @@ k_xz_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
 // Spectra of the 'f' statistics (cross.f90:1970-2016): for every (kx, kz)
 // line, sums over the planes of |u|^2, |v|^2, |w|^2, Re(u v*), |o1|^2,
 // |o2|^2, |o3|^2, Re(o1 o2*) (kx > 0 counted twice), added to sp and, times
-// the step, to sp2 ([8][P]).  One thread per line walks the planes in order,
-// so the sums do not depend on the launch.
-__global__ void __launch_bounds__(256)
+// the step, to sp2 ([8][P]).  A block takes 64 consecutive lines; its
+// SPEC_SEG waves walk fixed segments of the planes in order, and the segment
+// sums are added in segment order, so the sums do not depend on the launch.
+#define SPEC_SEG 8
+__global__ void __launch_bounds__(64 * SPEC_SEG)
 k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
             const cplx* __restrict__ hg, const cplx* __restrict__ hv,
             const cplx* __restrict__ dvordy, const double* __restrict__ u00,
@@ -362,14 +364,17 @@ k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
             double* __restrict__ sp2, const DevScalars* sc, int nhist) {
   if (sc->halted) return;
   if ((sc->steps_local - 1) % nhist != 0) return;
-  const long L = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (L >= g.P) return;
-  const int i = (int)(L / g.mz), k = (int)(L - (long)i * g.mz);
+  __shared__ double red[SPEC_SEG][8][64];
+  const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  const long L = (long)blockIdx.x * 64 + lane;
+  const bool on = L < g.P;
+  const int i = on ? (int)(L / g.mz) : 0, k = on ? (int)(L - (long)i * g.mz) : 0;
   const double wt = (i == 0) ? 1.0 : 2.0;
   double acc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-  for (int j = 0; j < g.my; ++j) {
+  const int j0 = (int)((long)g.my * seg / SPEC_SEG), j1 = on ? (int)((long)g.my * (seg + 1) / SPEC_SEG) : j0;
+  for (int j = j0; j < j1; ++j) {
     PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
     cplx d[6];
     derived6(g, q, i, k, u00[j], w00[j], dudy[j], dwdy[j], vbulk, d);
@@ -382,11 +387,19 @@ k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
     acc[6] += wt * cabs2(d[5]);
     acc[7] += wt * cmulc(d[3], d[4]).x;
   }
-  const double dt = sc->Deltat;   // the step of the sample, as in the reference's 'f' pass
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    sp[q * g.P + L] += acc[q];
-    sp2[q * g.P + L] += acc[q] * dt;
+  for (int q = 0; q < 8; ++q) red[seg][q][lane] = acc[q];
+  __syncthreads();
+  if (seg == 0 && on) {
+    const double dt = sc->Deltat;   // the step of the sample, as in the reference's 'f' pass
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double t = red[0][q][lane];
+#pragma unroll
+      for (int s2 = 1; s2 < SPEC_SEG; ++s2) t += red[s2][q][lane];
+      sp[q * g.P + L] += t;
+      sp2[q * g.P + L] += t * dt;
+    }
   }
 }
 
