@@ -64,6 +64,18 @@ def _child(my, dy, jobs, q):
                                 ctypes.byref(ctypes.c_double(job[2][l])),
                                 ro.ctypes.data_as(ctypes.c_void_p), _cplx(1.0), _cplx(1.0),
                                 ctypes.byref(m))
+                elif kind == "deriv_reuse":
+                    # cross.f90:440-442: a call with ias=1 factors the twisted
+                    # operator; the next call, ias=0, reuses that factor
+                    ua, shp_a, shp_b, shm_b, order = job[2], job[3][l], job[4][l], job[5][l], job[6]
+                    ai = np.ascontiguousarray(ua[:, l]).view(np.float64).copy()
+                    ao = np.zeros_like(ai)
+                    lib.derivyc_(ai.ctypes.data_as(ctypes.c_void_p), ao.ctypes.data_as(ctypes.c_void_p),
+                                 _cplx(shp_a), _cplx(np.conj(shp_a)), ctypes.byref(ctypes.c_int(order)),
+                                 ctypes.byref(m), ctypes.byref(ctypes.c_int(1)))
+                    lib.derivyc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
+                                 _cplx(shp_b), _cplx(shm_b), ctypes.byref(ctypes.c_int(order)),
+                                 ctypes.byref(m), ctypes.byref(ctypes.c_int(0)))
                 elif kind == "deriv":
                     shp, shm, order = job[2][l], job[3][l], job[4]
                     lib.derivyc_(ui.ctypes.data_as(ctypes.c_void_p), ro.ctypes.data_as(ctypes.c_void_p),
@@ -87,7 +99,10 @@ def run(my, dy, jobs, timeout=120):
     jobs: ("deriv", u[my, nl], shp[nl], shm[nl], order),
           ("lap", u[my, nl], k2[nl], shp[nl], shm[nl]),
           ("deriv_real", u[my, nl] real, order), ("lap_real", u[my, nl] real, k2[nl]),
-          ("filter", u[my, nl], shp[nl], shm[nl]) or ("filter_real", u[my, nl] real).
+          ("filter", u[my, nl], shp[nl], shm[nl]) or ("filter_real", u[my, nl] real),
+          ("deriv_reuse", u[my, nl], ua[my, nl], shp_a[nl], shp_b[nl], shm_b[nl], order):
+          per line, derivyc of ua with ias=1 at shp_a, then of u with ias=0 at
+          shp_b, shm_b (the second result is returned).
     Returns one array [my, nl] per job.
     """
     ctx = mp.get_context("spawn")

@@ -19,8 +19,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# SNF_LIB selects another build of the same library (A/B timing of variants)
-LIB_PATH = os.environ.get("SNF_LIB") or os.path.join(_HERE, "libshear_rk3.so")
+LIB_PATH = os.path.join(_HERE, "libshear_rk3.so")
 
 SNF_OK, SNF_BAD_PARAM, SNF_DEVICE_ERROR, SNF_NO_DEVICE, SNF_NOT_READY = 0, 1, 2, 3, 4
 SNF_RUNNING, SNF_STOP, SNF_DUMP = 0, 1, 2
@@ -127,6 +126,12 @@ def _c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
 
 
+def _need(a, shape, name):
+    """the C side reads exactly this many elements: refuse anything else"""
+    if tuple(np.shape(a)) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(np.shape(a))}, expected {tuple(shape)}")
+
+
 def advance_batch(boxes, nstep):
     """Advances every ShearBox of `boxes` by up to nstep steps with all their
     work in flight at once (one stream each); returns the statuses and
@@ -191,7 +196,10 @@ class ShearBox:
         vor, phi = _c128(vor), _c128(phi)
         u00 = np.ascontiguousarray(u00, dtype=np.float64)
         w00 = np.ascontiguousarray(w00, dtype=np.float64)
-        assert vor.shape == (self.my, self.kxn, self.mz) and phi.shape == vor.shape
+        for a, nm in ((vor, "vor"), (phi, "phi")):
+            _need(a, (self.my, self.kxn, self.mz), nm)
+        for a, nm in ((u00, "u00"), (w00, "w00")):
+            _need(a, (self.my,), nm)
         self._check(self.lib.snf_state_upload_ykxkz(self.h, _ptr(vor.view(np.float64)),
                                                     _ptr(phi.view(np.float64)), _ptr(u00),
                                                     _ptr(w00), time, xoffb, xofft))
@@ -200,10 +208,14 @@ class ShearBox:
         """vor, phi in the reference layout complex(0:my-1, 0:mx1, 0:mz-1)
         (Fortran order, i.e. numpy shape (mz, kxn, my) C-contiguous)"""
         vor, phi = _c128(vor), _c128(phi)
+        u00 = np.ascontiguousarray(u00, dtype=np.float64)
+        w00 = np.ascontiguousarray(w00, dtype=np.float64)
+        for a, nm in ((vor, "vor"), (phi, "phi")):
+            _need(a, (self.mz, self.kxn, self.my), nm)
+        for a, nm in ((u00, "u00"), (w00, "w00")):
+            _need(a, (self.my,), nm)
         self._check(self.lib.snf_state_upload(self.h, _ptr(vor.view(np.float64)),
-                                              _ptr(phi.view(np.float64)),
-                                              _ptr(np.ascontiguousarray(u00, dtype=np.float64)),
-                                              _ptr(np.ascontiguousarray(w00, dtype=np.float64)),
+                                              _ptr(phi.view(np.float64)), _ptr(u00), _ptr(w00),
                                               time, xoffb, xofft))
 
     def get_state(self):
@@ -289,6 +301,7 @@ class ShearBox:
     def shoot(self, x, T, t0=0.0, xoffb0=0.0, xofft0=0.0):
         """Phi_T(x): the shooting map of calc_dns_disk3d (gmres_shear.f90:2951-3120)"""
         x = np.ascontiguousarray(x, np.float64)
+        _need(x, (self.state_len(),), "x")
         y = np.empty_like(x)
         st = SnfRunStatus()
         self._check(self.lib.snf_shoot(self.h, _ptr(x), t0, xoffb0, xofft0, T, _ptr(y),
@@ -298,15 +311,26 @@ class ShearBox:
     def jvp_batch(self, x0, dirs, eps, T, y0=None, t0=0.0, xoffb0=0.0, xofft0=0.0):
         """(Phi_T(x0 + eps_d d) - Phi_T(x0)) / eps_d for the rows d of dirs
         (matvec_dns_disk3d, gmres_shear.f90:2666-2730)"""
+        n = self.state_len()
         x0 = np.ascontiguousarray(x0, np.float64)
         dirs = np.ascontiguousarray(np.atleast_2d(dirs), np.float64)
+        _need(x0, (n,), "x0")
+        _need(dirs, (dirs.shape[0], n), "dirs")
         eps = np.ascontiguousarray(np.broadcast_to(eps, (dirs.shape[0],)), np.float64)
         out = np.empty_like(dirs)
         y0a = np.ascontiguousarray(y0, np.float64) if y0 is not None else None
+        if y0a is not None:
+            _need(y0a, (n,), "y0")
         self._check(self.lib.snf_jvp_batch(self.h, _ptr(x0), _ptr(y0a) if y0a is not None else None,
                                            _ptr(dirs), dirs.shape[0], _ptr(eps), t0, xoffb0,
                                            xofft0, T, _ptr(out)))
         return out
+
+    def state_len(self):
+        """doubles in a state vector [vor | phi | u00 | w00]"""
+        n = ctypes.c_int64()
+        self._check(self.lib.snf_state_len(self.h, ctypes.byref(n)))
+        return n.value
 
     def set_timing(self, on):
         self._check(self.lib.snf_set_timing(self.h, 1 if on else 0))
@@ -324,9 +348,12 @@ class ShearBox:
     def derivyc(self, u, shp, shm, order):
         """u: (my, nl) complex lines; returns d^order u / dy^order"""
         u = _c128(u)
+        _need(u, (self.my, u.shape[1] if u.ndim == 2 else -1), "u")
         nl = u.shape[1]
         out = np.empty_like(u)
         shp, shm = _c128(shp), _c128(shm)
+        _need(shp, (nl,), "shp")
+        _need(shm, (nl,), "shm")
         self._check(self.lib.snf_op_derivy(self.h, _ptr(u.view(np.float64)),
                                            _ptr(out.view(np.float64)), nl,
                                            _ptr(shp.view(np.float64)), _ptr(shm.view(np.float64)),
@@ -335,10 +362,13 @@ class ShearBox:
 
     def lapcdy(self, u, k2, shp, shm):
         u = _c128(u)
+        _need(u, (self.my, u.shape[1] if u.ndim == 2 else -1), "u")
         nl = u.shape[1]
         out = np.empty_like(u)
         k2 = np.ascontiguousarray(k2, dtype=np.float64)
         shp, shm = _c128(shp), _c128(shm)
+        for a, nm in ((k2, "k2"), (shp, "shp"), (shm, "shm")):
+            _need(a, (nl,), nm)
         self._check(self.lib.snf_op_helmholtz(self.h, _ptr(u.view(np.float64)),
                                               _ptr(out.view(np.float64)), nl, _ptr(k2),
                                               _ptr(shp.view(np.float64)),
@@ -350,11 +380,13 @@ class ShearBox:
         direction=-1: the reverse."""
         if direction == 1:
             a = _c128(a)
+            _need(a, (a.shape[0] if a.ndim == 3 else -1, self.kxn, self.mz), "a")
             out = np.empty((a.shape[0], self.mgalx, self.mgalz))
             self._check(self.lib.snf_op_fourxz(self.h, _ptr(a.view(np.float64)), _ptr(out),
                                                a.shape[0], 1))
         else:
             a = np.ascontiguousarray(a, dtype=np.float64)
+            _need(a, (a.shape[0] if a.ndim == 3 else -1, self.mgalx, self.mgalz), "a")
             out = np.empty((a.shape[0], self.kxn, self.mz), dtype=np.complex128)
             self._check(self.lib.snf_op_fourxz(self.h, _ptr(a), _ptr(out.view(np.float64)),
                                                a.shape[0], -1))
@@ -365,6 +397,7 @@ class ShearBox:
         of the current offsets: iopt=+1 phi -> v, iopt=-1 v -> phi.  The
         (0,0) line of the result is zero."""
         f = _c128(f)
+        _need(f, (self.my, self.kxn, self.mz), "f")
         out = np.zeros_like(f)
         self._check(self.lib.snf_op_getv(self.h, _ptr(f.view(np.float64)),
                                          _ptr(out.view(np.float64)), int(iopt)))
@@ -380,6 +413,8 @@ class ShearBox:
     def hvhg(self, v, dvdy, dvordy):
         """Nonlinear terms of the uploaded state given v, dv/dy, d(omega_y)/dy."""
         v, dvdy, dvordy = _c128(v), _c128(dvdy), _c128(dvordy)
+        for a, nm in ((v, "v"), (dvdy, "dvdy"), (dvordy, "dvordy")):
+            _need(a, (self.my, self.kxn, self.mz), nm)
         o1 = np.empty_like(v)
         o2 = np.empty_like(v)
         o3 = np.empty_like(v)

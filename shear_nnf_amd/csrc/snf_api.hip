@@ -9,7 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/shear_rk3.h"
@@ -51,6 +54,19 @@ struct DevBuf {
   hipError_t alloc(size_t n) { return hipMalloc((void**)&p, n * sizeof(T)); }
   operator T*() const { return p; }
 };
+
+// hipFuncAttributeMaxDynamicSharedMemorySize is a per-kernel setting of the
+// process, shared by every handle on a device: raise it to the largest LDS
+// any live handle launches the kernel with, never lower it.
+void lds_attr(const void* k, size_t bytes, int dev) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> set;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = set[std::make_pair(k, dev)];
+  if (bytes <= cur) return;
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+    cur = bytes;
+}
 
 bool plan_of(int n, FFTPlan& p) {
   p.n = n;
@@ -754,7 +770,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
           (const void*)k_yf_filter<YFFT<7, 7>>, (const void*)k_yf_filter<YFFT<11, 7, 5>>,
           (const void*)k_yf_filter<YDirect<1>>, (const void*)k_yf_filter<YDirect<2>>,
           (const void*)k_yf_filter<YDirect<3>>})
-      (void)hipFuncSetAttribute(ky, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsY);
+      lds_attr(ky, h->ldsY, h->dev);
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
   // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning).
@@ -813,9 +829,9 @@ int snf_create(const snf_params* p, snf_handle** out) {
                            (const void*)k_xzw_C<4, cplxf>, (const void*)k_xzw_C<6, cplxf>,
                             (const void*)k_xzw_C<8, cplx>, (const void*)k_xzw_C<8, cplxf>};
       for (int q = 0; q < 10; ++q) {
-        (void)hipFuncSetAttribute(ka[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWA);
-        (void)hipFuncSetAttribute(kb[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWB);
-        (void)hipFuncSetAttribute(kc[q], hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsWC);
+        lds_attr(ka[q], h->ldsWA, h->dev);
+        lds_attr(kb[q], h->ldsWB, h->dev);
+        lds_attr(kc[q], h->ldsWC, h->dev);
       }
     }
   }
@@ -824,16 +840,15 @@ int snf_create(const snf_params* p, snf_handle** out) {
   h->ldsA = 2 * 3 * (size_t)mgalz * es;   // ping-pong, 3 sequences
   h->ldsB = 3 * (size_t)h->zb * mgalx * es;
   h->ldsC = 2 * 3 * (size_t)mgalz * es;
-  (void)hipFuncSetAttribute((const void*)k_ydealias, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(3 * (size_t)my * sizeof(cplx)));
+  lds_attr((const void*)k_ydealias, 3 * (size_t)my * sizeof(cplx), h->dev);
   for (const void* k : {(const void*)k_xz_A<cplx>, (const void*)k_xz_A<cplxf>})
-    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsA);
+    lds_attr(k, h->ldsA, h->dev);
   for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
                          (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
                          (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>})
-    (void)hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsB);
+    lds_attr(kb, h->ldsB, h->dev);
   for (const void* k : {(const void*)k_xz_C<cplx>, (const void*)k_xz_C<cplxf>})
-    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->ldsC);
+    lds_attr(k, h->ldsC, h->dev);
 #undef TRY
   *out = h;
   return SNF_OK;
@@ -981,14 +996,54 @@ struct AdvanceMark {
   int done0, nrec0, resume;
 };
 
+// The UNER records wait on the device in a ring until snf_history fetches
+// them (record n in slot n % cap).  Before up to nstep more steps run, the
+// ring grows to hold every unread record plus the ones those steps can
+// write, so no record is ever overwritten before it is read.  The captured
+// step graphs carry the ring's address and size, so they are rebuilt.
+static int reserve_records(snf_handle* h, int nrec_dev, int nstep) {
+  const long unread = (long)nrec_dev - h->nrec_read;
+  const long need = unread + nstep / h->p.nhist + 2;
+  if (need <= h->cap) return SNF_OK;
+  if (need > (1L << 30) / 24) return fail(SNF_BAD_PARAM, "too many unread UNER records");
+  long ncap = 2L * h->cap;
+  while (ncap < need) ncap *= 2;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> old((size_t)h->cap * 24), nw((size_t)ncap * 24, 0.0);
+  HIPCHK(hipMemcpy(old.data(), h->ring, old.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (long r = 0; r < unread; ++r) {
+    const long n = h->nrec_read + r;
+    std::memcpy(&nw[(size_t)(n % ncap) * 24], &old[(size_t)(n % h->cap) * 24], 24 * sizeof(double));
+  }
+  double* ring = nullptr;
+  HIPCHK(hipMalloc((void**)&ring, nw.size() * sizeof(double)));
+  const hipError_t e = hipMemcpy(ring, nw.data(), nw.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(ring);
+    return fail(SNF_DEVICE_ERROR, std::string("ring copy: ") + hipGetErrorString(e));
+  }
+  HIPCHK(hipFree(h->ring));
+  h->dev_bytes += (size_t)(ncap - h->cap) * 24 * sizeof(double);
+  h->ring = ring;
+  h->cap = (int)ncap;
+  for (hipGraphExec_t& ge : h->gexec)
+    if (ge) {
+      (void)hipGraphExecDestroy(ge);
+      ge = nullptr;
+    }
+  return SNF_OK;
+}
+
 // continue_call: this chunk continues the cross call of the previous one
 // (the shooting map runs one cross call in chunks of steps)
-static int advance_begin(snf_handle* h, bool continue_call, AdvanceMark& m) {
+static int advance_begin(snf_handle* h, bool continue_call, int nstep, AdvanceMark& m) {
   HIPCHK(hipSetDevice(h->dev));
   DevScalars s0{};
   HIPCHK(hipMemcpy(&s0, h->sc, sizeof(s0), hipMemcpyDeviceToHost));
   m.done0 = s0.steps_done;
   m.nrec0 = s0.nrec;
+  int rc;
+  if ((rc = reserve_records(h, s0.nrec, nstep))) return rc;
   // a call that follows a return at a dump continues that run; any other
   // call is a new cross call
   m.resume = continue_call || (s0.halted && s0.idump && !s0.istop);
@@ -1099,7 +1154,7 @@ static int advance_finish(snf_handle* h, const AdvanceMark& m, snf_run_status* s
 static int advance_impl(snf_handle* h, int32_t nstep, snf_run_status* st, bool continue_call) {
   AdvanceMark m{};
   int rc;
-  if ((rc = advance_begin(h, continue_call, m))) return rc;
+  if ((rc = advance_begin(h, continue_call, nstep, m))) return rc;
   if ((rc = advance_enqueue(h, nstep, m.resume))) return rc;
   return advance_finish(h, m, st);
 }
@@ -1122,7 +1177,7 @@ int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_s
   std::vector<AdvanceMark> m((size_t)n);
   int rc;
   for (int i = 0; i < n; ++i)
-    if ((rc = advance_begin(hs[i], false, m[i]))) return rc;
+    if ((rc = advance_begin(hs[i], false, nstep, m[i]))) return rc;
   for (int i = 0; i < n; ++i)
     if ((rc = advance_enqueue(hs[i], nstep, m[i].resume))) return rc;
   for (int i = 0; i < n; ++i)
@@ -1156,6 +1211,9 @@ static int shoot_many(snf_handle* const* hs, int n, const double* const* x, doub
     if ((rc = snf_reset_statistics(h))) return rc;
     DevScalars s{};
     HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
+    // a shooting starts a new record stream: unread records of earlier runs
+    // are dropped (the Newton drivers never read them)
+    h->nrec_read = s.nrec;
     s.etime = t0 + T;
     s.dumptime = 1e300;
     s.istop = s.idump = s.halted = 0;
@@ -1170,7 +1228,7 @@ static int shoot_many(snf_handle* const* hs, int n, const double* const* x, doub
     // the begins read device scalars with blocking copies: all of them
     // before any handle's steps are in flight
     for (int q = 0; q < n; ++q)
-      if (live[q] && (rc = advance_begin(hs[q], c > 0, m[q]))) return rc;
+      if (live[q] && (rc = advance_begin(hs[q], c > 0, chunk, m[q]))) return rc;
     for (int q = 0; q < n; ++q)
       if (live[q] && (rc = advance_enqueue(hs[q], chunk, m[q].resume))) return rc;
     for (int q = 0; q < n; ++q)
@@ -1265,11 +1323,9 @@ int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n) {
   HIPCHK(hipStreamSynchronize(h->stream));
   DevScalars s{};
   HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
-  int avail = s.nrec - h->nrec_read;
-  if (avail > h->cap) {
-    h->nrec_read = s.nrec - h->cap;
-    avail = h->cap;
-  }
+  const int avail = s.nrec - h->nrec_read;
+  // the advances grow the ring before it could wrap (reserve_records)
+  if (avail > h->cap) return fail(SNF_DEVICE_ERROR, "UNER record ring overflowed");
   const int cnt = avail < max_records ? avail : max_records;
   std::vector<double> ring((size_t)h->cap * 24);
   HIPCHK(hipMemcpy(ring.data(), h->ring, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -1498,15 +1554,34 @@ int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const do
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
   const size_t NC = h->NC;
+  // The evaluation takes v, dv/dy, d(omega_y)/dy in the handle's own
+  // buffers.  The next advance re-forms them from the state except on the
+  // (0,0) line, which no y stage writes and the dissipation sum reads: keep
+  // those three lines.
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<cplx> keep00(3 * (size_t)g.my);
+  cplx* bufs[3] = {h->v, h->dv, h->dome};
+  for (int f = 0; f < 3; ++f)
+    HIPCHK(hipMemcpy2D(keep00.data() + (size_t)f * g.my, sizeof(cplx), bufs[f], g.P * sizeof(cplx),
+                       sizeof(cplx), g.my, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(h->v, hg_v, NC * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dv, hv_dv, NC * sizeof(cplx), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dome, dvordy, NC * sizeof(cplx), hipMemcpyHostToDevice));
+  // the run scalars of the handle (a cross call that returned at a dump
+  // continues after this evaluation): k_begin_call below resets them
+  DevScalars keep{};
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&keep, h->sc, sizeof(keep), hipMemcpyDeviceToHost));
   // U, W derivatives as at the top of hvhg (cross.f90:1198-1205)
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], 0);
   k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
   int rc;
   if ((rc = launch_xz(h, 2))) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(h->sc, &keep, sizeof(keep), hipMemcpyHostToDevice));
+  for (int f = 0; f < 3; ++f)
+    HIPCHK(hipMemcpy2D(bufs[f], g.P * sizeof(cplx), keep00.data() + (size_t)f * g.my, sizeof(cplx),
+                       sizeof(cplx), g.my, hipMemcpyHostToDevice));
   if (out_dvordy) HIPCHK(hipMemcpy(out_dvordy, h->rdome, NC * sizeof(cplx), hipMemcpyDeviceToHost));
   if (out_hg) HIPCHK(hipMemcpy(out_hg, h->rhg, NC * sizeof(cplx), hipMemcpyDeviceToHost));
   if (out_hv) HIPCHK(hipMemcpy(out_hv, h->rhv, NC * sizeof(cplx), hipMemcpyDeviceToHost));

@@ -67,3 +67,26 @@ def test_compact_filter(my):
     ops = YOps(my, dy)
     assert rel_l2(ops.filter(u, shp, np.conj(shp)), f) < TOL
     assert rel_l2(ops.filter_real(ur), fr) < TOL
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_factor_reuse_ias0(order):
+    """ias=0 (diffy_shear_hepta.f90:389-406, 431-445) reuses the factor of the
+    previous ias=1 call; the reference pairs the calls with the same twist
+    (cross.f90:440-442, 659-660), where the oracle's one solve per twist gives
+    the same numbers.  Reusing a factor of another twist does not: the
+    reference really skips the refactor."""
+    my = 49
+    rng = np.random.default_rng(40 + order)
+    nl = 6
+    dy = 1.0 / my
+    u = rng.normal(size=(my, nl)) + 1j * rng.normal(size=(my, nl))
+    ua = rng.normal(size=(my, nl)) + 1j * rng.normal(size=(my, nl))
+    shp = np.exp(1j * rng.uniform(-math.pi, math.pi, nl))
+    other = np.exp(1j * rng.uniform(-math.pi, math.pi, nl))
+    same, mixed = ref.run(my, dy, [("deriv_reuse", u, ua, shp, shp, np.conj(shp), order),
+                                   ("deriv_reuse", u, ua, other, shp, np.conj(shp), order)])
+    ops = YOps(my, dy)
+    want = ops.deriv(u, shp, np.conj(shp), order)
+    assert rel_l2(same, want) < TOL
+    assert rel_l2(mixed, want) > 1e-6
