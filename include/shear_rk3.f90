@@ -171,6 +171,12 @@ module snf_gpu
        type(c_ptr), value :: h
        real(c_double), intent(out) :: energ(4)
      end function snf_energy
+     ! no counterpart in the reference: snf_energy allocating no device memory
+     integer(c_int) function snf_energy_noalloc(h, energ) bind(C, name="snf_energy_noalloc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: energ(4)
+     end function snf_energy_noalloc
      integer(c_int) function snf_op_fourxz(h, fin, fout, nplanes, dir) bind(C, name="snf_op_fourxz")
        import :: c_int, c_ptr, c_int32_t, c_double
        type(c_ptr), value :: h

@@ -194,6 +194,17 @@ int snf_op_getv(snf_handle* h, const double* in, double* out, int32_t dir);
 /* energy (operators.f90:870-924) of the device state: out[0..3] = sums of
  * |vor|^2, |phi|^2 (kx > 0 counted twice), U^2, W^2.                    */
 int snf_energy(snf_handle* h, double* out);
+/* The same sums as snf_energy, from the same kernels (bitwise equal
+ * results), with no device memory allocated or freed by the call: no
+ * counterpart in the reference; for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The partial sums live in a static device array of 16,416
+ * bytes (2 x 1024 partials + 4 sums) that every GPU the library runs on
+ * holds until the process ends; its size does not depend on the box, so no
+ * box exceeds it.  Calls on one GPU share that array: the library
+ * serialises them within a process, and every partial is written before it
+ * is read, so a call depends on no earlier call's contents.            */
+int snf_energy_noalloc(snf_handle* h, double* out);
 /* fourxz (fou3D_thread.f:22-73) on nplanes planes: dir=+1 spectral
  * (kxn x mz complex, [kx][kz]) -> physical (mgalx x mgalz, [x][z]);
  * dir=-1 the reverse with the 1/(mgalx mgalz) scaling.                   */

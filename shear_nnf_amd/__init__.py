@@ -32,7 +32,7 @@ EXPORTS = (
     "snf_op_derivy", "snf_op_helmholtz", "snf_op_fourxz", "snf_op_hvhg",
     "snf_set_timing", "snf_get_timing", "snf_copy_probe", "snf_op_getv", "snf_energy",
     "snf_spectra", "snf_reset_statistics", "snf_state_len", "snf_shoot", "snf_jvp_batch",
-    "snf_advance_batch",
+    "snf_advance_batch", "snf_energy_noalloc",
 )
 KERNEL_CLASSES = ("xz_A", "xz_B", "xz_C", "y_lines", "small")
 
@@ -97,6 +97,7 @@ def load_library(path=LIB_PATH):
     lib.snf_copy_probe.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, dp]
     lib.snf_op_getv.argtypes = [H, dp, dp, ctypes.c_int32]
     lib.snf_energy.argtypes = [H, dp]
+    lib.snf_energy_noalloc.argtypes = [H, dp]
     lib.snf_spectra.argtypes = [H, dp, dp]
     lib.snf_reset_statistics.argtypes = [H]
     lib.snf_state_len.argtypes = [H, P(ctypes.c_int64)]
@@ -408,6 +409,13 @@ class ShearBox:
         |vor|^2, |phi|^2 (kx > 0 counted twice), U^2, W^2."""
         out = np.empty(4)
         self._check(self.lib.snf_energy(self.h, _ptr(out)))
+        return out
+
+    def energy_noalloc(self):
+        """energy() through snf_energy_noalloc: no device allocation, the
+        partial sums in the library's static scratch"""
+        out = np.empty(4)
+        self._check(self.lib.snf_energy_noalloc(self.h, _ptr(out)))
         return out
 
     def hvhg(self, v, dvdy, dvordy):

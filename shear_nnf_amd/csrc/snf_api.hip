@@ -25,6 +25,16 @@
 #include "snf_yfk.hip"
 #include "snf_ydeal.hip"
 
+// Device scratch of snf_energy_noalloc, the entry point that allocates
+// nothing: the per-block partial sums of k_energy_part (2 doubles a block)
+// and the 4 sums of k_energy_final.  Every GPU the library runs on holds one
+// copy of it (16.4 KB) until the process ends.  Host code never passes the
+// array's name to a kernel: it takes the array's device address once per
+// device (scratch_addr) and passes that.
+#define SNF_ENERGY_BLOCKS 1024
+#define SNF_SCRATCH_DOUBLES (2 * SNF_ENERGY_BLOCKS + 4)
+static __device__ __align__(256) double scratch[SNF_SCRATCH_DOUBLES];
+
 namespace {
 
 thread_local std::string g_err;
@@ -66,6 +76,33 @@ void lds_attr(const void* k, size_t bytes, int dev) {
   if (bytes <= cur) return;
   if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
     cur = bytes;
+}
+
+// The device address of `scratch` on each device (each GPU holds its own
+// copy at its own address), looked up once with that device current, and a
+// lock per device that serialises the calls sharing it.
+struct ScratchSlot {
+  double* ptr = nullptr;
+  std::mutex call;
+};
+std::mutex g_scratch_mu;
+std::map<int, ScratchSlot> g_scratch;
+
+// dev must be the current device
+int scratch_addr(int dev, ScratchSlot** slot) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchSlot& sl = g_scratch[dev];
+  if (!sl.ptr) {
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(SNF_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    sl.ptr = p;
+  }
+  *slot = &sl;
+  return SNF_OK;
 }
 
 bool plan_of(int n, FFTPlan& p) {
@@ -577,6 +614,14 @@ int snf_create(const snf_params* p, snf_handle** out) {
   if (hipSetDevice(h->dev) != hipSuccess) {
     delete h;
     return fail(SNF_NO_DEVICE, "hipSetDevice failed");
+  }
+  {   // the no-allocation entry point's scratch on this device
+    ScratchSlot* sl = nullptr;
+    const int rs = scratch_addr(h->dev, &sl);
+    if (rs != SNF_OK) {
+      delete h;
+      return rs;
+    }
   }
   Geom& g = h->g;
   g.my = my;
@@ -1459,6 +1504,28 @@ int snf_energy(snf_handle* h, double* out) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  return SNF_OK;
+}
+
+int snf_energy_noalloc(snf_handle* h, double* out) {
+  if (!h || !out) return fail(SNF_BAD_PARAM, "null argument");
+  HIPCHK(hipSetDevice(h->dev));
+  const int nb = SNF_ENERGY_BLOCKS;
+  // partials of nb blocks and the 4 sums must fit the static scratch
+  if (2 * (size_t)nb + 4 > (size_t)SNF_SCRATCH_DOUBLES)
+    return fail(SNF_BAD_PARAM, "snf_energy_noalloc: partial sums exceed the static scratch");
+  ScratchSlot* sl = nullptr;
+  int rc;
+  if ((rc = scratch_addr(h->dev, &sl))) return rc;
+  std::lock_guard<std::mutex> lk(sl->call);
+  // every partial and every sum is written before it is read: nothing to clear
+  double* part = sl->ptr;
+  double* res = sl->ptr + 2 * nb;
+  k_energy_part<<<nb, 256, 0, h->stream>>>(h->g, h->vor, h->phi, part);
+  k_energy_final<<<1, 256, 0, h->stream>>>(part, nb, h->u00c, h->w00c, h->g.my, res);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, res, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return SNF_OK;
 }
 
