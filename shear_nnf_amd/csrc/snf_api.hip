@@ -24,6 +24,7 @@
 #include "snf_ykern.hip"
 #include "snf_yfk.hip"
 #include "snf_ydeal.hip"
+#include "snf_batch.hip"
 
 // Device scratch of snf_energy_noalloc, the entry point that allocates
 // nothing: the per-block partial sums of k_energy_part (2 doubles a block)
@@ -273,14 +274,6 @@ __global__ void k_copy_probe(const double2* __restrict__ a, double2* __restrict_
     b[q] = a[q];
 }
 
-__global__ void k_cplx_to_real(const cplx* a, const cplx* b, double* ra, double* rb, int n) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) {
-    ra[j] = a[j].x;
-    rb[j] = b[j].x;
-  }
-}
-
 // bracket a launch with events when timing is on (pairs: start, stop)
 struct TimedLaunch {
   snf_handle* h;
@@ -459,6 +452,52 @@ ScalarArgs sargs(snf_handle* h) {
   return a;
 }
 
+// ---- y-transform plans
+// The radix plan of each line length with one (the template arguments of the
+// spectral y kernels, pass order), and the direct DFT for other line
+// lengths up to Y_DIRECT_MAX.  with_yplan calls f with a tag of the plan of
+// my; y_plan_radices gives the same radices to the host twiddle table.
+template <class PL> struct PlanTag { typedef PL type; };
+#define Y_DIRECT_MAX 144
+template <class F>
+void with_yplan(int my, F&& f) {
+  switch (my) {
+    case 32: f(PlanTag<YFFT<8, 4>>{}); break;
+    case 33: f(PlanTag<YFFT<11, 3>>{}); break;
+    case 48: f(PlanTag<YFFT<4, 4, 3>>{}); break;
+    case 49: f(PlanTag<YFFT<7, 7>>{}); break;
+    case 64: f(PlanTag<YFFT<8, 8>>{}); break;
+    case 65: f(PlanTag<YFFT<13, 5>>{}); break;
+    case 96: f(PlanTag<YFFT<8, 4, 3>>{}); break;
+    case 128: f(PlanTag<YFFT<8, 4, 4>>{}); break;
+    case 192: f(PlanTag<YFFT<8, 8, 3>>{}); break;
+    case 256: f(PlanTag<YFFT<8, 8, 4>>{}); break;
+    case 385: f(PlanTag<YFFT<11, 7, 5>>{}); break;
+    default:
+      switch ((my + 63) / 64) {
+        case 1: f(PlanTag<YDirect<1>>{}); break;
+        case 2: f(PlanTag<YDirect<2>>{}); break;
+        default: f(PlanTag<YDirect<3>>{}); break;
+      }
+  }
+}
+int y_plan_radices(int my, int* R) {
+  static const int plans[][5] = {{32, 8, 4, 0, 0},   {33, 11, 3, 0, 0}, {48, 4, 4, 3, 0},
+                                 {49, 7, 7, 0, 0},   {64, 8, 8, 0, 0},  {65, 13, 5, 0, 0},
+                                 {96, 8, 4, 3, 0},   {128, 8, 4, 4, 0}, {192, 8, 8, 3, 0},
+                                 {256, 8, 8, 4, 0},  {385, 11, 7, 5, 0}};
+  for (const auto& pl : plans)
+    if (pl[0] == my) {
+      int np = 0;
+      while (np < 4 && pl[1 + np]) {
+        R[np] = pl[1 + np];
+        ++np;
+      }
+      return np;
+    }
+  return 0;
+}
+
 // the spectral y stage of one substep: explicit or implicit viscous terms
 template <class PL>
 void launch_yf_plan(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
@@ -475,33 +514,17 @@ void launch_yf_first_plan(snf_handle* h, unsigned nb, const YArgs& a) {
 void launch_yf_first(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
-  switch (h->g.my) {
-    case 33: launch_yf_first_plan<YFFT<11, 3>>(h, nb, a); break;
-    case 49: launch_yf_first_plan<YFFT<7, 7>>(h, nb, a); break;
-    case 385: launch_yf_first_plan<YFFT<11, 7, 5>>(h, nb, a); break;
-    default:
-      switch ((h->g.my + 63) / 64) {
-        case 1: launch_yf_first_plan<YDirect<1>>(h, nb, a); break;
-        case 2: launch_yf_first_plan<YDirect<2>>(h, nb, a); break;
-        default: launch_yf_first_plan<YDirect<3>>(h, nb, a); break;
-      }
-  }
+  with_yplan(h->g.my, [&](auto tag) {
+    launch_yf_first_plan<typename decltype(tag)::type>(h, nb, a);
+  });
 }
 
 void launch_yf(snf_handle* h, int rk) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
-  switch (h->g.my) {
-    case 33: launch_yf_plan<YFFT<11, 3>>(h, rk, nb, a); break;
-    case 49: launch_yf_plan<YFFT<7, 7>>(h, rk, nb, a); break;
-    case 385: launch_yf_plan<YFFT<11, 7, 5>>(h, rk, nb, a); break;
-    default:   // direct DFT, my <= Y_DIRECT_MAX
-      switch ((h->g.my + 63) / 64) {
-        case 1: launch_yf_plan<YDirect<1>>(h, rk, nb, a); break;
-        case 2: launch_yf_plan<YDirect<2>>(h, rk, nb, a); break;
-        default: launch_yf_plan<YDirect<3>>(h, rk, nb, a); break;
-      }
-  }
+  with_yplan(h->g.my, [&](auto tag) {
+    launch_yf_plan<typename decltype(tag)::type>(h, rk, nb, a);
+  });
 }
 
 // iydealiasing=3: compact filter of vor, phi, U, W at the current twist
@@ -527,19 +550,32 @@ void launch_ydealias(snf_handle* h) {
   k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
 }
 
+// Line lengths without a spectral y stage (the banded sweeps run the
+// substeps) filter through the direct DFT of up to 64 Q points, in its own
+// LDS plan: the transfer tile, a column per wave and the roots (9 my complex).
+template <class F>
+void with_filter_direct(int my, F&& f) {
+  if (my <= 320)
+    f(PlanTag<YDirect<5>>{});
+  else if (my <= 576)
+    f(PlanTag<YDirect<9>>{});
+  else
+    f(PlanTag<YDirect<17>>{});
+}
+inline size_t filter_direct_lds(int my) { return (size_t)my * (2 * YF_LINES + 1) * sizeof(cplx); }
+
 void launch_yfilter(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
-  switch (h->g.my) {
-    case 33: k_yf_filter<YFFT<11, 3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-    case 49: k_yf_filter<YFFT<7, 7>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-    case 385: k_yf_filter<YFFT<11, 7, 5>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-    default:
-      switch ((h->g.my + 63) / 64) {
-        case 1: k_yf_filter<YDirect<1>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-        case 2: k_yf_filter<YDirect<2>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-        default: k_yf_filter<YDirect<3>><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a); break;
-      }
+  if (h->yfft) {
+    with_yplan(h->g.my, [&](auto tag) {
+      k_yf_filter<typename decltype(tag)::type><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
+    });
+  } else {
+    const size_t lds = filter_direct_lds(h->g.my);
+    with_filter_direct(h->g.my, [&](auto tag) {
+      k_yf_filter<typename decltype(tag)::type><<<nb, 64 * YF_LINES, lds, h->stream>>>(a);
+    });
   }
 }
 
@@ -560,6 +596,306 @@ int launch_twist(snf_handle* h, int which, bool tails = true) {
 
 unsigned ylines_blocks(const snf_handle* h) { return (unsigned)((h->P + 2 + 127) / 128); }
 
+// the Stockham x-row kernel with nt threads and zb z-rows per block fits
+// its LDS and per-thread plans
+bool xz_b_fits(const snf_handle* h, int nt, int zb) {
+  const int mgalx = h->g.mgalx, mgalz = h->g.mgalz;
+  return zb >= 2 && mgalz % zb == 0 && 3 * (size_t)zb * mgalx * sizeof(cplx) <= 156 * 1024 &&
+         (zb / 2) * mgalx <= XZ_B_MAXW * nt && fft_max_values(h->px, 3 * zb, nt) <= XZ_MAXV;
+}
+
+// ---- box tables of the batched ensemble (kernels in snf_batch.hip)
+// Device tables of one size class, row [substep][box] (the step-begin table
+// [step][box]), for the 6 substeps of the two steps a graph replays.
+struct BoxTables {
+  void* blob = nullptr;
+  BxStep* step = nullptr;
+  BxC2R* c2r = nullptr;
+  BxXzA* A = nullptr;
+  BxHist* hf = nullptr;
+  BxHist* hs = nullptr;
+  BxXzB* B = nullptr;
+  BxXzC* C = nullptr;
+  BxPhist* ph = nullptr;
+  ScalarArgs* sc = nullptr;
+  BxYsym* ys = nullptr;
+  YArgs* y = nullptr;
+};
+
+template <class H>
+void tab_A(const snf_handle* h, const BxXzA* t, int n, hipStream_t s) {
+  lds_attr((const void*)k_xz_A_tab<H>, h->ldsA, h->dev);
+  if (n == 0) return;   // the LDS limit only (set before a graph capture)
+  k_xz_A_tab<H><<<dim3(h->g.kxn, h->g.my, n), 256, h->ldsA, s>>>(t);
+}
+template <class H>
+void tab_C(const snf_handle* h, const BxXzC* t, int n, hipStream_t s) {
+  lds_attr((const void*)k_xz_C_tab<H>, h->ldsC, h->dev);
+  if (n == 0) return;
+  k_xz_C_tab<H><<<dim3(h->g.kxn, h->g.my, n), 256, h->ldsC, s>>>(t);
+}
+template <int NT, class H>
+void tab_B_nt(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
+  lds_attr((const void*)k_xz_B_tab<NT, H>, h->ldsB, h->dev);
+  if (n == 0) return;
+  k_xz_B_tab<NT, H><<<dim3(h->g.mgalz / h->zb, h->g.my, n), NT, h->ldsB, s>>>(t, rk);
+}
+template <class H>
+void tab_B(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
+  if (h->bthreads == 256)
+    tab_B_nt<256, H>(h, t, n, rk, s);
+  else if (h->bthreads == 512)
+    tab_B_nt<512, H>(h, t, n, rk, s);
+  else
+    tab_B_nt<1024, H>(h, t, n, rk, s);
+}
+template <class PL>
+void tab_y_plan(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
+  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
+  if (h->p.explicit_visc == 1) {
+    lds_attr((const void*)k_yf_explicit_tab<PL>, h->ldsY, h->dev);
+    if (n == 0) return;
+    k_yf_explicit_tab<PL><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
+  } else {
+    lds_attr((const void*)k_yf_implicit_tab<PL>, h->ldsY, h->dev);
+    if (n == 0) return;
+    k_yf_implicit_tab<PL><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
+  }
+}
+void tab_y(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
+  with_yplan(h->g.my, [&](auto tag) { tab_y_plan<typename decltype(tag)::type>(h, t, n, rk, s); });
+}
+
+// One RK3 step (st = 0 or 1 of the pair the tables hold) of every box of a
+// size class: the launches of enqueue_steps for the Stockham x-z kernels
+// and the spectral y stage, each one launch for all n boxes.  h is any box
+// of the class (shapes, LDS sizes).
+void enqueue_table_step(const snf_handle* h, const BoxTables& d, int n, int st, hipStream_t s) {
+  const Geom& g = h->g;
+  k_step_begin_tab<<<dim3(1, 1, n), 1, 0, s>>>(d.step + (long)st * n);
+  for (int rk = 1; rk <= 3; ++rk) {
+    const long o = (long)(3 * st + rk - 1) * n;
+    k_cplx_to_real_tab<<<dim3((g.my + 255) / 256, 1, n), 256, 0, s>>>(d.c2r + o);
+    if (h->mixed)
+      tab_A<cplxf>(h, d.A + o, n, s);
+    else
+      tab_A<cplx>(h, d.A + o, n, s);
+    if (rk == 1) {
+      k_hist_f_tab<<<dim3(g.my, 1, n), 256, 0, s>>>(d.hf + o);
+      k_hist_spec_tab<<<dim3((unsigned)((g.P + 63) / 64), 1, n), 64 * SPEC_SEG, 0, s>>>(d.hs + o);
+    }
+    if (h->mixed) {
+      tab_B<cplxf>(h, d.B + o, n, rk, s);
+      tab_C<cplxf>(h, d.C + o, n, s);
+    } else {
+      tab_B<cplx>(h, d.B + o, n, rk, s);
+      tab_C<cplx>(h, d.C + o, n, s);
+    }
+    if (rk == 1) k_phist_plane_tab<<<dim3(g.my, 1, n), 64, 0, s>>>(d.ph + o);
+    k_scalar_tab<<<dim3(1, 1, n), 256, 0, s>>>(d.sc + o, rk);
+    k_ysym_tab<<<dim3(g.kxn, (g.my + 255) / 256, n), 256, 0, s>>>(d.ys + o);
+    tab_y(h, d.y + o, n, rk, s);
+  }
+}
+
+// the LDS limits of a class's table kernels, before any capture
+void table_attrs(const snf_handle* h, hipStream_t s) {
+  if (h->mixed) {
+    tab_A<cplxf>(h, nullptr, 0, s);
+    tab_B<cplxf>(h, nullptr, 0, 1, s);
+    tab_C<cplxf>(h, nullptr, 0, s);
+  } else {
+    tab_A<cplx>(h, nullptr, 0, s);
+    tab_B<cplx>(h, nullptr, 0, 1, s);
+    tab_C<cplx>(h, nullptr, 0, s);
+  }
+  tab_y(h, nullptr, 0, 1, s);
+}
+
+BxXzB box_B(const snf_handle* h) {
+  BxXzB b;
+  b.g = h->g;
+  b.hin = h->hyb_in;
+  b.hout = h->hyb_out;
+  b.px = h->px;
+  b.twx = h->twx;
+  b.umax_acc = h->umax_acc;
+  b.phist = h->phist;
+  b.sc = h->sc;
+  b.nhist = h->p.nhist;
+  b.zb = h->zb;
+  return b;
+}
+
+// Fills and uploads the tables of the boxes hs (their cur parity as it is
+// now: substep k of the pair runs at parity cur ^ (k & 1)).
+int build_tables(const std::vector<snf_handle*>& hs, BoxTables& d) {
+  const size_t n = hs.size();
+  std::vector<BxStep> step(2 * n);
+  std::vector<BxC2R> c2r(6 * n);
+  std::vector<BxXzA> A(6 * n);
+  std::vector<BxHist> hf(6 * n), hh(6 * n);
+  std::vector<BxXzB> B(6 * n);
+  std::vector<BxXzC> C(6 * n);
+  std::vector<BxPhist> ph(6 * n);
+  std::vector<ScalarArgs> sc(6 * n);
+  std::vector<BxYsym> ys(6 * n);
+  std::vector<YArgs> y(6 * n);
+  for (size_t q = 0; q < n; ++q) {
+    snf_handle* h = hs[q];
+    step[q] = step[n + q] = BxStep{h->sc, 0.0};
+    const int cur0 = h->cur;
+    for (int k = 0; k < 6; ++k) {
+      const size_t e = (size_t)k * n + q;
+      h->cur = cur0 ^ (k & 1);
+      c2r[e] = BxC2R{h->u00c, h->w00c, h->u00r, h->w00r, h->g.my};
+      A[e] = BxXzA{h->g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r, h->w00r, h->dudy,
+                   h->dwdy, h->p.vbulk, h->hyb_in, h->pz, h->twz, h->sc};
+      hf[e] = BxHist{h->g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r, h->w00r, h->dudy,
+                     h->dwdy, h->p.vbulk, h->stat_plane, nullptr, h->sc, h->p.nhist};
+      hh[e] = hf[e];
+      hh[e].out = h->spec;
+      hh[e].out2 = h->spec2;
+      B[e] = box_B(h);
+      C[e] = BxXzC{h->g, h->hyb_out, h->v, h->rdome, h->rhg, h->rhv, h->rf0u, h->rf0w, h->pz,
+                   h->twz, h->sc};
+      ph[e] = BxPhist{h->phist, h->g.mgalz / h->zb, h->pplane, h->sc, h->p.nhist};
+      sc[e] = sargs(h);
+      ys[e] = BxYsym{h->g.my, h->c, h->shp[1 - h->cur], h->twy, h->ysym[1 - h->cur], h->sc};
+      y[e] = yargs(h);
+    }
+    h->cur = cur0;
+  }
+  size_t off = 0;
+  auto slot = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_step = slot(step.size() * sizeof(BxStep)), o_c2r = slot(c2r.size() * sizeof(BxC2R)),
+               o_A = slot(A.size() * sizeof(BxXzA)), o_hf = slot(hf.size() * sizeof(BxHist)),
+               o_hs = slot(hh.size() * sizeof(BxHist)), o_B = slot(B.size() * sizeof(BxXzB)),
+               o_C = slot(C.size() * sizeof(BxXzC)), o_ph = slot(ph.size() * sizeof(BxPhist)),
+               o_sc = slot(sc.size() * sizeof(ScalarArgs)), o_ys = slot(ys.size() * sizeof(BxYsym)),
+               o_y = slot(y.size() * sizeof(YArgs));
+  std::vector<char> host(off, 0);
+  std::memcpy(&host[o_step], step.data(), step.size() * sizeof(BxStep));
+  std::memcpy(&host[o_c2r], c2r.data(), c2r.size() * sizeof(BxC2R));
+  std::memcpy(&host[o_A], A.data(), A.size() * sizeof(BxXzA));
+  std::memcpy(&host[o_hf], hf.data(), hf.size() * sizeof(BxHist));
+  std::memcpy(&host[o_hs], hh.data(), hh.size() * sizeof(BxHist));
+  std::memcpy(&host[o_B], B.data(), B.size() * sizeof(BxXzB));
+  std::memcpy(&host[o_C], C.data(), C.size() * sizeof(BxXzC));
+  std::memcpy(&host[o_ph], ph.data(), ph.size() * sizeof(BxPhist));
+  std::memcpy(&host[o_sc], sc.data(), sc.size() * sizeof(ScalarArgs));
+  std::memcpy(&host[o_ys], ys.data(), ys.size() * sizeof(BxYsym));
+  std::memcpy(&host[o_y], y.data(), y.size() * sizeof(YArgs));
+  HIPCHK(hipMalloc(&d.blob, off));
+  HIPCHK(hipMemcpy(d.blob, host.data(), off, hipMemcpyHostToDevice));
+  char* b = static_cast<char*>(d.blob);
+  d.step = reinterpret_cast<BxStep*>(b + o_step);
+  d.c2r = reinterpret_cast<BxC2R*>(b + o_c2r);
+  d.A = reinterpret_cast<BxXzA*>(b + o_A);
+  d.hf = reinterpret_cast<BxHist*>(b + o_hf);
+  d.hs = reinterpret_cast<BxHist*>(b + o_hs);
+  d.B = reinterpret_cast<BxXzB*>(b + o_B);
+  d.C = reinterpret_cast<BxXzC*>(b + o_C);
+  d.ph = reinterpret_cast<BxPhist*>(b + o_ph);
+  d.sc = reinterpret_cast<ScalarArgs*>(b + o_sc);
+  d.ys = reinterpret_cast<BxYsym*>(b + o_ys);
+  d.y = reinterpret_cast<YArgs*>(b + o_y);
+  return SNF_OK;
+}
+
+// Autotuner of the batched x-row kernel: the first batch of a size class in
+// the process times every fitting block shape (threads x z-rows per block)
+// of k_xz_B_tab over the class's own boxes (rk = 2: it writes only the
+// hybrid output planes, which the next step rewrites) and keeps the
+// fastest for the class.  The choice holds for these handles' later single
+// advances too; it moves only the grouping of the 'p' statistics partials.
+std::mutex g_tune_mu;
+std::map<std::vector<int>, std::pair<int, int>> g_tuned;
+
+std::vector<int> class_key(const snf_handle* h) {
+  return {h->dev, h->g.mgalx, h->g.my, h->g.mgalz, h->mixed ? 1 : 0, h->p.explicit_visc};
+}
+
+int set_b_shape(const std::vector<snf_handle*>& hs, int nt, int zb) {
+  for (snf_handle* h : hs) {
+    if (h->bthreads == nt && h->zb == zb) continue;
+    h->bthreads = nt;
+    h->zb = zb;
+    h->ldsB = 3 * (size_t)zb * h->g.mgalx * (h->mixed ? sizeof(cplxf) : sizeof(cplx));
+    for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
+                           (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
+                           (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>})
+      lds_attr(kb, h->ldsB, h->dev);
+    // the step graphs of the single advance carry the old shape
+    for (hipGraphExec_t& ge : h->gexec)
+      if (ge) {
+        (void)hipGraphExecDestroy(ge);
+        ge = nullptr;
+      }
+  }
+  return SNF_OK;
+}
+
+int tune_class(const std::vector<snf_handle*>& hs, hipStream_t s) {
+  const snf_handle* h0 = hs[0];
+  const std::vector<int> key = class_key(h0);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) return set_b_shape(hs, it->second.first, it->second.second);
+  const int n = (int)hs.size();
+  BxXzB* dB = nullptr;
+  HIPCHK(hipMalloc((void**)&dB, n * sizeof(BxXzB)));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  std::pair<int, int> pick(h0->bthreads, h0->zb);
+  int rc = SNF_OK;
+  for (int nt : {256, 512, 1024})
+    for (int zb : {2, 4, 8, 16}) {
+      if (!xz_b_fits(h0, nt, zb)) continue;
+      set_b_shape(hs, nt, zb);
+      std::vector<BxXzB> hb((size_t)n);
+      for (int q = 0; q < n; ++q) hb[q] = box_B(hs[q]);
+      if (hipMemcpy(dB, hb.data(), n * sizeof(BxXzB), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(SNF_DEVICE_ERROR, "autotune table copy");
+        break;
+      }
+      const snf_handle* h = hs[0];
+      auto run = [&]() {
+        if (h->mixed)
+          tab_B<cplxf>(h, dB, n, 2, s);
+        else
+          tab_B<cplx>(h, dB, n, 2, s);
+      };
+      run();   // warm-up
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < 3; ++r) run();
+      (void)hipEventRecord(e1, s);
+      float ms = 0.f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess ||
+          hipGetLastError() != hipSuccess) {
+        rc = fail(SNF_DEVICE_ERROR, "autotune timing");
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        pick = {nt, zb};
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(dB);
+  if (rc != SNF_OK) return rc;
+  g_tuned[key] = pick;
+  return set_b_shape(hs, pick.first, pick.second);
+}
+
+
 
 }  // namespace
 
@@ -567,22 +903,12 @@ extern "C" {
 
 const char* snf_last_error(void) { return g_err.c_str(); }
 
-// line lengths my with a compiled spectral y plan (radices of the passes)
-static bool y_plan(int my) { return my == 33 || my == 49 || my == 385; }
 // line lengths the spectral y stage serves: the radix plans, and the direct
 // DFT where its my^2 work per line still beats the banded sweeps (measured:
 // 3.2x faster at my = 32, 1.8x at 129, 2.2x slower at 193)
-#define Y_DIRECT_MAX 144
-static bool y_spectral(int my) { return y_plan(my) || my <= Y_DIRECT_MAX; }
-// radices of the compiled y-transform plans in pass order (the template
-// arguments of k_yf_explicit / k_yf_filter); returns the number of passes
-static int y_radices(int my, int* R) {
-  switch (my) {
-    case 33: R[0] = 11; R[1] = 3; return 2;
-    case 49: R[0] = 7; R[1] = 7; return 2;
-    case 385: R[0] = 11; R[1] = 7; R[2] = 5; return 3;
-  }
-  return 0;
+static bool y_spectral(int my) {
+  int R[4];
+  return y_plan_radices(my, R) > 0 || my <= Y_DIRECT_MAX;
 }
 
 int snf_create(const snf_params* p, snf_handle** out) {
@@ -600,8 +926,6 @@ int snf_create(const snf_params* p, snf_handle** out) {
     return fail(SNF_BAD_PARAM, "precision: SNF_PREC_FP64 or SNF_PREC_MIXED");
   if (p->ydealias < 0 || p->ydealias > 3)
     return fail(SNF_BAD_PARAM, "ydealias: 0 (none), 1, 2 (y-Fourier cuts) or 3 (compact filter)");
-  if (p->ydealias == 3 && !y_spectral(p->my))
-    return fail(SNF_BAD_PARAM, "ydealias=3 needs my <= 144 or a y-transform plan (my = 385)");
   snf_handle* h = new snf_handle();
   h->p = *p;
   h->mixed = p->precision == SNF_PREC_MIXED;
@@ -781,7 +1105,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // pass s > 0 of the plan: exp(-2 pi i r k / (Ns R)) at [r - 1][k]
     std::vector<cplx> tp(my, mkc(0.0, 0.0));
     int R[4];
-    const int np = y_radices(my, R);
+    const int np = y_plan_radices(my, R);
     if (np > 0) {
       if (yf_twiddle_count(R, np) > my) return fail(SNF_BAD_PARAM, "y twiddle table larger than my");
       int Ns = R[0], off = 0;
@@ -801,28 +1125,20 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // tile, a column per wave, roots, two multiplier rows, four phase tables
     // (1472 B at my = 385: 81552 B, two blocks still fit one CU's 160 KiB)
     h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
-    for (const void* ky :
-         {(const void*)k_yf_explicit<YFFT<11, 3>>, (const void*)k_yf_explicit<YFFT<7, 7>>,
-          (const void*)k_yf_explicit<YFFT<11, 7, 5>>, (const void*)k_yf_explicit<YDirect<1>>,
-          (const void*)k_yf_explicit<YDirect<2>>, (const void*)k_yf_explicit<YDirect<3>>,
-          (const void*)k_yf_implicit<YFFT<11, 3>>, (const void*)k_yf_implicit<YFFT<7, 7>>,
-          (const void*)k_yf_implicit<YFFT<11, 7, 5>>, (const void*)k_yf_implicit<YDirect<1>>,
-          (const void*)k_yf_implicit<YDirect<2>>, (const void*)k_yf_implicit<YDirect<3>>,
-          (const void*)k_yf_first<YFFT<11, 3>>, (const void*)k_yf_first<YFFT<7, 7>>,
-          (const void*)k_yf_first<YFFT<11, 7, 5>>, (const void*)k_yf_first<YDirect<1>>,
-          (const void*)k_yf_first<YDirect<2>>, (const void*)k_yf_first<YDirect<3>>,
-          (const void*)k_yf_filter<YFFT<11, 3>>,
-          (const void*)k_yf_filter<YFFT<7, 7>>, (const void*)k_yf_filter<YFFT<11, 7, 5>>,
-          (const void*)k_yf_filter<YDirect<1>>, (const void*)k_yf_filter<YDirect<2>>,
-          (const void*)k_yf_filter<YDirect<3>>})
-      lds_attr(ky, h->ldsY, h->dev);
+    with_yplan(my, [&](auto tag) {
+      typedef typename decltype(tag)::type PL;
+      for (const void* ky : {(const void*)k_yf_explicit<PL>, (const void*)k_yf_implicit<PL>,
+                             (const void*)k_yf_first<PL>, (const void*)k_yf_filter<PL>})
+        lds_attr(ky, h->ldsY, h->dev);
+    });
+    if (!h->yfft && p->ydealias == 3)
+      with_filter_direct(my, [&](auto tag) {
+        lds_attr((const void*)k_yf_filter<typename decltype(tag)::type>, filter_direct_lds(my), h->dev);
+      });
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
   // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning).
-  auto b_fits = [&](int nt, int zb) {
-    return zb >= 2 && mgalz % zb == 0 && 3 * (size_t)zb * mgalx * sizeof(cplx) <= 156 * 1024 &&
-           (zb / 2) * mgalx <= XZ_B_MAXW * nt && fft_max_values(h->px, 3 * zb, nt) <= XZ_MAXV;
-  };
+  auto b_fits = [&](int nt, int zb) { return xz_b_fits(h, nt, zb); };
   // 256 threads x 2 rows measured fastest at 384x385x384 (8 LDS-resident
   // rows per CU at 4 blocks/CU beat 1-2 larger blocks)
   h->bthreads = 256;
@@ -1097,12 +1413,12 @@ static int advance_begin(snf_handle* h, bool continue_call, int nstep, AdvanceMa
 
 static int enqueue_steps(snf_handle* h, int nstep);
 
-static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
+// start of cross: kx=0 Hermitian fix, twist of the current offsets,
+// v = lapcdy(phi) and the first D1(vor), D1(v)
+static int advance_prologue(snf_handle* h, int resume) {
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
   int rc;
-  // start of cross: kx=0 Hermitian fix, twist of the current offsets,
-  // v = lapcdy(phi) and the first D1(vor), D1(v)
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], resume);
   if (!resume) k_set_conj<<<g.my, 128, 0, h->stream>>>(g, h->vor, h->phi);
   if ((rc = launch_twist(h, h->cur))) return rc;
@@ -1113,6 +1429,12 @@ static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
   else
     k_y_first<<<ylines_blocks(h), 128, 0, h->stream>>>(yargs(h));
   HIPCHK(hipGetLastError());
+  return SNF_OK;
+}
+
+static int advance_enqueue(snf_handle* h, int32_t nstep, int resume) {
+  int rc;
+  if ((rc = advance_prologue(h, resume))) return rc;
   // steps in pairs from captured graphs (one launch per pair instead of about
   // fifty), the remainder launched one by one; with per-kernel timing on,
   // every step is launched one by one
@@ -1211,6 +1533,91 @@ int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st) {
   return advance_impl(h, nstep, st, false);
 }
 
+// ---- batched ensembles
+// Boxes of one size class advance together from box tables (one launch per
+// kernel for the class); the rest advance one stream each.
+static bool table_eligible(const snf_handle* h) {
+  return !h->wave_xz && h->yfft && h->p.ydealias == 0 && !h->timing && h->graphs;
+}
+
+// resources of one class's run, released after every box has finished
+struct ClassRun {
+  std::vector<snf_handle*> hs;
+  BoxTables d;
+  hipGraphExec_t ge = nullptr;
+  std::vector<hipEvent_t> ev;
+  void release() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    if (d.blob) (void)hipFree(d.blob);
+    ge = nullptr;
+    ev.clear();
+    d.blob = nullptr;
+  }
+};
+
+static int class_event(ClassRun& r, hipEvent_t* e) {
+  HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  r.ev.push_back(*e);
+  return SNF_OK;
+}
+
+// every box's start of cross on its own stream, then nstep steps of the
+// whole class on the stream of its first box: pairs of steps replayed from
+// one captured graph, an odd last step launched directly
+static int class_enqueue(ClassRun& r, const std::vector<int>& resume, int nstep) {
+  const int n = (int)r.hs.size();
+  snf_handle* h0 = r.hs[0];
+  const hipStream_t s = h0->stream;
+  int rc;
+  for (int q = 0; q < n; ++q)
+    if ((rc = advance_prologue(r.hs[q], resume[q]))) return rc;
+  HIPCHK(hipSetDevice(h0->dev));
+  for (int q = 1; q < n; ++q) {
+    hipEvent_t e;
+    if ((rc = class_event(r, &e))) return rc;
+    HIPCHK(hipEventRecord(e, r.hs[q]->stream));
+    HIPCHK(hipStreamWaitEvent(s, e, 0));
+  }
+  if ((rc = tune_class(r.hs, s))) return rc;
+  if ((rc = build_tables(r.hs, r.d))) return rc;
+  table_attrs(h0, s);
+  int left = nstep;
+  if (left >= 2) {
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_table_step(h0, r.d, n, 0, s);
+    enqueue_table_step(h0, r.d, n, 1, s);
+    const hipError_t ec = hipStreamEndCapture(s, &gr);
+    if (ec != hipSuccess || !gr || hipGraphInstantiate(&r.ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+      r.ge = nullptr;
+    } else {
+      (void)hipGraphDestroy(gr);
+    }
+    for (; left >= 2; left -= 2) {
+      if (r.ge) {
+        HIPCHK(hipGraphLaunch(r.ge, s));
+      } else {
+        enqueue_table_step(h0, r.d, n, 0, s);
+        enqueue_table_step(h0, r.d, n, 1, s);
+      }
+    }
+  }
+  if (left == 1) enqueue_table_step(h0, r.d, n, 0, s);
+  HIPCHK(hipGetLastError());
+  // three substeps a step flip each box's twist parity
+  for (snf_handle* h : r.hs) h->cur ^= (nstep & 1);
+  // the other boxes' streams wait for the class's steps (advance_finish
+  // synchronises each box's own stream)
+  hipEvent_t done;
+  if ((rc = class_event(r, &done))) return rc;
+  HIPCHK(hipEventRecord(done, s));
+  for (int q = 1; q < n; ++q) HIPCHK(hipStreamWaitEvent(r.hs[q]->stream, done, 0));
+  return SNF_OK;
+}
+
 int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_status* st) {
   if (!hs || n < 1 || nstep < 0) return fail(SNF_BAD_PARAM, "bad argument");
   for (int i = 0; i < n; ++i) {
@@ -1223,10 +1630,52 @@ int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_s
   int rc;
   for (int i = 0; i < n; ++i)
     if ((rc = advance_begin(hs[i], false, nstep, m[i]))) return rc;
+  // size classes of two or more table-eligible boxes; SNF_BATCH_TABLES=0
+  // keeps every box on its own stream
+  const char* et = getenv("SNF_BATCH_TABLES");
+  const bool tables = !(et && std::string(et) == "0");
+  std::map<std::vector<int>, std::vector<int>> cls;
+  std::vector<int> single;
+  for (int i = 0; i < n; ++i) {
+    if (tables && nstep > 0 && table_eligible(hs[i]))
+      cls[class_key(hs[i])].push_back(i);
+    else
+      single.push_back(i);
+  }
+  std::vector<ClassRun> runs;
+  for (auto& kv : cls) {
+    if (kv.second.size() < 2) {
+      single.push_back(kv.second[0]);
+      continue;
+    }
+    runs.emplace_back();
+    for (int i : kv.second) runs.back().hs.push_back(hs[i]);
+  }
+  auto cleanup = [&]() {
+    for (ClassRun& r : runs) r.release();
+  };
+  for (int i : single)
+    if ((rc = advance_enqueue(hs[i], nstep, m[i].resume))) {
+      cleanup();
+      return rc;
+    }
+  for (ClassRun& r : runs) {
+    std::vector<int> resume;
+    for (snf_handle* h : r.hs)
+      for (int i = 0; i < n; ++i)
+        if (hs[i] == h) resume.push_back(m[i].resume);
+    if ((rc = class_enqueue(r, resume, nstep))) {
+      for (snf_handle* h : r.hs) (void)hipStreamSynchronize(h->stream);
+      cleanup();
+      return rc;
+    }
+  }
   for (int i = 0; i < n; ++i)
-    if ((rc = advance_enqueue(hs[i], nstep, m[i].resume))) return rc;
-  for (int i = 0; i < n; ++i)
-    if ((rc = advance_finish(hs[i], m[i], st ? st + i : nullptr))) return rc;
+    if ((rc = advance_finish(hs[i], m[i], st ? st + i : nullptr))) {
+      cleanup();
+      return rc;
+    }
+  cleanup();
   return SNF_OK;
 }
 

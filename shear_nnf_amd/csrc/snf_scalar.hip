@@ -147,8 +147,7 @@ struct ScalarArgs {
 // j sums the x-row kernel's partials of plane j in a fixed order (lanes in
 // strides of 64, then a shuffle tree).  Launched at rkstep 1 after the x-row
 // kernel; a no-op off the statistics steps.
-__global__ void __launch_bounds__(64)
-k_phist_plane(const double* __restrict__ phist, int nzb, double* __restrict__ pplane,
+__device__ __forceinline__ void k_phist_plane_body(const double* __restrict__ phist, int nzb, double* __restrict__ pplane,
               const DevScalars* sc, int nhist) {
   if (sc->halted || (sc->steps_local - 1) % nhist != 0) return;
   const int j = blockIdx.x, lane = threadIdx.x;
@@ -160,9 +159,13 @@ k_phist_plane(const double* __restrict__ phist, int nzb, double* __restrict__ pp
   if (lane == 0)
     for (int q = 0; q < 3; ++q) pplane[(long)j * 3 + q] = ph[q];
 }
+__global__ void __launch_bounds__(64)
+k_phist_plane(const double* __restrict__ phist, int nzb, double* __restrict__ pplane,
+              const DevScalars* sc, int nhist) {
+  k_phist_plane_body(phist, nzb, pplane, sc, nhist);
+}
 
-__global__ void __launch_bounds__(256)
-k_scalar(ScalarArgs a, int rk) {
+__device__ __forceinline__ void k_scalar_body(ScalarArgs a, int rk) {
   DevScalars* sc = a.sc;
   if (sc->halted) return;
   const Geom& g = a.g;
@@ -311,6 +314,10 @@ k_scalar(ScalarArgs a, int rk) {
     }
   }
 }
+__global__ void __launch_bounds__(256)
+k_scalar(ScalarArgs a, int rk) {
+  k_scalar_body(a, rk);
+}
 
 // start of an advance call (cross.f90:81-103, 287-291, 383-406).  resume:
 // the previous call stopped at a dump and this one continues the same run,
@@ -344,7 +351,7 @@ __global__ void k_begin_call(Geom g, DevScalars* sc, cplx* shp, cplx* shm, int r
 // second one (cross.f90:353-374)
 // wrap_lx > 0 (iydealiasing=2): the offsets are first wrapped into one
 // period, xoffb = (xoffb/Lx - int(xoffb/Lx)) Lx, xofft = -xoffb (cross.f90:348-352)
-__global__ void k_step_begin(DevScalars* sc, double wrap_lx) {
+__device__ __forceinline__ void k_step_begin_body(DevScalars* sc, double wrap_lx) {
   if (sc->halted) return;
   if (wrap_lx > 0.0) {
     const double q = sc->xoffb / wrap_lx;
@@ -359,6 +366,9 @@ __global__ void k_step_begin(DevScalars* sc, double wrap_lx) {
     return;
   }
   sc->steps_local = next;
+}
+__global__ void k_step_begin(DevScalars* sc, double wrap_lx) {
+  k_step_begin_body(sc, wrap_lx);
 }
 
 // kx = 0 Hermitian fix on the state (cross.f90:287-291, set_conj)

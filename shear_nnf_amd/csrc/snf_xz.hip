@@ -96,8 +96,7 @@ __device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i,
 // H: element type of the transforms and hybrid planes (cplx, or cplxf in the
 // mixed precision mode); the spectral fields and the reconstruction are FP64
 template <class H>
-__global__ void __launch_bounds__(256)
-k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+__device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
        const cplx* __restrict__ hg, const cplx* __restrict__ hv,
        const cplx* __restrict__ dvordy, const double* __restrict__ u00,
        const double* __restrict__ w00, const double* __restrict__ dudy,
@@ -132,6 +131,16 @@ k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
     __syncthreads();
   }
 }
+template <class H>
+__global__ void __launch_bounds__(256)
+k_xz_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+       const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+       const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+       const double* __restrict__ w00, const double* __restrict__ dudy,
+       const double* __restrict__ dwdy, double vbulk, H* __restrict__ hyb, FFTPlan pz,
+       const cplx* __restrict__ twz, const DevScalars* sc) {
+  k_xz_A_body<H>(g, vor, phi, hg, hv, dvordy, u00, w00, dudy, dwdy, vbulk, hyb, pz, twz, sc);
+}
 
 // ---------------------------------------------------------------------------
 // B: ZB z-rows per block: c2r in x, u x omega, r2c in x.  Reads and writes
@@ -144,8 +153,7 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
 
 // NT threads per block (256, 512 or 1024), ZB z-rows per block
 template <int NT, class H>
-__global__ void __launch_bounds__(NT)
-k_xz_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout, FFTPlan px,
+__device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H* __restrict__ hout, FFTPlan px,
        const cplx* __restrict__ twx, int rk, unsigned long long* umax_acc,
        double* __restrict__ phist, const DevScalars* sc, int nhist, int ZB) {
   typedef typename wf_real<H>::T R;
@@ -293,14 +301,20 @@ k_xz_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout, FFTPlan px,
     hout[2 * fstride + off] = H3;
   }
 }
+template <int NT, class H>
+__global__ void __launch_bounds__(NT)
+k_xz_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout, FFTPlan px,
+       const cplx* __restrict__ twx, int rk, unsigned long long* umax_acc,
+       double* __restrict__ phist, const DevScalars* sc, int nhist, int ZB) {
+  k_xz_B_body<NT, H>(g, hin, hout, px, twx, rk, umax_acc, phist, sc, nhist, ZB);
+}
 
 // ---------------------------------------------------------------------------
 // C: hybrid -> spectral (forward z DFT), output combinations (iadd_mode=5,
 // cross.f90:1533-1544) and the 00-mode terms rf0u, rf0w (cross.f90:1489-1490)
 // ---------------------------------------------------------------------------
 template <class H>
-__global__ void __launch_bounds__(256)
-k_xz_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
+__device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
        cplx* __restrict__ dvordy, cplx* __restrict__ hg, cplx* __restrict__ hv,
        double* __restrict__ rf0u,
        double* __restrict__ rf0w, FFTPlan pz, const cplx* __restrict__ twz,
@@ -341,6 +355,15 @@ k_xz_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
     }
   }
 }
+template <class H>
+__global__ void __launch_bounds__(256)
+k_xz_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
+       cplx* __restrict__ dvordy, cplx* __restrict__ hg, cplx* __restrict__ hv,
+       double* __restrict__ rf0u,
+       double* __restrict__ rf0w, FFTPlan pz, const cplx* __restrict__ twz,
+       const DevScalars* sc) {
+  k_xz_C_body<H>(g, hout, vel, dvordy, hg, hv, rf0u, rf0w, pz, twz, sc);
+}
 
 // ---------------------------------------------------------------------------
 // hist 'f' (cross.f90:1798-1966): per-plane Fourier-space statistics.
@@ -355,8 +378,7 @@ k_xz_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
 // SPEC_SEG waves walk fixed segments of the planes in order, and the segment
 // sums are added in segment order, so the sums do not depend on the launch.
 #define SPEC_SEG 8
-__global__ void __launch_bounds__(64 * SPEC_SEG)
-k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+__device__ __forceinline__ void k_hist_spec_body(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
             const cplx* __restrict__ hg, const cplx* __restrict__ hv,
             const cplx* __restrict__ dvordy, const double* __restrict__ u00,
             const double* __restrict__ w00, const double* __restrict__ dudy,
@@ -402,9 +424,17 @@ k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
     }
   }
 }
+__global__ void __launch_bounds__(64 * SPEC_SEG)
+k_hist_spec(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+            const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+            const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+            const double* __restrict__ w00, const double* __restrict__ dudy,
+            const double* __restrict__ dwdy, double vbulk, double* __restrict__ sp,
+            double* __restrict__ sp2, const DevScalars* sc, int nhist) {
+  k_hist_spec_body(g, vor, phi, hg, hv, dvordy, u00, w00, dudy, dwdy, vbulk, sp, sp2, sc, nhist);
+}
 
-__global__ void __launch_bounds__(256)
-k_hist_f(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+__device__ __forceinline__ void k_hist_f_body(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
          const cplx* __restrict__ hg, const cplx* __restrict__ hv,
          const cplx* __restrict__ dvordy, const double* __restrict__ u00,
          const double* __restrict__ w00, const double* __restrict__ dudy,
@@ -467,6 +497,15 @@ k_hist_f(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
     __syncthreads();
   }
   if (threadIdx.x < 11) out[(long)j * STAT_W + threadIdx.x] = red[threadIdx.x][0];
+}
+__global__ void __launch_bounds__(256)
+k_hist_f(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+         const cplx* __restrict__ hg, const cplx* __restrict__ hv,
+         const cplx* __restrict__ dvordy, const double* __restrict__ u00,
+         const double* __restrict__ w00, const double* __restrict__ dudy,
+         const double* __restrict__ dwdy, double vbulk, double* __restrict__ out,
+         const DevScalars* sc, int nhist) {
+  k_hist_f_body(g, vor, phi, hg, hv, dvordy, u00, w00, dudy, dwdy, vbulk, out, sc, nhist);
 }
 
 // ---------------------------------------------------------------------------

@@ -14,7 +14,7 @@
 // (x -> X_m = sum_j x_j exp(-i w_m j)), multiplied, and transformed back.
 //
 // Transforms: one wave per line, mixed-radix Stockham passes (radices 2, 3,
-// 4, 5, 7, 8, 11) in place on the line's column of an LDS tile, every pass a
+// 4, 5, 7, 8, 11, 13) in place on the line's column of an LDS tile, every pass a
 // read-all / write-all of the wave's own column (no block barrier).
 // Twiddles from a pass-major table staged in LDS: pass s > 0 (stride Ns,
 // radix R) holds exp(-2 pi i r k / (Ns R)) at [r - 1][k], r < R, k < Ns, so
@@ -100,6 +100,21 @@ template <> struct OddRoots<11> {
                                    -0.9096319953545183714117, -0.5406408174555975821076};
 };
 
+template <> struct OddRoots<13> {
+  static constexpr double c[13] = {1.0, 0.8854560256532098959004, 0.5680647467311558025118,
+                                   0.1205366802553230533491, -0.3546048870425356259696,
+                                   -0.7485107481711010986346, -0.9709418174260520271570,
+                                   -0.9709418174260520271570, -0.7485107481711010986346,
+                                   -0.3546048870425356259696, 0.1205366802553230533491,
+                                   0.5680647467311558025118, 0.8854560256532098959004};
+  static constexpr double s[13] = {0.0, 0.4647231720437685456560, 0.8229838658936563945796,
+                                   0.9927088740980539928008, 0.9350162426854148234398,
+                                   0.6631226582407952023768, 0.2393156642875577671488,
+                                   -0.2393156642875577671488, -0.6631226582407952023768,
+                                   -0.9350162426854148234398, -0.9927088740980539928008,
+                                   -0.8229838658936563945796, -0.4647231720437685456560};
+};
+
 template <int R, int SIGN>
 __host__ __device__ __forceinline__ void yf_dft_odd(cplx* v) {
   constexpr int H = (R - 1) / 2;
@@ -133,8 +148,8 @@ __host__ __device__ __forceinline__ void yf_dft_odd(cplx* v) {
 
 template <int R, int SIGN>
 __host__ __device__ __forceinline__ void yf_dft(cplx* v) {
-  if (R == 5 || R == 7 || R == 11)
-    yf_dft_odd<(R == 5 ? 5 : (R == 7 ? 7 : 11)), SIGN>(v);
+  if (R == 5 || R == 7 || R == 11 || R == 13)
+    yf_dft_odd<(R == 5 ? 5 : (R == 7 ? 7 : (R == 11 ? 11 : 13))), SIGN>(v);
   else
     wf_dft<(R == 2 ? 2 : (R == 3 ? 3 : (R == 4 ? 4 : 8))), SIGN>(v);
 }

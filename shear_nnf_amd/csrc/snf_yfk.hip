@@ -140,8 +140,7 @@ __device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
 // and mode m at the twist shp[kx]: out[(kx N + m) YS + {0,1,2}] = R1/B1,
 // R2/B2, R_f/B_f at w_m = (2 pi m + arg shp)/N.  (The Helmholtz multiplier
 // of a line is 1/(R2/B2 - k2).)
-__global__ void __launch_bounds__(256)
-k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt,
+__device__ __forceinline__ void k_ysym_body(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt,
        cplx* __restrict__ out, const DevScalars* sc) {
   if (sc && sc->halted) return;
   const int kx = blockIdx.x, m = blockIdx.y * blockDim.x + threadIdx.x;
@@ -154,10 +153,14 @@ k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt
   gst(o + 1, yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1)));
   gst(o + 2, mkc(yf_filter_symbol(e1), 0.0));
 }
+__global__ void __launch_bounds__(256)
+k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt,
+       cplx* __restrict__ out, const DevScalars* sc) {
+  k_ysym_body(N, c, shp, rt, out, sc);
+}
 
 template <class PL>
-__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
-k_yf_explicit(YArgs a, int rk) {
+__device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   constexpr int Q = PL::Q;
   const int N = PL::direct ? a.g.my : PL::NC;
   if (a.sc->halted) return;
@@ -526,6 +529,11 @@ k_yf_explicit(YArgs a, int rk) {
   }
   yf_flush(T, a.v, P, L0, nv, N);
 }
+template <class PL>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_explicit(YArgs a, int rk) {
+  k_yf_explicit_body<PL>(a, rk);
+}
 
 // Implicit RK3 substep of the y stage (cross.f90:797-990) in the same block
 // layout as k_yf_explicit, with the Helmholtz solves (lapcdy with k2 + dtr1,
@@ -541,8 +549,7 @@ k_yf_explicit(YArgs a, int rk) {
 // dv = D1 of them.  U, W (m=1 calls, last block) follow cross.f90:829-843,
 // 858-863, 900-918.  13 line transforms; every field read and written once.
 template <class PL>
-__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
-k_yf_implicit(YArgs a, int rk) {
+__device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   constexpr int Q = PL::Q;
   const int N = PL::direct ? a.g.my : PL::NC;
   if (a.sc->halted) return;
@@ -935,6 +942,11 @@ k_yf_implicit(YArgs a, int rk) {
     if (j < N) T[tc(j)] = x[q];
   }
   yf_flush(T, a.v, P, L0, nv, N);
+}
+template <class PL>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_implicit(YArgs a, int rk) {
+  k_yf_implicit_body<PL>(a, rk);
 }
 
 // Start of a cross call on the spectral y stage (cross.f90:408-445), the

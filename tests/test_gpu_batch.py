@@ -44,3 +44,39 @@ def test_batch_rejects_a_repeated_handle():
     b = _boxes()[:1]
     with pytest.raises(snf.SnfError):
         snf.advance_batch([b[0], b[0]], 1)
+
+
+# several boxes per size class: each class advances from box tables (one
+# launch per kernel for all its boxes, pairs of steps from one graph)
+TABLE_CASES = [((32, 33, 32), 500.0, {}), ((32, 33, 32), 2900.0, {}), ((32, 33, 32), 1700.0, {}),
+               ((48, 49, 48), 900.0, {"explicit": 0}), ((48, 49, 48), 2500.0, {"explicit": 0}),
+               ((64, 65, 64), 1200.0, {"precision": "mixed"}),
+               ((64, 65, 64), 2200.0, {"precision": "mixed"}),
+               ((96, 97, 96), 3000.0, {}), ((96, 97, 96), 600.0, {}), ((128, 49, 128), 800.0, {})]
+
+
+def _table_boxes(nhist=1):
+    import shear_nnf_amd as snf
+    out = []
+    for q, (grid, Re, kw) in enumerate(TABLE_CASES):
+        b = snf.ShearBox(*grid, Re, alp=ALP, gam=GAM, Ly=LY, nhist=nhist, **kw)
+        b.set_state(*box_ic(*grid, seed=20260100 + q))
+        out.append(b)
+    return out
+
+
+@pytest.mark.parametrize("nsteps", [(5,), (2, 3, 1)])
+def test_box_tables_equal_separate_advances(nsteps):
+    """odd and even step counts, and calls that continue one another (the
+    twist parity of each box carries over from call to call)"""
+    import shear_nnf_amd as snf
+    batch, single = _table_boxes(), _table_boxes()
+    for k in nsteps:
+        st = snf.advance_batch(batch, k)
+        assert all(r.steps_done == k for r in st)
+    for b, s in zip(batch, single):
+        for k in nsteps:
+            s.cross(k)
+        for x, y in zip(b.get_state(), s.get_state()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(np.array(b.history), np.array(s.history))

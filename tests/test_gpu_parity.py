@@ -176,8 +176,12 @@ def test_cross_spectral_y_3_steps(lib, grid):
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
 
-# iydealiasing=3: compact filter of vor, phi, U, W at every substep
-@pytest.mark.parametrize("grid,explicit", [((32, 33, 32), 1), ((48, 49, 48), 1), ((48, 49, 48), 0)])
+# iydealiasing=3: compact filter of vor, phi, U, W at every substep; line
+# lengths without a spectral plan (145, 193, 513: banded substeps) filter
+# through the direct DFT
+@pytest.mark.parametrize("grid,explicit", [((32, 33, 32), 1), ((48, 49, 48), 1), ((48, 49, 48), 0),
+                                           ((32, 145, 32), 1), ((32, 193, 32), 1),
+                                           ((32, 193, 32), 0), ((24, 513, 24), 1)])
 def test_cross_compact_filter(lib, grid, explicit):
     st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 4, grid=grid, ydealias=3)
     assert rel_l2(vor, c.vor) < STEP_TOL
@@ -322,6 +326,35 @@ def test_spectral_and_banded_implicit_agree(lib, monkeypatch, grid):
     for path in ("spectral", "banded"):
         monkeypatch.setenv("SNF_Y_PATH", path)
         box = _box(*grid, explicit=0)
+        box.set_state(vor, phi, u00, w00)
+        box.cross(5)
+        outs.append(box.get_state())
+    for a, b in zip(*outs):
+        assert rel_l2(a, b) < STEP_TOL
+
+
+# radix plans of the y transforms for the ensemble's line lengths and the
+# powers of two up to 256 (8*4, 4*4*3, 8*8, 13*5, 8*4*3, 8*4*4, 8*8*3, 8*8*4)
+@pytest.mark.parametrize("grid,explicit", [((32, 32, 32), 1), ((32, 48, 32), 1), ((32, 64, 32), 1),
+                                           ((32, 65, 32), 0), ((32, 96, 32), 1), ((32, 96, 32), 0),
+                                           ((32, 128, 32), 1), ((32, 192, 32), 1),
+                                           ((32, 192, 32), 0), ((32, 256, 32), 1)])
+def test_cross_radix_y_plans_3_steps(lib, grid, explicit):
+    st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 3, grid=grid)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("grid", [(64, 192, 64), (48, 256, 48)])
+def test_radix_and_banded_y_paths_agree(lib, monkeypatch, grid):
+    vor, phi, u00, w00 = box_ic(*grid)
+    outs = []
+    for path in ("spectral", "banded"):
+        monkeypatch.setenv("SNF_Y_PATH", path)
+        box = _box(*grid, explicit=1)
         box.set_state(vor, phi, u00, w00)
         box.cross(5)
         outs.append(box.get_state())
