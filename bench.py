@@ -30,14 +30,16 @@ def parse():
     ap.add_argument("--explicit", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed precision leg")
-    ap.add_argument("--batch", type=int, default=32,
+    ap.add_argument("--batch", type=int, default=128,
                     help="boxes of the ensemble leg (0: skip it)")
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="precision of the main timed leg (mixed: no separate mixed leg)")
     ap.add_argument("--cmp-steps", type=int, default=2000,
                     help="steps of the FP64-vs-mixed statistics comparison (2000: the "
                          "length the mixed-precision workload is specified over)")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="oracle processes of the CPU baseline (capped at the host's CPUs)")
     ap.add_argument("--cpu-grid", type=str, default="",
                     help="grid of the CPU baseline sample (default: the workload grid)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -86,11 +88,10 @@ def measured_traffic(cls, grid, explicit, precision="fp64"):
     return d["classes"][cls]["bytes_per_launch"], os.path.relpath(path, os.path.dirname(path) + "/..")
 
 
-def cpu_baseline(grid, sample_grid, Re, explicit, seed):
-    """Times the numpy oracle for one RK3 step on the host at sample_grid and
-    scales to grid by the number of spectral coefficients (the oracle's cost
-    is linear in Nc up to the log factor of the FFTs).  One 384^3 step of the
-    oracle takes ~100 s, so the default sample is 192x193x192 (~15 s)."""
+def _cpu_sample(args):
+    """one process: nstep RK3 steps of the numpy oracle at the sample grid;
+    returns (seconds per step, start and end wall-clock times)"""
+    grid, sample_grid, Re, explicit, seed, nstep = args
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle.grid import Grid
     from oracle.cross import Cross
@@ -101,14 +102,55 @@ def cpu_baseline(grid, sample_grid, Re, explicit, seed):
     c = Cross(g, explicit=explicit, nhist=10 ** 9)
     c.set_state(vor, phi, u00, w00)
     t0 = time.perf_counter()
-    nstep = 3
     c.run(nstep)
-    dt = (time.perf_counter() - t0) / nstep
+    t1 = time.perf_counter()
+    return (t1 - t0) / nstep, t0, t1
+
+
+def host_cpu():
+    """CPU model and logical core count of this host"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(grid, sample_grid, Re, explicit, seed, procs):
+    """The numpy oracle on the host cores: procs processes at once, each
+    running 2 RK3 steps of the same seeded sample (one numpy thread each;
+    the oracle has no threading of its own), so every core works on the
+    reference algorithm.  The aggregate rate at the sample grid is scaled
+    to the workload grid by the number of spectral coefficients (the
+    oracle's cost is linear in Nc up to the log factor of the FFTs).  One
+    384^3 step of the oracle takes ~100 s on one core, so the sample is
+    192x193x192.  Runs before anything initialises the GPU (the workers
+    are fresh processes)."""
+    import multiprocessing as mp
+    nstep = 2
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_sample, [(grid, sample_grid, Re, explicit, seed, nstep)] * procs)
+    per_step = [r[0] for r in res]
+    span = max(r[2] for r in res) - min(r[1] for r in res)
+    rate = procs * nstep / span
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.grid import Grid
     G = Grid(grid[0], grid[1], grid[2], Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0)
+    g = Grid(*sample_grid, Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0)
     scale = g.nc / G.nc
-    return scale / dt, (f"{nstep} RK3 steps of the numpy oracle at {mgalx}x{my}x{mgalz}, "
-                        f"{dt:.2f} s per step; scaled to "
-                        f"{grid[0]}x{grid[1]}x{grid[2]} by Nc ratio {scale:.4f}")
+    model, ncpu = host_cpu()
+    return rate * scale, (f"{procs} processes at once, each {nstep} RK3 steps of the numpy oracle "
+                          f"at {sample_grid[0]}x{sample_grid[1]}x{sample_grid[2]} "
+                          f"({min(per_step):.2f}-{max(per_step):.2f} s per step per process, "
+                          f"{span:.1f} s for all); aggregate {rate:.3f} steps/s scaled to "
+                          f"{grid[0]}x{grid[1]}x{grid[2]} by the Nc ratio {scale:.4f}; host: "
+                          f"{model}, {ncpu} logical CPUs")
 
 
 def uner_stats(hist):
@@ -163,8 +205,9 @@ def mixed_error(snf, ic, grid, Re, explicit, nstep):
 def batch_leg(snf, make_ic, nbox, steps, seed):
     """Ensemble of independent boxes (grids n^3 and n x (n+1) x n, n cycling
     over 32, 48, 64, 96; Re_z uniform in [500, 3000]; seeded fields): box-steps
-    per second with every box's steps in flight at once (snf_advance_batch),
-    against the same boxes advanced one after another."""
+    per second of snf_advance_batch (each size class advanced by one launch
+    per kernel from box tables), against the same boxes advanced one after
+    another and against one stream per box."""
     import numpy as np
     rng = np.random.default_rng(seed)
     ns = (32, 48, 64, 96)
@@ -184,7 +227,11 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
         return out
 
     rates = {}
-    for mode, prec in (("sequential", "fp64"), ("batched", "fp64"), ("batched_mixed", "mixed")):
+    for mode, prec in (("sequential", "fp64"), ("batched", "fp64"), ("batched_mixed", "mixed"),
+                       ("batched_streams", "fp64")):
+        # batched: one launch per kernel for each size class (box tables);
+        # batched_streams: every box on its own stream, for comparison
+        os.environ["SNF_BATCH_TABLES"] = "0" if mode == "batched_streams" else "1"
         bs = boxes(prec)
         snf.advance_batch(bs, 2)                        # warm-up
         t0 = time.perf_counter()
@@ -200,6 +247,8 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
             "unit": "box-steps/s", "dtype": "f64 (batched_mixed: the mixed precision mode)",
             "batched": rates["batched"], "sequential": rates["sequential"],
             "speedup": rates["batched"] / rates["sequential"],
+            "batched_streams": rates["batched_streams"],
+            "tables_vs_streams": rates["batched"] / rates["batched_streams"],
             "batched_mixed": rates["batched_mixed"],
             "mixed_vs_fp64_batched": rates["batched_mixed"] / rates["batched"]}
 
@@ -216,6 +265,19 @@ def main():
         print("bench.py: one GPU only; the sharded multi-GPU path is designed, not built "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    mgalx, my, mgalz = (int(x) for x in a.grid.split(","))
+    seed = 20260001
+    cpu = None
+    if not a.no_cpu and a.copy_probe <= 0:
+        # before anything touches the GPU: the workers are fresh processes
+        if a.cpu_grid:
+            cg = tuple(int(x) for x in a.cpu_grid.split(","))
+        else:
+            cg = (max(mgalx // 2, 32), max((my - 1) // 2 + 1, 33), max(mgalz // 2, 32))
+        procs = max(1, min(a.cpu_procs, os.cpu_count() or 1))
+        v, sample = cpu_baseline((mgalx, my, mgalz), cg, a.Re, a.explicit, seed, procs)
+        cpu = {"value": v, "unit": "steps/s", "cores": procs, "kind": "port",
+               "sample": sample + " (numpy FFT and line solves)"}
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import shear_nnf_amd as snf
     from shear_nnf_amd.ic import make_ic
@@ -225,8 +287,6 @@ def main():
         print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms,
                           "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
-    mgalx, my, mgalz = (int(x) for x in a.grid.split(","))
-    seed = 20260001
     ic = make_ic(mgalx, my, mgalz, math.pi, 2 * math.pi, 1.0, seed=seed)
     grid = (mgalx, my, mgalz)
     # timed region: exactly K steps in one advance call, which synchronises the
@@ -290,15 +350,8 @@ def main():
         }
     if a.batch > 0 and a.precision == "fp64":
         out["ensemble"] = batch_leg(snf, make_ic, a.batch, a.batch_steps, seed)
-    if not a.no_cpu:
-        if a.cpu_grid:
-            cg = tuple(int(x) for x in a.cpu_grid.split(","))
-        else:
-            cg = (max(mgalx // 2, 32), max((my - 1) // 2 + 1, 33), max(mgalz // 2, 32))
-        v, sample = cpu_baseline((mgalx, my, mgalz), cg, a.Re, a.explicit, seed)
-        ncores = 1
-        out["cpu_baseline"] = {"value": v, "unit": "steps/s", "cores": ncores, "kind": "port",
-                               "sample": sample + " (numpy FFT and line solves, single thread)"}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -182,6 +182,7 @@ struct snf_handle {
   // spectral y stage (explicit substeps) for my with factors 2,3,5,7,11
   bool yfft = false;
   bool ysym_on = false;   // multiplier tables maintained (spectral y or filter)
+  bool ymfma = false;     // line lengths without a radix plan: DFT on the matrix cores
   cplx* twy = nullptr;
   cplx* tpy = nullptr;                  // pass-major twiddles of the y plan
   cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
@@ -459,8 +460,9 @@ ScalarArgs sargs(snf_handle* h) {
 // my; y_plan_radices gives the same radices to the host twiddle table.
 template <class PL> struct PlanTag { typedef PL type; };
 #define Y_DIRECT_MAX 144
+#define Y_DFT_MFMA_DEFAULT false
 template <class F>
-void with_yplan(int my, F&& f) {
+void with_yplan(int my, bool mfma, F&& f) {
   switch (my) {
     case 32: f(PlanTag<YFFT<8, 4>>{}); break;
     case 33: f(PlanTag<YFFT<11, 3>>{}); break;
@@ -474,10 +476,18 @@ void with_yplan(int my, F&& f) {
     case 256: f(PlanTag<YFFT<8, 8, 4>>{}); break;
     case 385: f(PlanTag<YFFT<11, 7, 5>>{}); break;
     default:
-      switch ((my + 63) / 64) {
-        case 1: f(PlanTag<YDirect<1>>{}); break;
-        case 2: f(PlanTag<YDirect<2>>{}); break;
-        default: f(PlanTag<YDirect<3>>{}); break;
+      if (mfma) {
+        switch ((my + 63) / 64) {
+          case 1: f(PlanTag<YMfma<1>>{}); break;
+          case 2: f(PlanTag<YMfma<2>>{}); break;
+          default: f(PlanTag<YMfma<3>>{}); break;
+        }
+      } else {
+        switch ((my + 63) / 64) {
+          case 1: f(PlanTag<YDirect<1>>{}); break;
+          case 2: f(PlanTag<YDirect<2>>{}); break;
+          default: f(PlanTag<YDirect<3>>{}); break;
+        }
       }
   }
 }
@@ -514,7 +524,7 @@ void launch_yf_first_plan(snf_handle* h, unsigned nb, const YArgs& a) {
 void launch_yf_first(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
-  with_yplan(h->g.my, [&](auto tag) {
+  with_yplan(h->g.my, h->ymfma, [&](auto tag) {
     launch_yf_first_plan<typename decltype(tag)::type>(h, nb, a);
   });
 }
@@ -522,7 +532,7 @@ void launch_yf_first(snf_handle* h) {
 void launch_yf(snf_handle* h, int rk) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
-  with_yplan(h->g.my, [&](auto tag) {
+  with_yplan(h->g.my, h->ymfma, [&](auto tag) {
     launch_yf_plan<typename decltype(tag)::type>(h, rk, nb, a);
   });
 }
@@ -568,7 +578,7 @@ void launch_yfilter(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   const YArgs a = yargs(h);
   if (h->yfft) {
-    with_yplan(h->g.my, [&](auto tag) {
+    with_yplan(h->g.my, h->ymfma, [&](auto tag) {
       k_yf_filter<typename decltype(tag)::type><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
     });
   } else {
@@ -663,7 +673,7 @@ void tab_y_plan(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t 
   }
 }
 void tab_y(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
-  with_yplan(h->g.my, [&](auto tag) { tab_y_plan<typename decltype(tag)::type>(h, t, n, rk, s); });
+  with_yplan(h->g.my, h->ymfma, [&](auto tag) { tab_y_plan<typename decltype(tag)::type>(h, t, n, rk, s); });
 }
 
 // One RK3 step (st = 0 or 1 of the pair the tables hold) of every box of a
@@ -1095,6 +1105,12 @@ int snf_create(const snf_params* p, snf_handle** out) {
     const char* e = getenv("SNF_Y_PATH");
     h->yfft = y_spectral(my) && !(e && std::string(e) == "banded");
     h->ysym_on = h->yfft || p->ydealias == 3;
+    // the direct DFT of line lengths without a radix plan: on the matrix
+    // cores (YMfma) or as per-wave sums (YDirect); SNF_Y_DFT=mfma|direct
+    const char* ed = getenv("SNF_Y_DFT");
+    h->ymfma = Y_DFT_MFMA_DEFAULT;
+    if (ed && std::string(ed) == "mfma") h->ymfma = true;
+    if (ed && std::string(ed) == "direct") h->ymfma = false;
     std::vector<cplx> ty(my);
     for (int t = 0; t < my; ++t) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / my;
@@ -1125,7 +1141,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // tile, a column per wave, roots, two multiplier rows, four phase tables
     // (1472 B at my = 385: 81552 B, two blocks still fit one CU's 160 KiB)
     h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
-    with_yplan(my, [&](auto tag) {
+    with_yplan(my, h->ymfma, [&](auto tag) {
       typedef typename decltype(tag)::type PL;
       for (const void* ky : {(const void*)k_yf_explicit<PL>, (const void*)k_yf_implicit<PL>,
                              (const void*)k_yf_first<PL>, (const void*)k_yf_filter<PL>})

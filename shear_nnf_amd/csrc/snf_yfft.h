@@ -65,10 +65,13 @@ struct TileCol {
   __device__ __forceinline__ int operator()(int j) const {
     return j * YF_LINES + (c ^ yf_swz(j));
   }
+  // the same kind of column of line L of the block (this column is line w's)
+  __device__ __forceinline__ TileCol peer(int L, int, int) const { return TileCol{L}; }
 };
 struct OwnCol {
   int base;
   __device__ __forceinline__ int operator()(int j) const { return base + j; }
+  __device__ __forceinline__ OwnCol peer(int L, int w, int N) const { return OwnCol{base + (L - w) * N}; }
 };
 
 // ---- small odd DFTs by symmetric pairs: y_k = sum_j x_j exp(SIGN 2 pi i jk/R)
@@ -241,6 +244,83 @@ __device__ __forceinline__ void yf_dft_direct(cplx* T, Col col, const cplx* rt, 
   wf_sync();
 }
 
+// Direct DFT of the block's YF_LINES columns on the matrix cores: a
+// block-collective form of yf_dft_direct, called by every wave of the block
+// with its own column.  With W[m][j] = exp(SIGN 2 pi i m j / N) and the four
+// lines' inputs as the columns of B = [Re x_0..3 | Im x_0..3 | 0] (N x 16),
+// each wave forms P = Re W . B and Q = Im W . B for its tiles of 16 modes
+// with v_mfma_f64_16x16x4f64 (K = 4 inputs a step), and
+//   Re X_L = P[., L] - Q[., 4 + L],   Im X_L = P[., 4 + L] + Q[., L].
+// Operand layout of the instruction (A 16 x 4, B 4 x 16): lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; result value r of lane l is
+// D[4 r + l / 16][l % 16].  W comes from the root table rt[t] = exp(-2 pi i
+// t / N) with t = m j mod N advanced by 4 m per step.  Half of each
+// instruction's 16 columns carries data (4 lines, re and im).
+typedef double yf_v4d __attribute__((ext_vector_type(4)));
+template <int SIGN, int QM, class Col>
+__device__ __forceinline__ void yf_dft_mfma(cplx* T, Col col, const cplx* rt, int N, bool active) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, kq = lane >> 4;
+  // which lines take part (a wave may hold its column out)
+  __shared__ int act[YF_LINES];
+  if (lane == 0) act[w] = active ? 1 : 0;
+  __syncthreads();   // every wave's column is written
+  const bool take = act[i & 3] != 0;     // line of this lane's B column and output
+  const int mt = (N + 15) / 16;                // tiles of 16 modes; this wave takes w, w + 4, ...
+  const int ks = (N + 3) / 4;                  // steps of 4 inputs
+  yf_v4d P[QM], Q[QM];
+  int idx[QM], stp[QM];
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    P[t] = yf_v4d{0.0, 0.0, 0.0, 0.0};
+    Q[t] = yf_v4d{0.0, 0.0, 0.0, 0.0};
+    const int m = (w + 4 * t) * 16 + i;
+    idx[t] = (m * kq) % N;                     // t = m j mod N at j = kq
+    stp[t] = (4 * m) % N;
+  }
+  const Col src = col.peer(i & 3, w, N);       // column of line (n mod 4) for B
+  for (int k = 0; k < ks; ++k) {
+    const int j = 4 * k + kq;
+    double b = 0.0;
+    if (i < 8 && j < N && take) {
+      const cplx x = T[src(j)];
+      b = i < 4 ? x.x : x.y;
+    }
+#pragma unroll
+    for (int t = 0; t < QM; ++t) {
+      const int tile = w + 4 * t;
+      if (tile < mt) {
+        const int m = tile * 16 + i;
+        double ar = 0.0, ai = 0.0;
+        if (m < N && j < N) {
+          const cplx r = rt[idx[t]];
+          ar = r.x;
+          ai = SIGN < 0 ? r.y : -r.y;
+        }
+        P[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b, P[t], 0, 0, 0);
+        Q[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b, Q[t], 0, 0, 0);
+        idx[t] += stp[t];
+        if (idx[t] >= N) idx[t] -= N;
+      }
+    }
+  }
+  __syncthreads();   // every input read before any output lands
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    const int tile = w + 4 * t;
+    if (tile < mt) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double pr = P[t][r], qr = Q[t][r];
+        const double p4 = __shfl_down(pr, 4, 16), q4 = __shfl_down(qr, 4, 16);
+        const int m = tile * 16 + 4 * r + kq;
+        if (i < 4 && m < N && take) T[col.peer(i, w, N)(m)] = mkc(pr - q4, p4 + qr);
+      }
+    }
+  }
+  __syncthreads();   // the columns of the other waves are written
+}
+
 // Plans of the y-line transforms, the template argument of the y kernels.
 //   YFFT<Rs...>  mixed-radix Stockham passes, N = prod Rs at compile time,
 //                twiddles from the pass-major table;
@@ -250,9 +330,16 @@ template <int... Rs> struct YFFT {
   static constexpr int Q = (YPlanT<Rs...>::N + 63) / 64;
   static constexpr bool direct = false;
   static constexpr int NC = YPlanT<Rs...>::N;   // line length (compile time)
+  // active = false: this wave's column stays as it is (the call keeps the
+  // waves of a block in step for the block-collective plan)
   template <int SIGN, class Col>
-  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* tw, int) {
-    yf_fft<SIGN, Col, Rs...>(T, c, tw);
+  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* tw, int,
+                                             bool active = true) {
+    if (active) yf_fft<SIGN, Col, Rs...>(T, c, tw);
+  }
+  template <int SIGN, class Col>   // one wave on its own (the 00 lines)
+  __device__ __forceinline__ static void fft_wave(cplx* T, Col c, const cplx* tw, int n) {
+    fft<SIGN>(T, c, tw, n);
   }
 };
 template <int QM> struct YDirect {
@@ -260,7 +347,28 @@ template <int QM> struct YDirect {
   static constexpr bool direct = true;
   static constexpr int NC = 0;                  // line length: my, at run time
   template <int SIGN, class Col>
-  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N) {
+  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N,
+                                             bool active = true) {
+    if (active) yf_dft_direct<SIGN, QM>(T, c, rt, N);
+  }
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft_wave(cplx* T, Col c, const cplx* rt, int N) {
+    yf_dft_direct<SIGN, QM>(T, c, rt, N);
+  }
+};
+// the direct DFT on the matrix cores (block-collective: every wave of a
+// line block calls fft at the same point); the 00 lines keep the per-wave sum
+template <int QM> struct YMfma {
+  static constexpr int Q = QM;
+  static constexpr bool direct = true;
+  static constexpr int NC = 0;
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N,
+                                             bool active = true) {
+    yf_dft_mfma<SIGN, QM>(T, c, rt, N, active);
+  }
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft_wave(cplx* T, Col c, const cplx* rt, int N) {
     yf_dft_direct<SIGN, QM>(T, c, rt, N);
   }
 };

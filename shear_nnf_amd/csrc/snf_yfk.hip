@@ -212,7 +212,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
         T[ac(j)] = x[q];
       }
     }
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -220,7 +220,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
         T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m + 1));
       }
     }
-    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -234,7 +234,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
         T[ac(j)] = mkc(un, 0.0);
       }
     }
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -242,7 +242,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
         T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m));
       }
     }
-    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -456,69 +456,53 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   // v = lapcdy(phi) and dv = D1(v) with the new twist; the (0,0) mode keeps
   // its v (lapcdy is not called for it) and only takes dv = D1(v)
   if (L0 == 0) yf_fill(T, a.v, P, L0, nv, N);   // block of the (0,0) mode
-  if (!is00) {
-    {
-      YPh<true> ph(phn, lane);
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int j = lane + 64 * q;
-        const cplx e = ph.at(q);
-        if (j < N) T[ac(j)] = cmul(x[q], e);
-      }
-    }
-    PL::template fft<-1>(T, ac, rt, N);
+  // every wave of the block reaches the same transforms in the same order
+  // (the matrix-core plan transforms the block's columns together); the
+  // (0,0) mode takes no Helmholtz solve and leaves out the last transform
+  {
+    YPh<true> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int m = lane + 64 * q;
-      if (m < N) {
-        const cplx s1 = syn[m], s2 = syn[N + m];
+      const int j = lane + 64 * q;
+      const cplx e = ph.at(q);
+      if (j < N) {
+        if (is00) x[q] = T[tc(j)];             // v of the (0,0) mode, unchanged
+        T[ac(j)] = cmul(x[q], e);
+      }
+    }
+  }
+  PL::template fft<-1>(T, ac, rt, N);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = lane + 64 * q;
+    if (m < N) {
+      const cplx s1 = syn[m], s2 = syn[N + m];
+      if (!is00) {
         const cplx H = yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y));   // lapcdy
         const cplx vh = cmul(T[ac(m)], H);
         T[tc(m)] = cmul(vh, s1);
         T[ac(m)] = vh;
+      } else {
+        T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
       }
     }
-    PL::template fft<1>(T, ac, rt, N);
-    PL::template fft<1>(T, tc, rt, N);
+  }
+  PL::template fft<1>(T, ac, rt, N);
+  PL::template fft<1>(T, tc, rt, N, !is00);
+  {
     YPh<false> ph(phn, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
       const cplx e = ph.at(q);
       if (j < N) {
-        x[q] = cmul(T[ac(j)], e);              // v
-        T[tc(j)] = cmul(T[tc(j)], e);          // dv
-      }
-    }
-  } else {
-    {
-      YPh<true> ph(phn, lane);
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int j = lane + 64 * q;
-        const cplx e = ph.at(q);
-        if (j < N) {
-          x[q] = T[tc(j)];                     // v of the (0,0) mode, unchanged
-          T[ac(j)] = cmul(x[q], e);
+        if (!is00) {
+          x[q] = cmul(T[ac(j)], e);            // v
+          T[tc(j)] = cmul(T[tc(j)], e);        // dv
+        } else {
+          T[tc(j)] = cmul(T[ac(j)], e);        // dv
         }
       }
-    }
-    PL::template fft<-1>(T, ac, rt, N);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int m = lane + 64 * q;
-      if (m < N) {
-        const cplx s1 = syn[m];
-        T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
-      }
-    }
-    PL::template fft<1>(T, ac, rt, N);
-    YPh<false> ph(phn, lane);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int j = lane + 64 * q;
-      const cplx e = ph.at(q);
-      if (j < N) T[tc(j)] = cmul(T[ac(j)], e);   // dv
     }
   }
   yf_flush(T, a.dv, P, L0, nv, N);
@@ -598,13 +582,13 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
         T[ac(j)] = x[q];
       }
     }
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = cmul(T[ac(m)], gld(sy + YS * m + 1));
     }
-    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
     // wk = facb (gama rf [+ wk] + alpha/Re D2 U + U/dt)
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -619,7 +603,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
       }
     }
     // U = lapcdy(wk, dtr1); dU/dy
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -630,8 +614,8 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
         T[tc(m)] = cmul(uh, s1);
       }
     }
-    PL::template fft<1>(T, ac, rt, N);
-    PL::template fft<1>(T, tc, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, tc, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -911,7 +895,11 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
         T[ac(j)] = cmul(x[q], e);
       }
     }
-    PL::template fft<-1>(T, ac, rt, N);
+  }
+  // every wave reaches this transform (the matrix-core plan transforms the
+  // block's columns together); only the (0,0) mode's column takes part
+  PL::template fft<-1>(T, ac, rt, N, is00);
+  if (is00) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
@@ -984,13 +972,13 @@ k_yf_first(YArgs a) {
       const int j = lane + 64 * q;
       if (j < N) T[ac(j)] = gld(u + j);
     }
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, gld(sy + YS * m)));
     }
-    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -1138,13 +1126,13 @@ k_yf_filter(YArgs a) {
       const int j = lane + 64 * q;
       if (j < N) T[ac(j)] = gld(u + j);
     }
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = crmul(invN * gld(sy + YS * m + 2).x, T[ac(m)]);
     }
-    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;
@@ -1154,13 +1142,13 @@ k_yf_filter(YArgs a) {
         T[ac(j)] = uf;
       }
     }
-    PL::template fft<-1>(T, ac, rt, N);
+    PL::template fft_wave<-1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, gld(sy + YS * m)));
     }
-    PL::template fft<1>(T, ac, rt, N);
+    PL::template fft_wave<1>(T, ac, rt, N);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int j = lane + 64 * q;

@@ -360,3 +360,18 @@ def test_radix_and_banded_y_paths_agree(lib, monkeypatch, grid):
         outs.append(box.get_state())
     for a, b in zip(*outs):
         assert rel_l2(a, b) < STEP_TOL
+
+
+# the direct DFT of line lengths without a radix plan on the matrix cores
+# (v_mfma_f64_16x16x4f64, the block's four lines transformed together):
+# same bar against the oracle as the per-wave sums it can replace
+@pytest.mark.parametrize("grid,explicit,ydealias", [((48, 40, 48), 1, 0), ((32, 97, 32), 0, 0),
+                                                    ((64, 97, 64), 1, 0), ((32, 129, 32), 1, 3)])
+def test_cross_mfma_dft_y_3_steps(lib, monkeypatch, grid, explicit, ydealias):
+    monkeypatch.setenv("SNF_Y_DFT", "mfma")
+    st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 3, grid=grid, ydealias=ydealias)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
