@@ -48,29 +48,33 @@ def parse():
 
 
 def algorithmic_bytes(kxn, mz, my, mgalz, precision="fp64"):
-    """compulsory HBM bytes per launch of each kernel class: FP64 complex
-    spectral values (16 B) and hybrid values of 16 B (FP64) or 8 B (mixed).
-    Nc = kxn*mz*my spectral coefficients, Nh = kxn*mgalz*my hybrid values."""
+    """compulsory HBM bytes per launch of each kernel class.  Nc = kxn*mz*my
+    spectral coefficients, Nh = kxn*mgalz*my hybrid values.  FP64: every
+    field 16 B a value.  Mixed: hybrid planes and the six inter-stage fields
+    (v, dv/dy, d(omega_y)/dy; rdome, rhg, rhv) 8 B, the state and the RK
+    accumulators 16 B."""
     nc = kxn * mz * my
     nh = kxn * mgalz * my
-    hb = 16.0 if precision == "fp64" else 8.0
+    hb = 16.0 if precision == "fp64" else 8.0   # hybrid planes
+    ib = 16.0 if precision == "fp64" else 8.0   # inter-stage fields
     return {
         # A: reads vor, phi, v, dv/dy, d(omega_y)/dy; writes 6 hybrid fields
-        "xz_A": 16.0 * 5 * nc + hb * 6 * nh,
+        "xz_A": 16.0 * 2 * nc + ib * 3 * nc + hb * 6 * nh,
         # B: reads 6 hybrid fields, writes 3
         "xz_B": hb * 9 * nh,
         # C: reads 3 hybrid fields and v, writes 3 spectral fields
-        "xz_C": hb * 3 * nh + 16.0 * 4 * nc,
-        # fused y stage, averaged over the 3 substeps: reads dvordy, hv, hg,
+        "xz_C": hb * 3 * nh + ib * 4 * nc,
+        # fused y stage, averaged over the 3 substeps: reads rdome, rhv, rhg,
         # vor, phi (+ vorwk, phiwk at rk 2,3); writes vor, phi, vorwk,
-        # phiwk (not at rk 3), hg, dvordy, hv
-        "y_lines": 16.0 * nc * (12 + 14 + 12) / 3.0,
+        # phiwk (not at rk 3), v, dv/dy, d(omega_y)/dy
+        "y_lines": nc * (6 * ib + 4 * 16.0 + 4 * 16.0 * 2.0 / 3.0),
     }
 
 
-def step_bytes(nc):
-    """B64 = 992 Nc: compulsory bytes of one RK3 step with planes on chip"""
-    return 992.0 * nc
+def step_bytes(nc, precision="fp64"):
+    """compulsory bytes of one RK3 step with the physical planes on chip:
+    B64 = 992 Nc (FP64), B_mix = 704 Nc (mixed: FP32 inter-stage fields)"""
+    return (992.0 if precision == "fp64" else 704.0) * nc
 
 
 def measured_traffic(cls, grid, explicit, precision="fp64"):
@@ -327,9 +331,9 @@ def main():
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": ab[dom],
                      "avg_launch_ms": per_cls[dom]},
-        "step_roofline": {"bytes_per_step_B64": step_bytes(nc),
-                          "achieved_GBs": step_bytes(nc) * sps / 1e9,
-                          "frac": step_bytes(nc) * sps / 1e9 / HBM_PEAK_GBS},
+        "step_roofline": {"bytes_per_step": step_bytes(nc, a.precision),
+                          "achieved_GBs": step_bytes(nc, a.precision) * sps / 1e9,
+                          "frac": step_bytes(nc, a.precision) * sps / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_per_launch": per_cls,
         "kernel_ms_per_step": {k: timing[k][0] / max(st.steps_done, 1) for k in timing},
         "sim_time_advanced": st2.time,
@@ -344,7 +348,8 @@ def main():
             "dtype": "f32 x-z transforms, hybrid planes and u x omega; f64 state, "
                      "KMM reconstruction, y operators, RK update, statistics sums",
             "kernel_ms_per_launch": {k: ms / cnt for k, (ms, cnt) in timm.items() if cnt},
-            "step_roofline_frac": step_bytes(nc) * spsm / 1e9 / HBM_PEAK_GBS,
+            "step_roofline_frac": step_bytes(nc, "mixed") * spsm / 1e9 / HBM_PEAK_GBS,
+            "step_bytes_B_mix": step_bytes(nc, "mixed"),
             "rel_err_vs_fp64": mixed_error(snf, ic, grid, a.Re, a.explicit, a.cmp_steps)
             if a.cmp_steps > 0 else None,
         }

@@ -509,17 +509,28 @@ int y_plan_radices(int my, int* R) {
 }
 
 // the spectral y stage of one substep: explicit or implicit viscous terms
+// IO: the storage of the inter-stage fields (Geom::io32: 0 FP64, 3 FP32)
+template <class PL, int IO>
+void launch_yf_plan_io(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
+  if (h->p.explicit_visc == 1)
+    k_yf_explicit<PL, IO><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+  else
+    k_yf_implicit<PL, IO><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+}
 template <class PL>
 void launch_yf_plan(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
-  if (h->p.explicit_visc == 1)
-    k_yf_explicit<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+  if (h->g.io32)
+    launch_yf_plan_io<PL, 3>(h, rk, nb, a);
   else
-    k_yf_implicit<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+    launch_yf_plan_io<PL, 0>(h, rk, nb, a);
 }
 // start of a cross call: v, dv, dome (and dU/dy, dW/dy) at the current twist
 template <class PL>
 void launch_yf_first_plan(snf_handle* h, unsigned nb, const YArgs& a) {
-  k_yf_first<PL><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
+  if (h->g.io32)
+    k_yf_first<PL, 3><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
+  else
+    k_yf_first<PL, 0><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
 }
 void launch_yf_first(snf_handle* h) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
@@ -659,18 +670,25 @@ void tab_B(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
   else
     tab_B_nt<1024, H>(h, t, n, rk, s);
 }
-template <class PL>
-void tab_y_plan(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
+template <class PL, int IO>
+void tab_y_plan_io(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
   const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
   if (h->p.explicit_visc == 1) {
-    lds_attr((const void*)k_yf_explicit_tab<PL>, h->ldsY, h->dev);
+    lds_attr((const void*)k_yf_explicit_tab<PL, IO>, h->ldsY, h->dev);
     if (n == 0) return;
-    k_yf_explicit_tab<PL><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
+    k_yf_explicit_tab<PL, IO><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
   } else {
-    lds_attr((const void*)k_yf_implicit_tab<PL>, h->ldsY, h->dev);
+    lds_attr((const void*)k_yf_implicit_tab<PL, IO>, h->ldsY, h->dev);
     if (n == 0) return;
-    k_yf_implicit_tab<PL><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
+    k_yf_implicit_tab<PL, IO><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
   }
+}
+template <class PL>
+void tab_y_plan(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
+  if (h->g.io32)
+    tab_y_plan_io<PL, 3>(h, t, n, rk, s);
+  else
+    tab_y_plan_io<PL, 0>(h, t, n, rk, s);
 }
 void tab_y(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
   with_yplan(h->g.my, h->ymfma, [&](auto tag) { tab_y_plan<typename decltype(tag)::type>(h, t, n, rk, s); });
@@ -1105,6 +1123,9 @@ int snf_create(const snf_params* p, snf_handle** out) {
     const char* e = getenv("SNF_Y_PATH");
     h->yfft = y_spectral(my) && !(e && std::string(e) == "banded");
     h->ysym_on = h->yfft || p->ydealias == 3;
+    // the mixed precision mode with a spectral y stage stores the six
+    // inter-stage fields as FP32 (the banded sweeps read them as FP64)
+    h->g.io32 = (h->mixed && h->yfft) ? 3 : 0;
     // the direct DFT of line lengths without a radix plan: on the matrix
     // cores (YMfma) or as per-wave sums (YDirect); SNF_Y_DFT=mfma|direct
     const char* ed = getenv("SNF_Y_DFT");
@@ -1143,8 +1164,10 @@ int snf_create(const snf_params* p, snf_handle** out) {
     h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
     with_yplan(my, h->ymfma, [&](auto tag) {
       typedef typename decltype(tag)::type PL;
-      for (const void* ky : {(const void*)k_yf_explicit<PL>, (const void*)k_yf_implicit<PL>,
-                             (const void*)k_yf_first<PL>, (const void*)k_yf_filter<PL>})
+      for (const void* ky : {(const void*)k_yf_explicit<PL, 0>, (const void*)k_yf_implicit<PL, 0>,
+                             (const void*)k_yf_first<PL, 0>, (const void*)k_yf_explicit<PL, 3>,
+                             (const void*)k_yf_implicit<PL, 3>, (const void*)k_yf_first<PL, 3>,
+                             (const void*)k_yf_filter<PL>})
         lds_attr(ky, h->ldsY, h->dev);
     });
     if (!h->yfft && p->ydealias == 3)
@@ -2086,37 +2109,58 @@ int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const do
   HIPCHK(hipSetDevice(h->dev));
   const Geom& g = h->g;
   const size_t NC = h->NC;
+  // inter-stage fields: FP64, or FP32 complex (g.io32, the mixed mode):
+  // bit 0 for the three inputs, bit 1 for the three outputs
+  const int in32 = g.io32 & 1, out32 = g.io32 & 2;
+  const size_t es = in32 ? sizeof(cplxf) : sizeof(cplx);
+  auto put = [&](cplx* dst, const double* src) -> int {
+    if (!in32) {
+      HIPCHK(hipMemcpy(dst, src, NC * sizeof(cplx), hipMemcpyHostToDevice));
+      return SNF_OK;
+    }
+    std::vector<float> f(2 * NC);
+    for (size_t e = 0; e < 2 * NC; ++e) f[e] = (float)src[e];
+    HIPCHK(hipMemcpy(dst, f.data(), NC * sizeof(cplxf), hipMemcpyHostToDevice));
+    return SNF_OK;
+  };
+  auto get = [&](double* dst, const cplx* src) -> int {
+    if (!out32) {
+      HIPCHK(hipMemcpy(dst, src, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+      return SNF_OK;
+    }
+    std::vector<float> f(2 * NC);
+    HIPCHK(hipMemcpy(f.data(), src, NC * sizeof(cplxf), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < 2 * NC; ++e) dst[e] = (double)f[e];
+    return SNF_OK;
+  };
   // The evaluation takes v, dv/dy, d(omega_y)/dy in the handle's own
   // buffers.  The next advance re-forms them from the state except on the
   // (0,0) line, which no y stage writes and the dissipation sum reads: keep
   // those three lines.
   HIPCHK(hipStreamSynchronize(h->stream));
-  std::vector<cplx> keep00(3 * (size_t)g.my);
+  std::vector<char> keep00(3 * (size_t)g.my * es);
   cplx* bufs[3] = {h->v, h->dv, h->dome};
   for (int f = 0; f < 3; ++f)
-    HIPCHK(hipMemcpy2D(keep00.data() + (size_t)f * g.my, sizeof(cplx), bufs[f], g.P * sizeof(cplx),
-                       sizeof(cplx), g.my, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(h->v, hg_v, NC * sizeof(cplx), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->dv, hv_dv, NC * sizeof(cplx), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->dome, dvordy, NC * sizeof(cplx), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy2D(keep00.data() + (size_t)f * g.my * es, es, bufs[f], g.P * es, es, g.my,
+                       hipMemcpyDeviceToHost));
+  int rc;
+  if ((rc = put(h->v, hg_v)) || (rc = put(h->dv, hv_dv)) || (rc = put(h->dome, dvordy))) return rc;
   // the run scalars of the handle (a cross call that returned at a dump
   // continues after this evaluation): k_begin_call below resets them
   DevScalars keep{};
-  HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(&keep, h->sc, sizeof(keep), hipMemcpyDeviceToHost));
   // U, W derivatives as at the top of hvhg (cross.f90:1198-1205)
   k_begin_call<<<1, 256, 0, h->stream>>>(g, h->sc, h->shp[h->cur], h->shm[h->cur], 0);
   k_d1_00<<<1, 64, 0, h->stream>>>(yargs(h));
-  int rc;
   if ((rc = launch_xz(h, 2))) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(h->sc, &keep, sizeof(keep), hipMemcpyHostToDevice));
   for (int f = 0; f < 3; ++f)
-    HIPCHK(hipMemcpy2D(bufs[f], g.P * sizeof(cplx), keep00.data() + (size_t)f * g.my, sizeof(cplx),
-                       sizeof(cplx), g.my, hipMemcpyHostToDevice));
-  if (out_dvordy) HIPCHK(hipMemcpy(out_dvordy, h->rdome, NC * sizeof(cplx), hipMemcpyDeviceToHost));
-  if (out_hg) HIPCHK(hipMemcpy(out_hg, h->rhg, NC * sizeof(cplx), hipMemcpyDeviceToHost));
-  if (out_hv) HIPCHK(hipMemcpy(out_hv, h->rhv, NC * sizeof(cplx), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy2D(bufs[f], g.P * es, keep00.data() + (size_t)f * g.my * es, es, es, g.my,
+                       hipMemcpyHostToDevice));
+  if (out_dvordy && (rc = get(out_dvordy, h->rdome))) return rc;
+  if (out_hg && (rc = get(out_hg, h->rhg))) return rc;
+  if (out_hv && (rc = get(out_hv, h->rhv))) return rc;
   if (rf0u) HIPCHK(hipMemcpy(rf0u, h->rf0u, g.my * sizeof(double), hipMemcpyDeviceToHost));
   if (rf0w) HIPCHK(hipMemcpy(rf0w, h->rf0w, g.my * sizeof(double), hipMemcpyDeviceToHost));
   return SNF_OK;

@@ -79,7 +79,24 @@ struct Geom {
   int my, kxn, mz, nz, mgalx, mgalz;
   long P;            // kxn*mz: complex coefficients per y-plane (= number of y-lines)
   double alp, gam, Ly, dy, s, chi, Re;
+  // Storage of the fields between the two stages of a substep, bit 0: v,
+  // dv/dy, d(omega_y)/dy (y stage -> x-z stage), bit 1: rdome, rhg, rhv
+  // (x-z stage -> y stage); a set bit stores those three as FP32 complex
+  // (the mixed precision mode with a spectral y stage).  Arithmetic on them
+  // stays FP64.
+  int io32;
 };
+
+// loads and stores of the inter-stage fields in the storage the grid says
+__device__ __forceinline__ cplx ld_io(const cplx* f, long idx, int io32) {
+  return io32 ? to_d(gld(reinterpret_cast<const cplxf*>(f) + idx)) : gld(f + idx);
+}
+__device__ __forceinline__ void st_io(cplx* f, long idx, cplx v, int io32) {
+  if (io32)
+    gst(reinterpret_cast<cplxf*>(f) + idx, to_h<cplxf>(v));
+  else
+    gst(f + idx, v);
+}
 
 // Compact-scheme stencils (offsets -3..3) and untwisted base factors.
 struct YCoef {

@@ -43,9 +43,9 @@ __device__ __forceinline__ PlaneIn load_in(const Geom& g, const cplx* vor, const
     r.phic = phi[idx];
     r.ome2c = vor[idx];
   }
-  r.v = hg[idx];
-  r.dv = hv[idx];
-  r.dome = dvordy[idx];
+  r.v = ld_io(hg, idx, g.io32 & 1);
+  r.dv = ld_io(hv, idx, g.io32 & 1);
+  r.dome = ld_io(dvordy, idx, g.io32 & 1);
   return r;
 }
 
@@ -343,12 +343,12 @@ __device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, 
     const cplx h3 = crmul(norm, to_d(r[2 * Nz + pos]));
     const double kz = kz_of(k, g.nz, g.mz, g.gam);
     const long idx = base + k;
-    const cplx v = vel[idx];
-    dvordy[idx] = cadd(mul_i(kx, h1), mul_i(kz, h3));
+    const cplx v = ld_io(vel, idx, g.io32 & 1);
+    st_io(dvordy, idx, cadd(mul_i(kx, h1), mul_i(kz, h3)), g.io32 & 2);
     const cplx d = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);
-    hg[idx] = cadd(mul_i(-kx, h3), mul_i(kz, d));
+    st_io(hg, idx, cadd(mul_i(-kx, h3), mul_i(kz, d)), g.io32 & 2);
     const double kk = kx2 + kz * kz;
-    hv[idx] = mkc(-h2.x * kk, -h2.y * kk);
+    st_io(hv, idx, mkc(-h2.x * kk, -h2.y * kk), g.io32 & 2);
     if (i == 0 && k == 0) {
       rf0u[j] = h1.x;
       rf0w[j] = h3.x;

@@ -76,9 +76,9 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
         q.phic = gld(phi + base + k);
         q.ome2c = gld(vor + base + k);
       }
-      q.v = gld(hg + base + k);
-      q.dv = gld(hv + base + k);
-      q.dome = gld(dvordy + base + k);
+      q.v = ld_io(hg, base + k, g.io32 & 1);
+      q.dv = ld_io(hv, base + k, g.io32 & 1);
+      q.dome = ld_io(dvordy, base + k, g.io32 & 1);
       cplx o[6];
       derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, o);
 #pragma unroll
@@ -343,12 +343,12 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
                  h3 = crmul(norm, to_d(h[2][m]));
       const double kz = kz_of(k, g.nz, g.mz, g.gam);
       const long idx = base + k;
-      const cplx v = gld(vel + idx);
-      gst(dvordy + idx, cadd(mul_i(kx, h1), mul_i(kz, h3)));
+      const cplx v = ld_io(vel, idx, g.io32 & 1);
+      st_io(dvordy, idx, cadd(mul_i(kx, h1), mul_i(kz, h3)), g.io32 & 2);
       const cplx dd = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);
-      gst(hg + idx, cadd(mul_i(-kx, h3), mul_i(kz, dd)));
+      st_io(hg, idx, cadd(mul_i(-kx, h3), mul_i(kz, dd)), g.io32 & 2);
       const double kk = kx2 + kz * kz;
-      gst(hv + idx, mkc(-h2.x * kk, -h2.y * kk));
+      st_io(hv, idx, mkc(-h2.x * kk, -h2.y * kk), g.io32 & 2);
       if (i == 0 && k == 0) {
         gstd(rf0u + j, h1.x);
         gstd(rf0w + j, h3.x);

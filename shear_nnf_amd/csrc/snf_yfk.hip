@@ -110,6 +110,64 @@ __device__ __forceinline__ void yf_commit(cplx* T, const cplx (&r)[Q], int N) {
   __syncthreads();
 }
 
+// The same transfers for the inter-stage fields (rhg, rhv, rdome in; v, dv,
+// dome out), stored FP32 when io32 (the mixed precision mode).  The y
+// kernels take the storage as a template argument IO (the bits of
+// Geom::io32), so that the FP64 kernels carry no FP32 path.
+// An FP32 element travels as its 8-byte pattern in the .x of the register
+// until yf_commit_io widens it, so that the load stays in flight like an
+// FP64 one (a conversion right after the load would wait for it).
+__device__ __forceinline__ double yf_raw64(const cplx* F, long idx) {
+  return *(const SNF_AS1 double*)(reinterpret_cast<const cplxf*>(F) + idx);
+}
+__device__ __forceinline__ cplx yf_widen(double raw) {
+  const long long b = __double_as_longlong(raw);
+  return mkc((double)__int_as_float((int)(b & 0xffffffffLL)), (double)__int_as_float((int)(b >> 32)));
+}
+template <int Q>
+__device__ __forceinline__ void yf_issue_io(cplx (&r)[Q], const cplx* F, long P, long L0, int nv,
+                                            int N, int io32) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = threadIdx.x + q * 64 * YF_LINES;
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    const long idx = (long)j * P + L0 + c;
+    if (e < N * YF_LINES && c < nv)
+      r[q] = io32 ? mkc(yf_raw64(F, idx), 0.0) : gld(F + idx);
+    else
+      r[q] = io32 ? mkc(__longlong_as_double(0LL), 0.0) : mkc(0.0, 0.0);
+  }
+}
+template <int Q>
+__device__ __forceinline__ void yf_commit_io(cplx* T, const cplx (&r)[Q], int N, int io32) {
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = threadIdx.x + q * 64 * YF_LINES;
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    if (e < N * YF_LINES) T[TileCol{c}(j)] = io32 ? yf_widen(r[q].x) : r[q];
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void yf_fill_io(cplx* T, const cplx* F, long P, long L0, int nv, int N,
+                                           int io32) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    T[TileCol{c}(j)] = (c < nv) ? ld_io(F, (long)j * P + L0 + c, io32) : mkc(0.0, 0.0);
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void yf_flush_io(const cplx* T, cplx* F, long P, long L0, int nv, int N,
+                                            int io32) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    if (c < nv) st_io(F, (long)j * P + L0 + c, T[TileCol{c}(j)], io32);
+  }
+  __syncthreads();
+}
+
 // The RK accumulators vorwk, phiwk are private to this path and kept
 // block-major: block lb owns the contiguous chunk W + lb N YF_LINES, in the
 // element order of the tile transfers (e = j YF_LINES + c).  The chunk moves
@@ -159,7 +217,7 @@ k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt
   k_ysym_body(N, c, shp, rt, out, sc);
 }
 
-template <class PL>
+template <class PL, int IO>
 __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   constexpr int Q = PL::Q;
   const int N = PL::direct ? a.g.my : PL::NC;
@@ -282,7 +340,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   // each is issued about two stages before the tile needs it
   cplx pfa[Q], pfb[Q];
   yf_commit(T, pv, N);                     // vor
-  yf_issue(pfa, a.rhg, P, L0, nv, N);
+  yf_issue_io(pfa, a.rhg, P, L0, nv, N, IO & 2);
   if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
   {
     YPh<true> ph(pho, lane);
@@ -307,8 +365,8 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
     }
   }
   PL::template fft<1>(T, ac, rt, N);
-  yf_commit(T, pfa, N);                    // rhg
-  yf_issue(pfa, a.rdome, P, L0, nv, N);
+  yf_commit_io(T, pfa, N, IO & 2);  // rhg
+  yf_issue_io(pfa, a.rdome, P, L0, nv, N, IO & 2);
   {
     YPh<false> ph(pho, lane);
 #pragma unroll
@@ -370,11 +428,11 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
       if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
     }
   }
-  yf_flush(T, a.dome, P, L0, nv, N);
+  yf_flush_io(T, a.dome, P, L0, nv, N, IO & 1);
 
   // ===== phi chain =====
-  yf_commit(T, pfa, N);                    // rdome
-  yf_issue(pfa, a.rhv, P, L0, nv, N);
+  yf_commit_io(T, pfa, N, IO & 2);  // rdome
+  yf_issue_io(pfa, a.rhv, P, L0, nv, N, IO & 2);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -418,7 +476,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
     }
   }
   PL::template fft<1>(T, ac, rt, N);
-  yf_commit(T, pfa, N);                    // rhv
+  yf_commit_io(T, pfa, N, IO & 2);  // rhv
   {
     YPh<false> ph(pho, lane);
 #pragma unroll
@@ -455,7 +513,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   yf_flush(T, a.phi, P, L0, nv, N);
   // v = lapcdy(phi) and dv = D1(v) with the new twist; the (0,0) mode keeps
   // its v (lapcdy is not called for it) and only takes dv = D1(v)
-  if (L0 == 0) yf_fill(T, a.v, P, L0, nv, N);   // block of the (0,0) mode
+  if (L0 == 0) yf_fill_io(T, a.v, P, L0, nv, N, IO & 1);   // block of the (0,0) mode
   // every wave of the block reaches the same transforms in the same order
   // (the matrix-core plan transforms the block's columns together); the
   // (0,0) mode takes no Helmholtz solve and leaves out the last transform
@@ -505,18 +563,18 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.dv, P, L0, nv, N);
+  yf_flush_io(T, a.dv, P, L0, nv, N, IO & 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
     if (j < N) T[tc(j)] = x[q];
   }
-  yf_flush(T, a.v, P, L0, nv, N);
+  yf_flush_io(T, a.v, P, L0, nv, N, IO & 1);
 }
-template <class PL>
+template <class PL, int IO>
 __global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
 k_yf_explicit(YArgs a, int rk) {
-  k_yf_explicit_body<PL>(a, rk);
+  k_yf_explicit_body<PL, IO>(a, rk);
 }
 
 // Implicit RK3 substep of the y stage (cross.f90:797-990) in the same block
@@ -532,7 +590,7 @@ k_yf_explicit(YArgs a, int rk) {
 // The (0,0) line keeps vor, phi and v (cross.f90:866, 952) and takes dome,
 // dv = D1 of them.  U, W (m=1 calls, last block) follow cross.f90:829-843,
 // 858-863, 900-918.  13 line transforms; every field read and written once.
-template <class PL>
+template <class PL, int IO>
 __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   constexpr int Q = PL::Q;
   const int N = PL::direct ? a.g.my : PL::NC;
@@ -660,7 +718,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   // ===== omega_y chain =====
   cplx pfa[Q], pfb[Q];
   yf_commit(T, pv, N);                     // vor
-  yf_issue(pfa, a.rhg, P, L0, nv, N);
+  yf_issue_io(pfa, a.rhg, P, L0, nv, N, IO & 2);
   if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
   {
     YPh<true> ph(pho, lane);
@@ -700,8 +758,8 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
       }
     }
   }
-  yf_commit(T, pfa, N);                    // rhg
-  yf_issue(pfa, a.rdome, P, L0, nv, N);
+  yf_commit_io(T, pfa, N, IO & 2);  // rhg
+  yf_issue_io(pfa, a.rdome, P, L0, nv, N, IO & 2);
   yf_issue(pfb, a.phi, P, L0, nv, N);
   {
     YPh<false> m1(pm1, lane);
@@ -744,7 +802,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.dome, P, L0, nv, N);
+  yf_flush_io(T, a.dome, P, L0, nv, N, IO & 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
@@ -754,7 +812,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
 
   // ===== phi chain =====
   yf_commit(T, pfb, N);                    // phi
-  yf_issue(pfb, a.rhv, P, L0, nv, N);
+  yf_issue_io(pfb, a.rhv, P, L0, nv, N, IO & 2);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -788,7 +846,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
     }
   }
   // the (0,0) line keeps phi: x holds it until the flush
-  yf_commit(T, pfa, N);                    // rdome
+  yf_commit_io(T, pfa, N, IO & 2);  // rdome
   if (rk > 1) yf_issue_blk(pfa, pwk_blk, N);
   {
     YPh<true> ph(pho, lane);
@@ -825,7 +883,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], T[ac(j)]);
     }
   }
-  yf_commit(T, pfb, N);                    // rhv
+  yf_commit_io(T, pfb, N, IO & 2);  // rhv
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -870,7 +928,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
     if (j < N) T[tc(j)] = x[q];
   }
   yf_flush(T, a.phi, P, L0, nv, N);
-  if (L0 == 0) yf_fill(T, a.v, P, L0, nv, N);   // block of the (0,0) mode
+  if (L0 == 0) yf_fill_io(T, a.v, P, L0, nv, N, IO & 1);   // block of the (0,0) mode
   if (!is00) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -923,25 +981,25 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
       }
     }
   }
-  yf_flush(T, a.dv, P, L0, nv, N);
+  yf_flush_io(T, a.dv, P, L0, nv, N, IO & 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
     if (j < N) T[tc(j)] = x[q];
   }
-  yf_flush(T, a.v, P, L0, nv, N);
+  yf_flush_io(T, a.v, P, L0, nv, N, IO & 1);
 }
-template <class PL>
+template <class PL, int IO>
 __global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
 k_yf_implicit(YArgs a, int rk) {
-  k_yf_implicit_body<PL>(a, rk);
+  k_yf_implicit_body<PL, IO>(a, rk);
 }
 
 // Start of a cross call on the spectral y stage (cross.f90:408-445), the
 // role of k_y_first: at the current twist, v = lapcdy(phi) (the (0,0) line
 // keeps its v), dv = D1(v), dome = D1(vor); dU/dy, dW/dy (cross.f90:1198-1205)
 // in the last block.  Same block layout as k_yf_explicit; 6 line transforms.
-template <class PL>
+template <class PL, int IO>
 __global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
 k_yf_first(YArgs a) {
   constexpr int Q = PL::Q;
@@ -1035,11 +1093,11 @@ k_yf_first(YArgs a) {
       if (j < N) T[tc(j)] = cmul(T[ac(j)], e);
     }
   }
-  yf_flush(T, a.dome, P, L0, nv, N);
+  yf_flush_io(T, a.dome, P, L0, nv, N, IO & 1);
 
   // v = lapcdy(phi) (the (0,0) line keeps v), dv = D1(v)
   if (L0 == 0) {                 // block of the (0,0) mode
-    yf_fill(T, a.v, P, L0, nv, N);
+    yf_fill_io(T, a.v, P, L0, nv, N, IO & 1);
     if (is00) {
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
@@ -1084,13 +1142,13 @@ k_yf_first(YArgs a) {
       }
     }
   }
-  yf_flush(T, a.dv, P, L0, nv, N);
+  yf_flush_io(T, a.dv, P, L0, nv, N, IO & 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
     if (j < N) T[tc(j)] = x[q];
   }
-  yf_flush(T, a.v, P, L0, nv, N);
+  yf_flush_io(T, a.v, P, L0, nv, N, IO & 1);
 }
 
 // iydealiasing=3: the compact filter (filterc, filter_shear_hepta.f90:128-165)
