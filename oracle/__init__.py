@@ -30,8 +30,13 @@ the tests:
   (``tests/test_oracle_getv.py``);
 * ``rdt``      -- a high-accuracy integration of the Kelvin-mode equations.
 
-Parity unpinned (no check short of the compiled cross.f90, which does not
-build here): the values of the 00-mode forcing rf0u/rf0w (every closed-form
-state leaves them at rounding level), ``_new_dt`` beyond the etime and
-dump clippings.  They follow cross.f90 line by line, with citations.
+Also pinned, by ``tests/test_oracle_pins.py`` and ``tests/test_ref_compact.py``:
+the 00-mode forcing rf0u/rf0w (plane means of u x omega formed by direct
+double DFT sums), every branch of ``_new_dt`` (states with exact physical
+maxima: the umax(0) x term, the y and z terms, the explicit viscous cap, the
+dt < 1e-10 stop, the dump and etime clips, the umax >= 1 first step), the
+``ias=0`` factor reuse (through the compiled derivyc) and the variant-2
+dealiasing windows (mode by mode against hand-worked cut sets).  The
+192x193x192 fixture ``tests/golden/config2_192x193x192.npz`` is this oracle's
+10-step run, made by ``tests/golden/make_config2.py``.
 """

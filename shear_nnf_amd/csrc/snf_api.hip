@@ -1283,13 +1283,27 @@ int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc) {
   return SNF_OK;
 }
 
-static int upload_common(snf_handle* h, const std::vector<cplx>& hv, const std::vector<cplx>& hp,
+// jik = true: vor, phi arrive in the reference's jik order and are
+// transposed on the device (staged in the banded-sweep scratch s1, s2, which
+// no advance holds between calls)
+static int upload_common(snf_handle* h, const double* vor, const double* phi, bool jik,
                          const double* u00, const double* w00, double time, double xoffb,
                          double xofft) {
   HIPCHK(hipSetDevice(h->dev));
+  HIPCHK(hipStreamSynchronize(h->stream));
   const int my = h->g.my;
-  HIPCHK(hipMemcpy(h->vor, hv.data(), h->NC * sizeof(cplx), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->phi, hp.data(), h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+  if (jik) {
+    HIPCHK(hipMemcpy(h->s1, vor, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->s2, phi, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+    const dim3 gt((my + JIK_T - 1) / JIK_T, (h->g.mz + JIK_T - 1) / JIK_T, h->g.kxn);
+    k_jik_transpose<<<gt, 256, 0, h->stream>>>(h->g, h->s1, h->vor, 1);
+    k_jik_transpose<<<gt, 256, 0, h->stream>>>(h->g, h->s2, h->phi, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+  } else {
+    HIPCHK(hipMemcpy(h->vor, vor, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->phi, phi, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+  }
   std::vector<cplx> uc(my), wc(my);
   for (int j = 0; j < my; ++j) {
     uc[j] = mkc(u00[j], 0.0);
@@ -1313,38 +1327,35 @@ static int upload_common(snf_handle* h, const std::vector<cplx>& hv, const std::
 int snf_state_upload_ykxkz(snf_handle* h, const double* vor, const double* phi, const double* u00,
                            const double* w00, double time, double xoffb, double xofft) {
   if (!h || !vor || !phi || !u00 || !w00) return fail(SNF_BAD_PARAM, "null argument");
-  std::vector<cplx> a(h->NC), b(h->NC);
-  std::memcpy(a.data(), vor, h->NC * sizeof(cplx));
-  std::memcpy(b.data(), phi, h->NC * sizeof(cplx));
-  return upload_common(h, a, b, u00, w00, time, xoffb, xofft);
+  return upload_common(h, vor, phi, false, u00, w00, time, xoffb, xofft);
 }
 
 int snf_state_upload(snf_handle* h, const double* vor, const double* phi, const double* u00,
                      const double* w00, double time, double xoffb, double xofft) {
   if (!h || !vor || !phi || !u00 || !w00) return fail(SNF_BAD_PARAM, "null argument");
-  const int my = h->g.my, kxn = h->g.kxn, mz = h->g.mz;
-  std::vector<cplx> a(h->NC), b(h->NC);
-  // reference jik: ((k*kxn + i)*my + j) complex  ->  device (j*kxn + i)*mz + k
-  for (int k = 0; k < mz; ++k)
-    for (int i = 0; i < kxn; ++i)
-      for (int j = 0; j < my; ++j) {
-        const size_t src = ((size_t)k * kxn + i) * my + j;
-        const size_t dst = ((size_t)j * kxn + i) * mz + k;
-        a[dst] = mkc(vor[2 * src], vor[2 * src + 1]);
-        b[dst] = mkc(phi[2 * src], phi[2 * src + 1]);
-      }
-  return upload_common(h, a, b, u00, w00, time, xoffb, xofft);
+  // reference jik: ((k*kxn + i)*my + j) complex -> device (j*kxn + i)*mz + k
+  return upload_common(h, vor, phi, true, u00, w00, time, xoffb, xofft);
 }
 
-static int download_common(snf_handle* h, std::vector<cplx>& a, std::vector<cplx>& b, double* u00,
+// jik = true: vor, phi leave in the reference's jik order, transposed on the
+// device (through the scratch s1, s2)
+static int download_common(snf_handle* h, double* vor, double* phi, bool jik, double* u00,
                            double* w00) {
   HIPCHK(hipSetDevice(h->dev));
   HIPCHK(hipStreamSynchronize(h->stream));
   const int my = h->g.my;
-  a.resize(h->NC);
-  b.resize(h->NC);
-  HIPCHK(hipMemcpy(a.data(), h->vor, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(b.data(), h->phi, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  if (jik) {
+    const dim3 gt((my + JIK_T - 1) / JIK_T, (h->g.mz + JIK_T - 1) / JIK_T, h->g.kxn);
+    if (vor) k_jik_transpose<<<gt, 256, 0, h->stream>>>(h->g, h->vor, h->s1, -1);
+    if (phi) k_jik_transpose<<<gt, 256, 0, h->stream>>>(h->g, h->phi, h->s2, -1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (vor) HIPCHK(hipMemcpy(vor, h->s1, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+    if (phi) HIPCHK(hipMemcpy(phi, h->s2, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  } else {
+    if (vor) HIPCHK(hipMemcpy(vor, h->vor, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+    if (phi) HIPCHK(hipMemcpy(phi, h->phi, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+  }
   std::vector<cplx> uc(my), wc(my);
   HIPCHK(hipMemcpy(uc.data(), h->u00c, my * sizeof(cplx), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(wc.data(), h->w00c, my * sizeof(cplx), hipMemcpyDeviceToHost));
@@ -1357,29 +1368,14 @@ static int download_common(snf_handle* h, std::vector<cplx>& a, std::vector<cplx
 
 int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi, double* u00, double* w00) {
   if (!h) return fail(SNF_BAD_PARAM, "null handle");
-  std::vector<cplx> a, b;
-  int rc = download_common(h, a, b, u00, w00);
-  if (rc) return rc;
-  if (vor) std::memcpy(vor, a.data(), h->NC * sizeof(cplx));
-  if (phi) std::memcpy(phi, b.data(), h->NC * sizeof(cplx));
-  return SNF_OK;
+  return download_common(h, vor, phi, false, u00, w00);
 }
 
 int snf_state_download(snf_handle* h, double* vor, double* phi, double* u00, double* w00,
                        double* time, double* xoffb, double* xofft) {
   if (!h) return fail(SNF_BAD_PARAM, "null handle");
-  std::vector<cplx> a, b;
-  int rc = download_common(h, a, b, u00, w00);
+  int rc = download_common(h, vor, phi, true, u00, w00);
   if (rc) return rc;
-  const int my = h->g.my, kxn = h->g.kxn, mz = h->g.mz;
-  for (int k = 0; k < mz; ++k)
-    for (int i = 0; i < kxn; ++i)
-      for (int j = 0; j < my; ++j) {
-        const size_t dst = ((size_t)k * kxn + i) * my + j;
-        const size_t src = ((size_t)j * kxn + i) * mz + k;
-        if (vor) { vor[2 * dst] = a[src].x; vor[2 * dst + 1] = a[src].y; }
-        if (phi) { phi[2 * dst] = b[src].x; phi[2 * dst + 1] = b[src].y; }
-      }
   DevScalars s{};
   HIPCHK(hipMemcpy(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost));
   if (time) *time = s.time;

@@ -371,6 +371,46 @@ __global__ void k_step_begin(DevScalars* sc, double wrap_lx) {
   k_step_begin_body(sc, wrap_lx);
 }
 
+// Layout change between the reference's jik order (y fastest, then kx, then
+// kz: element ((k kxn + i) my + j)) and the device order [y][kx][kz]
+// (((j kxn + i) mz + k)), the transpose of chjik2ikj / chikj2jik
+// (change.allfac_ieor.f:80-87, 117-124) on one GPU.  For each kx the two
+// orders are transposes of a my x mz matrix; a block moves a 32 x 32 tile
+// through LDS so that both its reads and its writes are row runs.
+// dir = +1: jik -> [y][kx][kz]; dir = -1: the reverse.
+#define JIK_T 32
+__global__ void __launch_bounds__(256)
+k_jik_transpose(Geom g, const cplx* __restrict__ src, cplx* __restrict__ dst, int dir) {
+  __shared__ cplx t[JIK_T][JIK_T + 1];
+  const int i = blockIdx.z;                       // kx
+  const int tj = blockIdx.x * JIK_T, tk = blockIdx.y * JIK_T;
+  const int tx = threadIdx.x & (JIK_T - 1), ty = threadIdx.x / JIK_T;   // 32 x 8
+  const long kxn = g.kxn, my = g.my, mz = g.mz;
+  if (dir > 0) {
+    // read rows of j (contiguous in jik) for each k of the tile
+    for (int r = ty; r < JIK_T; r += 8) {
+      const int k = tk + r, j = tj + tx;
+      if (k < mz && j < my) t[r][tx] = gld(src + ((long)k * kxn + i) * my + j);
+    }
+    __syncthreads();
+    // write rows of k (contiguous in [y][kx][kz]) for each j
+    for (int r = ty; r < JIK_T; r += 8) {
+      const int j = tj + r, k = tk + tx;
+      if (k < mz && j < my) gst(dst + ((long)j * kxn + i) * mz + k, t[tx][r]);
+    }
+  } else {
+    for (int r = ty; r < JIK_T; r += 8) {
+      const int j = tj + r, k = tk + tx;
+      if (k < mz && j < my) t[tx][r] = gld(src + ((long)j * kxn + i) * mz + k);
+    }
+    __syncthreads();
+    for (int r = ty; r < JIK_T; r += 8) {
+      const int k = tk + r, j = tj + tx;
+      if (k < mz && j < my) gst(dst + ((long)k * kxn + i) * my + j, t[r][tx]);
+    }
+  }
+}
+
 // kx = 0 Hermitian fix on the state (cross.f90:287-291, set_conj)
 __global__ void k_set_conj(Geom g, cplx* vor, cplx* phi) {
   const int j = blockIdx.x;

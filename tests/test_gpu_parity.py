@@ -240,9 +240,12 @@ def test_cross_generated_ic_first_dt(lib):
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
 
-def test_jik_roundtrip(lib):
-    vor, phi, u00, w00 = box_ic(48, 49, 48)
-    box = _box()
+@pytest.mark.parametrize("grid", [(48, 49, 48), (96, 97, 96), (128, 33, 64)])
+def test_jik_roundtrip(lib, grid):
+    """the reference's jik order through the device transposes (tiles of 32
+    that do not divide my, mz) and back: bit-exact"""
+    vor, phi, u00, w00 = box_ic(*grid)
+    box = _box(*grid)
     jv = np.ascontiguousarray(vor.transpose(2, 1, 0))
     jp = np.ascontiguousarray(phi.transpose(2, 1, 0))
     box.set_state_jik(jv, jp, u00, w00, time=1.5, xoffb=-0.25, xofft=0.25)

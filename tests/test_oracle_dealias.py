@@ -10,9 +10,8 @@ cross.f90:459-475, dealiasing_y operators.f90:352-486).
 * Runs: a Kelvin mode (kx != 0) is untouched by variant 1 and converges to
   its closed form; the kx = 0 lift-up mode and U, W profiles below the cut
   pass both variants; variant 2 keeps the offsets wrapped to one period.
-Parity unpinned: the values of the shifted windows djda/djdb of variant 2
-for kx != 0 (a transcription of operators.f90:433-448; only the compiled
-reference could pin them).
+The values of the shifted windows djda/djdb of variant 2 for kx != 0
+(operators.f90:433-460) are pinned mode by mode in tests/test_oracle_pins.py.
 """
 import math
 
