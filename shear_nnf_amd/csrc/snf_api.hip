@@ -134,6 +134,7 @@ struct snf_handle {
   YCoef c{};
   int dev = 0;
   long P = 0, PS = 0, NC = 0;
+  long NF = 0;   // elements of a spectral field in the device's kz-chunk layout
   hipStream_t stream = nullptr;
   FFTPlan px{}, pz{};
   // device memory
@@ -309,7 +310,7 @@ void collect_timing(snf_handle* h) {
 template <int R, class H>
 void launch_xzw_A(snf_handle* h) {
   const Geom& g = h->g;
-  const unsigned nb = (unsigned)(((long)g.my * g.kxn + WA_WAVES - 1) / WA_WAVES);
+  const unsigned nb = wrow_blocks(g, WA_WAVES);
   k_xzw_A<R, H><<<nb, 64 * WA_WAVES, h->ldsWA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv,
       h->dome, h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk, reinterpret_cast<H*>(h->hyb_in),
       h->twz, h->sc);
@@ -324,7 +325,7 @@ void launch_xzw_B(snf_handle* h, int rk) {
 template <int R, class H>
 void launch_xzw_C(snf_handle* h) {
   const Geom& g = h->g;
-  const unsigned nb = (unsigned)(((long)g.my * g.kxn + WC_WAVES - 1) / WC_WAVES);
+  const unsigned nb = wrow_blocks(g, WC_WAVES);
   k_xzw_C<R, H><<<nb, 64 * WC_WAVES, h->ldsWC, h->stream>>>(g, reinterpret_cast<const H*>(h->hyb_out),
       h->v, h->rdome, h->rhg, h->rhv, h->rf0u, h->rf0w, h->twz, h->sc);
 }
@@ -361,7 +362,7 @@ void launch_xzw_any(snf_handle* h, int which, int rk) {
 template <class H>
 void launch_xz_generic(snf_handle* h, int which, int rk) {
   const Geom& g = h->g;
-  dim3 gA(g.kxn, g.my);
+  dim3 gA(g.my, g.kxn);
   H* hin = reinterpret_cast<H*>(h->hyb_in);
   H* hout = reinterpret_cast<H*>(h->hyb_out);
   if (which == 0) {
@@ -647,13 +648,13 @@ template <class H>
 void tab_A(const snf_handle* h, const BxXzA* t, int n, hipStream_t s) {
   lds_attr((const void*)k_xz_A_tab<H>, h->ldsA, h->dev);
   if (n == 0) return;   // the LDS limit only (set before a graph capture)
-  k_xz_A_tab<H><<<dim3(h->g.kxn, h->g.my, n), 256, h->ldsA, s>>>(t);
+  k_xz_A_tab<H><<<dim3(h->g.my, h->g.kxn, n), 256, h->ldsA, s>>>(t);
 }
 template <class H>
 void tab_C(const snf_handle* h, const BxXzC* t, int n, hipStream_t s) {
   lds_attr((const void*)k_xz_C_tab<H>, h->ldsC, h->dev);
   if (n == 0) return;
-  k_xz_C_tab<H><<<dim3(h->g.kxn, h->g.my, n), 256, h->ldsC, s>>>(t);
+  k_xz_C_tab<H><<<dim3(h->g.my, h->g.kxn, n), 256, h->ldsC, s>>>(t);
 }
 template <int NT, class H>
 void tab_B_nt(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
@@ -983,6 +984,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   g.mz = 2 * (mgalz / 3) - 1;
   g.kxn = mx / 2;
   g.nz = (g.mz - 1) / 2;
+  g.nkzb = (g.mz + SNF_CW - 1) / SNF_CW;
   g.P = (long)g.kxn * g.mz;
   g.alp = p->alp;
   g.gam = p->gam;
@@ -994,6 +996,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   h->P = g.P;
   h->PS = g.P + 2;
   h->NC = g.P * my;
+  h->NF = sfield_len(g);
   if (!plan_of(mgalx, h->px) || !plan_of(mgalz, h->pz)) {
     delete h;
     return fail(SNF_BAD_PARAM, "mgalx and mgalz must be of the form 2^a 3^b");
@@ -1033,22 +1036,21 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }                            \
   } while (0)
   TRY(hipStreamCreate(&h->stream) == hipSuccess ? SNF_OK : fail(SNF_DEVICE_ERROR, "stream"));
-  const size_t NC = h->NC, NS = (size_t)my * h->PS;
-  TRY(A(&h->vor, NC));
-  TRY(A(&h->phi, NC));
-  TRY(A(&h->v, NC));
-  TRY(A(&h->dome, NC));
-  TRY(A(&h->dv, NC));
-  TRY(A(&h->rdome, NC));
-  TRY(A(&h->rhg, NC));
-  TRY(A(&h->rhv, NC));
+  // spectral fields in the kz-chunk layout (NF >= NC elements, the padding
+  // zero); the banded sweeps' scratch s1, s2, s3 in [y][line] order (NS >=
+  // NC, also the staging of host arrays in [y][kx][kz] or jik order)
+  const size_t NF = h->NF, NS = (size_t)my * h->PS;
+  TRY(A(&h->vor, NF));
+  TRY(A(&h->phi, NF));
+  TRY(A(&h->v, NF));
+  TRY(A(&h->dome, NF));
+  TRY(A(&h->dv, NF));
+  TRY(A(&h->rdome, NF));
+  TRY(A(&h->rhg, NF));
+  TRY(A(&h->rhv, NF));
   TRY(A(&h->s3, NS));
-  // the spectral y path keeps the accumulators block-major: kxn * nch
-  // blocks of YF_LINES lines (nch = ceil(mz / YF_LINES)), at least NC
-  const size_t NWK = std::max((size_t)NC, (size_t)g.kxn * ((g.mz + YF_LINES - 1) / YF_LINES) *
-                                              YF_LINES * (size_t)my);
-  TRY(A(&h->vorwk, NWK));
-  TRY(A(&h->phiwk, NWK));
+  TRY(A(&h->vorwk, NF));
+  TRY(A(&h->phiwk, NF));
   TRY(A(&h->s1, NS));
   TRY(A(&h->s2, NS));
   const size_t NH = (size_t)my * g.kxn * mgalz;
@@ -1283,9 +1285,13 @@ int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc) {
   return SNF_OK;
 }
 
-// jik = true: vor, phi arrive in the reference's jik order and are
-// transposed on the device (staged in the banded-sweep scratch s1, s2, which
-// no advance holds between calls)
+// vor, phi arrive in the reference's jik order (jik = true) or in [y][kx][kz]
+// order, staged in the banded-sweep scratch s1, s2 (which no advance holds
+// between calls) and rearranged on the device into its kz-chunk layout
+static unsigned relayout_blocks(const snf_handle* h) {
+  return (unsigned)std::min<long>((h->NF + 255) / 256, 8192);
+}
+
 static int upload_common(snf_handle* h, const double* vor, const double* phi, bool jik,
                          const double* u00, const double* w00, double time, double xoffb,
                          double xofft) {
@@ -1301,8 +1307,12 @@ static int upload_common(snf_handle* h, const double* vor, const double* phi, bo
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
   } else {
-    HIPCHK(hipMemcpy(h->vor, vor, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->phi, phi, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->s1, vor, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->s2, phi, h->NC * sizeof(cplx), hipMemcpyHostToDevice));
+    k_relayout<cplx><<<relayout_blocks(h), 256, 0, h->stream>>>(h->g, h->s1, h->vor, 1);
+    k_relayout<cplx><<<relayout_blocks(h), 256, 0, h->stream>>>(h->g, h->s2, h->phi, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
   }
   std::vector<cplx> uc(my), wc(my);
   for (int j = 0; j < my; ++j) {
@@ -1333,12 +1343,12 @@ int snf_state_upload_ykxkz(snf_handle* h, const double* vor, const double* phi, 
 int snf_state_upload(snf_handle* h, const double* vor, const double* phi, const double* u00,
                      const double* w00, double time, double xoffb, double xofft) {
   if (!h || !vor || !phi || !u00 || !w00) return fail(SNF_BAD_PARAM, "null argument");
-  // reference jik: ((k*kxn + i)*my + j) complex -> device (j*kxn + i)*mz + k
+  // reference jik: ((k*kxn + i)*my + j) complex -> the device layout
   return upload_common(h, vor, phi, true, u00, w00, time, xoffb, xofft);
 }
 
-// jik = true: vor, phi leave in the reference's jik order, transposed on the
-// device (through the scratch s1, s2)
+// vor, phi leave in the reference's jik order (jik = true) or in [y][kx][kz]
+// order, rearranged on the device into the scratch s1, s2
 static int download_common(snf_handle* h, double* vor, double* phi, bool jik, double* u00,
                            double* w00) {
   HIPCHK(hipSetDevice(h->dev));
@@ -1353,8 +1363,12 @@ static int download_common(snf_handle* h, double* vor, double* phi, bool jik, do
     if (vor) HIPCHK(hipMemcpy(vor, h->s1, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
     if (phi) HIPCHK(hipMemcpy(phi, h->s2, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
   } else {
-    if (vor) HIPCHK(hipMemcpy(vor, h->vor, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
-    if (phi) HIPCHK(hipMemcpy(phi, h->phi, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+    if (vor) k_relayout<cplx><<<relayout_blocks(h), 256, 0, h->stream>>>(h->g, h->vor, h->s1, -1);
+    if (phi) k_relayout<cplx><<<relayout_blocks(h), 256, 0, h->stream>>>(h->g, h->phi, h->s2, -1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (vor) HIPCHK(hipMemcpy(vor, h->s1, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
+    if (phi) HIPCHK(hipMemcpy(phi, h->s2, h->NC * sizeof(cplx), hipMemcpyDeviceToHost));
   }
   std::vector<cplx> uc(my), wc(my);
   HIPCHK(hipMemcpy(uc.data(), h->u00c, my * sizeof(cplx), hipMemcpyDeviceToHost));
@@ -2109,35 +2123,37 @@ int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const do
   // bit 0 for the three inputs, bit 1 for the three outputs
   const int in32 = g.io32 & 1, out32 = g.io32 & 2;
   const size_t es = in32 ? sizeof(cplxf) : sizeof(cplx);
+  // host arrays in [y][kx][kz] order (FP64) pass through the scratch s1 and
+  // are rearranged on the device, with the FP32 rounding of a mixed handle
   auto put = [&](cplx* dst, const double* src) -> int {
-    if (!in32) {
-      HIPCHK(hipMemcpy(dst, src, NC * sizeof(cplx), hipMemcpyHostToDevice));
-      return SNF_OK;
-    }
-    std::vector<float> f(2 * NC);
-    for (size_t e = 0; e < 2 * NC; ++e) f[e] = (float)src[e];
-    HIPCHK(hipMemcpy(dst, f.data(), NC * sizeof(cplxf), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->s1, src, NC * sizeof(cplx), hipMemcpyHostToDevice));
+    if (in32)
+      k_relayout<cplxf><<<relayout_blocks(h), 256, 0, h->stream>>>(g, h->s1, dst, 1);
+    else
+      k_relayout<cplx><<<relayout_blocks(h), 256, 0, h->stream>>>(g, h->s1, dst, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
     return SNF_OK;
   };
   auto get = [&](double* dst, const cplx* src) -> int {
-    if (!out32) {
-      HIPCHK(hipMemcpy(dst, src, NC * sizeof(cplx), hipMemcpyDeviceToHost));
-      return SNF_OK;
-    }
-    std::vector<float> f(2 * NC);
-    HIPCHK(hipMemcpy(f.data(), src, NC * sizeof(cplxf), hipMemcpyDeviceToHost));
-    for (size_t e = 0; e < 2 * NC; ++e) dst[e] = (double)f[e];
+    if (out32)
+      k_relayout<cplxf><<<relayout_blocks(h), 256, 0, h->stream>>>(g, src, h->s1, -1);
+    else
+      k_relayout<cplx><<<relayout_blocks(h), 256, 0, h->stream>>>(g, src, h->s1, -1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(dst, h->s1, NC * sizeof(cplx), hipMemcpyDeviceToHost));
     return SNF_OK;
   };
   // The evaluation takes v, dv/dy, d(omega_y)/dy in the handle's own
   // buffers.  The next advance re-forms them from the state except on the
   // (0,0) line, which no y stage writes and the dissipation sum reads: keep
-  // those three lines.
+  // those three lines (line (0,0) starts at element 0, stride SNF_CW).
   HIPCHK(hipStreamSynchronize(h->stream));
   std::vector<char> keep00(3 * (size_t)g.my * es);
   cplx* bufs[3] = {h->v, h->dv, h->dome};
   for (int f = 0; f < 3; ++f)
-    HIPCHK(hipMemcpy2D(keep00.data() + (size_t)f * g.my * es, es, bufs[f], g.P * es, es, g.my,
+    HIPCHK(hipMemcpy2D(keep00.data() + (size_t)f * g.my * es, es, bufs[f], SNF_CW * es, es, g.my,
                        hipMemcpyDeviceToHost));
   int rc;
   if ((rc = put(h->v, hg_v)) || (rc = put(h->dv, hv_dv)) || (rc = put(h->dome, dvordy))) return rc;
@@ -2152,7 +2168,7 @@ int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv, const do
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(h->sc, &keep, sizeof(keep), hipMemcpyHostToDevice));
   for (int f = 0; f < 3; ++f)
-    HIPCHK(hipMemcpy2D(bufs[f], g.P * es, keep00.data() + (size_t)f * g.my * es, es, es, g.my,
+    HIPCHK(hipMemcpy2D(bufs[f], SNF_CW * es, keep00.data() + (size_t)f * g.my * es, es, es, g.my,
                        hipMemcpyHostToDevice));
   if (out_dvordy && (rc = get(out_dvordy, h->rdome))) return rc;
   if (out_hg && (rc = get(out_hg, h->rhg))) return rc;

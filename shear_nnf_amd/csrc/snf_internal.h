@@ -77,6 +77,7 @@ __host__ __device__ __forceinline__ double kz_of(int k, int nz, int mz, double g
 // Grid description passed by value to every kernel.
 struct Geom {
   int my, kxn, mz, nz, mgalx, mgalz;
+  int nkzb;          // kz chunks of SNF_CW lines: ceil(mz / SNF_CW)
   long P;            // kxn*mz: complex coefficients per y-plane (= number of y-lines)
   double alp, gam, Ly, dy, s, chi, Re;
   // Storage of the fields between the two stages of a substep, bit 0: v,
@@ -86,6 +87,24 @@ struct Geom {
   // stays FP64.
   int io32;
 };
+
+// Device layout of every spectral field (vor, phi, the RK accumulators and
+// the inter-stage fields): kz-chunk-major, [kx][kz / SNF_CW][y][kz % SNF_CW].
+// The SNF_CW lines of a chunk are one contiguous run of my * SNF_CW elements,
+// which the y stage moves whole; a row (kx, y) of the x-z stage is nkzb
+// pieces of SNF_CW elements, which blocks of consecutive y read together.
+// Line (i, k) starts at sline(g, i, k) and has stride SNF_CW in y.  A field
+// holds kxn * nkzb * my * SNF_CW elements; those with k >= mz (the last chunk
+// when SNF_CW does not divide mz) stay zero and are never read.
+#define SNF_CW 4
+__host__ __device__ __forceinline__ long sidx(const Geom& g, int j, int i, int k) {
+  return (((long)i * g.nkzb + (k >> 2)) * g.my + j) * SNF_CW + (k & (SNF_CW - 1));
+}
+__host__ __device__ __forceinline__ long sline(const Geom& g, int i, int k) { return sidx(g, 0, i, k); }
+__host__ __device__ __forceinline__ long sfield_len(const Geom& g) {
+  return (long)g.kxn * g.nkzb * g.my * SNF_CW;
+}
+static_assert(SNF_CW == 4, "sidx takes k >> 2 as the chunk of k");
 
 // loads and stores of the inter-stage fields in the storage the grid says
 __device__ __forceinline__ cplx ld_io(const cplx* f, long idx, int io32) {

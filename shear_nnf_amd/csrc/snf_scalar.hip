@@ -372,12 +372,12 @@ __global__ void k_step_begin(DevScalars* sc, double wrap_lx) {
 }
 
 // Layout change between the reference's jik order (y fastest, then kx, then
-// kz: element ((k kxn + i) my + j)) and the device order [y][kx][kz]
-// (((j kxn + i) mz + k)), the transpose of chjik2ikj / chikj2jik
-// (change.allfac_ieor.f:80-87, 117-124) on one GPU.  For each kx the two
-// orders are transposes of a my x mz matrix; a block moves a 32 x 32 tile
-// through LDS so that both its reads and its writes are row runs.
-// dir = +1: jik -> [y][kx][kz]; dir = -1: the reverse.
+// kz: element ((k kxn + i) my + j)) and the device's kz-chunk order
+// (sidx(g, j, i, k)), the transpose of chjik2ikj / chikj2jik
+// (change.allfac_ieor.f:80-87, 117-124) on one GPU.  For each kx a block
+// moves a 32 x 32 (y, kz) tile through LDS: the jik side is read or written
+// in runs of 32 y, the device side in runs of SNF_CW kz over 32 consecutive y.
+// dir = +1: jik -> device; dir = -1: the reverse.
 #define JIK_T 32
 __global__ void __launch_bounds__(256)
 k_jik_transpose(Geom g, const cplx* __restrict__ src, cplx* __restrict__ dst, int dir) {
@@ -393,15 +393,21 @@ k_jik_transpose(Geom g, const cplx* __restrict__ src, cplx* __restrict__ dst, in
       if (k < mz && j < my) t[r][tx] = gld(src + ((long)k * kxn + i) * my + j);
     }
     __syncthreads();
-    // write rows of k (contiguous in [y][kx][kz]) for each j
+    // write the device side: lanes in (kz % SNF_CW, y, kz / SNF_CW) order
     for (int r = ty; r < JIK_T; r += 8) {
-      const int j = tj + r, k = tk + tx;
-      if (k < mz && j < my) gst(dst + ((long)j * kxn + i) * mz + k, t[tx][r]);
+      const int u = r * JIK_T + tx;
+      const int kk = (u & (SNF_CW - 1)) + SNF_CW * (u / (SNF_CW * JIK_T));
+      const int jj = (u / SNF_CW) % JIK_T;
+      const int j = tj + jj, k = tk + kk;
+      if (k < mz && j < my) gst(dst + sidx(g, j, i, k), t[kk][jj]);
     }
   } else {
     for (int r = ty; r < JIK_T; r += 8) {
-      const int j = tj + r, k = tk + tx;
-      if (k < mz && j < my) t[tx][r] = gld(src + ((long)j * kxn + i) * mz + k);
+      const int u = r * JIK_T + tx;
+      const int kk = (u & (SNF_CW - 1)) + SNF_CW * (u / (SNF_CW * JIK_T));
+      const int jj = (u / SNF_CW) % JIK_T;
+      const int j = tj + jj, k = tk + kk;
+      if (k < mz && j < my) t[kk][jj] = gld(src + sidx(g, j, i, k));
     }
     __syncthreads();
     for (int r = ty; r < JIK_T; r += 8) {
@@ -415,9 +421,35 @@ k_jik_transpose(Geom g, const cplx* __restrict__ src, cplx* __restrict__ dst, in
 __global__ void k_set_conj(Geom g, cplx* vor, cplx* phi) {
   const int j = blockIdx.x;
   for (int k = g.nz + 1 + threadIdx.x; k < g.mz; k += blockDim.x) {
-    const long b = (long)j * g.P;
-    vor[b + k] = cconj(vor[b + g.mz - k]);
-    phi[b + k] = cconj(phi[b + g.mz - k]);
+    const long d = sidx(g, j, 0, k), s = sidx(g, j, 0, g.mz - k);
+    vor[d] = cconj(vor[s]);
+    phi[d] = cconj(phi[s]);
+  }
+}
+
+// Layout change between [y][kx][kz] order (((j kxn + i) mz + k), the order of
+// the host state vectors and operator arrays) and the device's kz-chunk
+// order, for fields of element type E on the device side (FP64, or FP32 for
+// the inter-stage fields of the mixed precision mode); the [y][kx][kz] side
+// is always FP64.  One thread per element, in device order (dir = +1:
+// [y][kx][kz] -> device; -1: the reverse).  The device side is read or
+// written whole, the other side in runs of SNF_CW.
+template <class E>
+__global__ void __launch_bounds__(256) k_relayout(Geom g, const void* __restrict__ src,
+                                                  void* __restrict__ dst, int dir) {
+  const long n = sfield_len(g), chunk = (long)g.nkzb * g.my * SNF_CW;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / chunk);
+    const long r = e - (long)i * chunk;
+    const int kb = (int)(r / ((long)g.my * SNF_CW));
+    const int j = (int)(r / SNF_CW - (long)kb * g.my);
+    const int k = kb * SNF_CW + (int)(e & (SNF_CW - 1));
+    if (k >= g.mz) continue;
+    const long p = ((long)j * g.kxn + i) * g.mz + k;
+    if (dir > 0)
+      gst(reinterpret_cast<E*>(dst) + e, to_h<E>(gld(reinterpret_cast<const cplx*>(src) + p)));
+    else
+      gst(reinterpret_cast<cplx*>(dst) + p, to_d(gld(reinterpret_cast<const E*>(src) + e)));
   }
 }
 

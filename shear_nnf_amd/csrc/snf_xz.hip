@@ -3,13 +3,13 @@
 // forward transforms and the three output combinations.
 //
 // Data movement per substep (Nz = mgalz, Mx = mgalx):
-//   A  spectral [y][kx][kz] (5 inputs)  -> hybrid [f][y][kx][z], 6 fields,
-//      z-inverse DFT per kx column                 (grid kxn x my)
+//   A  spectral (5 inputs, kz-chunk layout) -> hybrid [f][y][kx][z], 6 fields,
+//      z-inverse DFT per (kx, y) row               (grid my x kxn)
 //   B  hybrid rows (all kx at one z)   -> physical x rows in LDS,
 //      u x omega, x-forward DFT -> hybrid [f][y][kx][z], 3 fields
 //                                                  (grid Nz/2 x my)
 //   C  hybrid columns -> z-forward DFT -> 3 spectral outputs
-//                                                  (grid kxn x my)
+//                                                  (grid my x kxn)
 // The kx=0 real-to-complex convention of a c2r transform (imaginary part
 // of the kx=0 coefficient ignored) and the 3/2-rule mode map of
 // fill00/pack00 (fou3D_thread.f:75-161) are reproduced exactly.
@@ -34,11 +34,11 @@ __device__ __forceinline__ PlaneIn load_in(const Geom& g, const cplx* vor, const
                                            const cplx* hg, const cplx* hv, const cplx* dvordy,
                                            int j, int i, int k) {
   PlaneIn r;
-  const long base = (long)j * g.P + (long)i * g.mz;
-  const long idx = base + k;
+  const long idx = sidx(g, j, i, k);
   if (i == 0 && k > g.nz) {
-    r.phic = cconj(phi[base + (g.mz - k)]);
-    r.ome2c = cconj(vor[base + (g.mz - k)]);
+    const long cj = sidx(g, j, 0, g.mz - k);
+    r.phic = cconj(phi[cj]);
+    r.ome2c = cconj(vor[cj]);
   } else {
     r.phic = phi[idx];
     r.ome2c = vor[idx];
@@ -86,8 +86,8 @@ __device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i,
 }
 
 // ---------------------------------------------------------------------------
-// A: spectral -> hybrid (inverse z DFT) of u,v,w,o1,o2,o3; one kx column per
-// block, two groups of three sequences (ping-pong LDS buffers)
+// A: spectral -> hybrid (inverse z DFT) of u,v,w,o1,o2,o3; one (kx, y) row per
+// block (grid my x kxn), two groups of three sequences (ping-pong LDS buffers)
 // ---------------------------------------------------------------------------
 #define XZ_MAXV 12       // values per thread per FFT pass in the x-row kernel
 #define XZ_B_MAXW 2      // (row pair, x) product items per thread
@@ -108,7 +108,7 @@ __device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor
   const int Nz = g.mgalz;
   H* a = reinterpret_cast<H*>(xz_smem);
   H* b = a + 3 * Nz;
-  const int i = blockIdx.x, j = blockIdx.y;
+  const int j = blockIdx.x, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
   const double u00j = u00[j], w00j = w00[j], dudyj = dudy[j], dwdyj = dwdy[j];
   const long fstride = (long)g.my * g.kxn * Nz;
   for (int grp = 0; grp < 2; ++grp) {
@@ -324,7 +324,7 @@ __device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, 
   const int Nz = g.mgalz;
   H* a = reinterpret_cast<H*>(xz_smem);
   H* b = a + 3 * Nz;
-  const int i = blockIdx.x, j = blockIdx.y;
+  const int j = blockIdx.x, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
   const long fstride = (long)g.my * g.kxn * Nz;
   for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
     const int f = t / Nz, z = t - f * Nz;
@@ -335,14 +335,13 @@ __device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, 
   const double norm = 1.0 / ((double)g.mgalx * (double)g.mgalz);
   const double kx = g.alp * (double)i;
   const double kx2 = kx * kx;
-  const long base = (long)j * g.P + (long)i * g.mz;
   for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
     const int pos = kz_pos(k, g.nz, g.mz, Nz);
     const cplx h1 = crmul(norm, to_d(r[pos]));
     const cplx h2 = crmul(norm, to_d(r[Nz + pos]));
     const cplx h3 = crmul(norm, to_d(r[2 * Nz + pos]));
     const double kz = kz_of(k, g.nz, g.mz, g.gam);
-    const long idx = base + k;
+    const long idx = sidx(g, j, i, k);
     const cplx v = ld_io(vel, idx, g.io32 & 1);
     st_io(dvordy, idx, cadd(mul_i(kx, h1), mul_i(kz, h3)), g.io32 & 2);
     const cplx d = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);

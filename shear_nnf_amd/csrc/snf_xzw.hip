@@ -3,16 +3,17 @@
 // snf_wfft.h: every transform runs inside one wave, in registers, with two
 // wave-private LDS exchanges and no block barrier.
 //
-//   WA  one wave per (kx, y) row: loads the 5 spectral inputs with the
-//       fill00 map, KMM reconstruction of u, v, w, omega (cross.f90:1221-
-//       1276), inverse z transform of the 6 fields -> hybrid [f][y][kx][z]
+//   WA  one wave per (kx, y) row, a block 4 consecutive y of one kx: loads
+//       the 5 spectral inputs with the fill00 map, KMM reconstruction of u,
+//       v, w, omega (cross.f90:1221-1276), inverse z transform of the 6
+//       fields -> hybrid [f][y][kx][z]
 //   WB  one block per (4 z-rows, y), one wave per row: the 6 hybrid fields
 //       staged through LDS (coalesced 64-byte runs of 4 z values), three
 //       paired c2r x transforms, u x omega, umax and the 'p' statistics
 //       (cross.f90:1430-1462, 2030-2045), r2c x transforms -> hybrid, 3 fields
-//   WC  one wave per (kx, y) row: forward z transform, pack00, normalisation
-//       1/(mgalx mgalz), output combinations and rf0u/rf0w
-//       (cross.f90:1477-1546)
+//   WC  one wave per (kx, y) row, blocks as WA: forward z transform,
+//       pack00, normalisation 1/(mgalx mgalz), output combinations and
+//       rf0u/rf0w (cross.f90:1477-1546)
 // Same data layout and the same results (to rounding) as the generic
 // Stockham kernels of snf_xz.hip, which serve every other grid size.
 #include "snf_internal.h"
@@ -30,6 +31,21 @@ __device__ __forceinline__ int wz_k(int pos, int nz, int mz, int Nz) {
   if (pos <= nz) return pos;
   if (pos >= Nz - nz) return pos - Nz + mz;
   return -1;
+}
+
+// Row (j, i) of wave w in the row kernels A and C: a block takes WAVES
+// consecutive y of one kx, so that together its waves read whole runs of
+// the kz chunks (WAVES * SNF_CW elements of each); consecutive blocks, on
+// one XCD, take the next y.  Grid: kxn * ceil(my / WAVES) blocks.
+__device__ __forceinline__ bool wrow(const Geom& g, int waves, int w, int& j, int& i) {
+  const long nyg = (g.my + waves - 1) / waves;
+  const long lb = xcd_block(blockIdx.x, gridDim.x);
+  i = (int)(lb / nyg);
+  j = (int)(lb - (long)i * nyg) * waves + w;
+  return i < g.kxn && j < g.my;
+}
+inline unsigned wrow_blocks(const Geom& g, int waves) {
+  return (unsigned)((long)g.kxn * ((g.my + waves - 1) / waves));
 }
 
 template <class H>
@@ -57,9 +73,9 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
   H* buf = lds + Nz + wave * W::LDS;
   stage_roots(rt, twz, Nz);
   __syncthreads();
-  const long row = (long)blockIdx.x * WA_WAVES + wave;   // row = j*kxn + i
-  if (row >= (long)g.my * g.kxn) return;
-  const int j = (int)(row / g.kxn), i = (int)(row - (long)j * g.kxn);
+  int j, i;
+  if (!wrow(g, WA_WAVES, wave, j, i)) return;
+  const long row = (long)j * g.kxn + i;
   const double u00j = gldd(u00 + j), w00j = gldd(w00 + j);
   const double dudyj = gldd(dudy + j), dwdyj = gldd(dwdy + j);
   H d[6][RZ];
@@ -67,18 +83,19 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
   for (int m = 0; m < RZ; ++m) {
     const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
     if (k >= 0) {
-      const long base = (long)j * g.P + (long)i * g.mz;
+      const long idx = sidx(g, j, i, k);
       PlaneIn q;
       if (i == 0 && k > g.nz) {   // set_conj (operators.f90:1382-1429)
-        q.phic = cconj(gld(phi + base + (g.mz - k)));
-        q.ome2c = cconj(gld(vor + base + (g.mz - k)));
+        const long cj = sidx(g, j, 0, g.mz - k);
+        q.phic = cconj(gld(phi + cj));
+        q.ome2c = cconj(gld(vor + cj));
       } else {
-        q.phic = gld(phi + base + k);
-        q.ome2c = gld(vor + base + k);
+        q.phic = gld(phi + idx);
+        q.ome2c = gld(vor + idx);
       }
-      q.v = ld_io(hg, base + k, g.io32 & 1);
-      q.dv = ld_io(hv, base + k, g.io32 & 1);
-      q.dome = ld_io(dvordy, base + k, g.io32 & 1);
+      q.v = ld_io(hg, idx, g.io32 & 1);
+      q.dv = ld_io(hv, idx, g.io32 & 1);
+      q.dome = ld_io(dvordy, idx, g.io32 & 1);
       cplx o[6];
       derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, o);
 #pragma unroll
@@ -316,9 +333,9 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
   H* buf = lds + Nz + wave * W::LDS;
   stage_roots(rt, twz, Nz);
   __syncthreads();
-  const long row = (long)blockIdx.x * WC_WAVES + wave;
-  if (row >= (long)g.my * g.kxn) return;
-  const int j = (int)(row / g.kxn), i = (int)(row - (long)j * g.kxn);
+  int j, i;
+  if (!wrow(g, WC_WAVES, wave, j, i)) return;
+  const long row = (long)j * g.kxn + i;
   const long fstride = (long)g.my * g.kxn * Nz;
   const H* src = hout + row * Nz;
   H h[3][RZ];
@@ -334,7 +351,6 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
   const double norm = 1.0 / ((double)g.mgalx * (double)Nz);
   const double kx = g.alp * (double)i;
   const double kx2 = kx * kx;
-  const long base = (long)j * g.P + (long)i * g.mz;
 #pragma unroll
   for (int m = 0; m < RZ; ++m) {
     const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
@@ -342,7 +358,7 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
       const cplx h1 = crmul(norm, to_d(h[0][m])), h2 = crmul(norm, to_d(h[1][m])),
                  h3 = crmul(norm, to_d(h[2][m]));
       const double kz = kz_of(k, g.nz, g.mz, g.gam);
-      const long idx = base + k;
+      const long idx = sidx(g, j, i, k);
       const cplx v = ld_io(vel, idx, g.io32 & 1);
       st_io(dvordy, idx, cadd(mul_i(kx, h1), mul_i(kz, h3)), g.io32 & 2);
       const cplx dd = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);

@@ -72,8 +72,8 @@ __global__ void __launch_bounds__(64) k_ydealias(YDealiasArgs a) {
       p = f == 0 ? a.u00c : a.w00c;
       stride = 1;
     } else {
-      p = (f == 0 ? a.vor : a.phi) + L;
-      stride = g.P;
+      p = (f == 0 ? a.vor : a.phi) + sline(g, i, (int)(L - (long)i * g.mz));
+      stride = SNF_CW;
     }
     __syncthreads();
     // moving frame exp(i kx y fac) (operators.f90:86-145); identity at kx = 0

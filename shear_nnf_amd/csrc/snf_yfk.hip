@@ -1,7 +1,7 @@
 // Explicit RK3 substep of the y stage (cross.f90:565-796) with the compact
 // y-operators applied spectrally (snf_yfft.h): one wave per (kx,kz) line,
 // YF_LINES lines per block.  The block moves every field through an LDS tile
-// of [my][YF_LINES] (coalesced rows of the [y][kx][kz] layout); each wave
+// of [my][YF_LINES] (one contiguous run of the kz-chunk layout); each wave
 // transforms its line in a private LDS column (and, for a second spectrum,
 // in its column of the tile).
 //
@@ -67,7 +67,10 @@ __device__ __forceinline__ cplx yph_entry(int t, int Q, const double* a, const d
   return yf_expi(ang);
 }
 
-// block-collective transfers of field F (stride P) for lines L0..L0+nv-1
+// block-collective transfers of field F (y stride P) for the nv lines from L0:
+// element (j, c) at F + j P + L0 + c.  In the kz-chunk layout P = SNF_CW and
+// the block's lines are one contiguous run.
+static_assert(YF_LINES == SNF_CW, "a y block is one kz chunk");
 __device__ __forceinline__ void yf_fill(cplx* T, const cplx* F, long P, long L0, int nv, int N) {
   __syncthreads();
   for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
@@ -168,10 +171,8 @@ __device__ __forceinline__ void yf_flush_io(const cplx* T, cplx* F, long P, long
   __syncthreads();
 }
 
-// The RK accumulators vorwk, phiwk are private to this path and kept
-// block-major: block lb owns the contiguous chunk W + lb N YF_LINES, in the
-// element order of the tile transfers (e = j YF_LINES + c).  The chunk moves
-// as one 24.6 KB run at my = 385, instead of my rows of 64 bytes.
+// Transfers of a block's chunk W (the RK accumulators vorwk, phiwk at the
+// block's L0) in the element order of the tile transfers (e = j YF_LINES + c).
 template <int Q>
 __device__ __forceinline__ void yf_issue_blk(cplx (&r)[Q], const cplx* W, int N) {
 #pragma unroll
@@ -224,18 +225,18 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   if (a.sc->halted) return;
   extern __shared__ cplx T[];
   const Geom& g = a.g;
-  const long P = g.P;
+  const long P = SNF_CW;   // y stride of a line
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // blocks: YF_LINES consecutive kz lines of one kx (nch chunks per kx)
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
   // ---- (kx, kz) lines; every wave takes part in every block transfer.
-  // Neighbouring blocks share the 128-byte rows of the field: keep them on
-  // one XCD.  A line block issues its vor loads first, so that they are in
-  // flight while the block stages its tables.
+  // The blocks of one kx share its multiplier rows: keep them on one XCD.
+  // A line block issues its vor loads first, so that they are in flight
+  // while the block stages its tables.
   const long lb = xcd_block(blockIdx.x, nbl);
   const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
-  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const long L0 = sline(g, i, YF_LINES * ch);   // the block's chunk: my * YF_LINES elements
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
   cplx pv[Q];
   if (blockIdx.x != nbl) yf_issue(pv, a.vor, P, L0, nv, N);
@@ -309,8 +310,8 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
     return;
   }
 
-  cplx* vwk_blk = a.vorwk + lb * (long)N * YF_LINES;    // block-major accumulators
-  cplx* pwk_blk = a.phiwk + lb * (long)N * YF_LINES;
+  cplx* vwk_blk = a.vorwk + L0;    // the block's chunk of the accumulators
+  cplx* pwk_blk = a.phiwk + L0;
   const int k = min(YF_LINES * ch + w, g.mz - 1);
   const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
   const double k2 = kx * kx + kzv * kzv;
@@ -597,14 +598,14 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   if (a.sc->halted) return;
   extern __shared__ cplx T[];
   const Geom& g = a.g;
-  const long P = g.P;
+  const long P = SNF_CW;   // y stride of a line
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
   // a line block's vor loads are in flight while it stages its tables
   const long lb = xcd_block(blockIdx.x, nbl);
   const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
-  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const long L0 = sline(g, i, YF_LINES * ch);   // the block's chunk: my * YF_LINES elements
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
   cplx pv[Q];
   if (blockIdx.x != nbl) yf_issue(pv, a.vor, P, L0, nv, N);
@@ -685,8 +686,8 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
     return;
   }
 
-  cplx* vwk_blk = a.vorwk + lb * (long)N * YF_LINES;    // block-major accumulators
-  cplx* pwk_blk = a.phiwk + lb * (long)N * YF_LINES;
+  cplx* vwk_blk = a.vorwk + L0;    // the block's chunk of the accumulators
+  cplx* pwk_blk = a.phiwk + L0;
   const int k = min(YF_LINES * ch + w, g.mz - 1);
   const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
   const double k2 = kx * kx + kzv * kzv;
@@ -1007,7 +1008,7 @@ k_yf_first(YArgs a) {
   if (a.sc->halted) return;
   extern __shared__ cplx T[];
   const Geom& g = a.g;
-  const long P = g.P;
+  const long P = SNF_CW;   // y stride of a line
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
@@ -1047,7 +1048,7 @@ k_yf_first(YArgs a) {
 
   const long lb = xcd_block(blockIdx.x, nbl);
   const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
-  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const long L0 = sline(g, i, YF_LINES * ch);   // the block's chunk: my * YF_LINES elements
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
   const int k = min(YF_LINES * ch + w, g.mz - 1);
   const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
@@ -1164,7 +1165,7 @@ k_yf_filter(YArgs a) {
   if (a.sc->halted) return;
   extern __shared__ cplx T[];
   const Geom& g = a.g;
-  const long P = g.P;
+  const long P = SNF_CW;   // y stride of a line
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
@@ -1216,7 +1217,7 @@ k_yf_filter(YArgs a) {
   }
   const long lb = xcd_block(blockIdx.x, nbl);
   const int i = (int)(lb / nch), ch = (int)(lb - (long)i * nch);
-  const long L0 = (long)i * g.mz + YF_LINES * ch;
+  const long L0 = sline(g, i, YF_LINES * ch);   // the block's chunk: my * YF_LINES elements
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);
   const cplx so = gld(a.shp_old + i);
   const double tho = atan2(so.y, so.x) * invN;

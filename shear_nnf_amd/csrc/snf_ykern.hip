@@ -248,10 +248,11 @@ k_y_first(YArgs a) {
     int i, k;
     double kx, k2;
     line_geom(a.g, L, i, k, kx, k2);
-    if (L != 0) helm_line(a, L, k2, gld(a.shp_old + i), gld(a.shm_old + i), a.phi + L, P, a.s2 + L, a.v + L, P);
+    const long LB = sline(a.g, i, k), FS = SNF_CW;   // the line in the fields
+    if (L != 0) helm_line(a, L, k2, gld(a.shp_old + i), gld(a.shm_old + i), a.phi + LB, FS, a.s2 + L, a.v + LB, FS);
     const DOp o = make_dop(a.b1, a.tw_old + i, gld(a.shp_old + i), a.g.my);
-    d1_one<8>(a, o, gld(a.shm_old + i), a.vor + L, P, a.s1 + L, a.dome + L, P);
-    d1_one<8>(a, o, gld(a.shm_old + i), a.v + L, P, a.s2 + L, a.dv + L, P);
+    d1_one<8>(a, o, gld(a.shm_old + i), a.vor + LB, FS, a.s1 + L, a.dome + LB, FS);
+    d1_one<8>(a, o, gld(a.shm_old + i), a.v + LB, FS, a.s2 + L, a.dv + LB, FS);
   } else if (L == P) {
     d1_real(a, L, a.u00c, a.dudy);
   } else {
@@ -373,26 +374,27 @@ k_y_explicit(YArgs a, int rk) {
     int i, k;
     double kx, k2;
     line_geom(a.g, L, i, k, kx, k2);
+    const long LB = sline(a.g, i, k), FS = SNF_CW;   // the line in the fields
     const DOp o1 = make_dop(a.b1, a.tw_old + i, gld(a.shp_old + i), N);
     const DOp o2 = make_dop(a.b2, a.tw_old + a.g.kxn + i, gld(a.shp_old + i), N);
     const DOp o1n = make_dop(a.b1, a.tw_new + i, gld(a.shp_new + i), N);
     if (vchain) {
-      const cplx* in[1] = {a.vor + L};
+      const cplx* in[1] = {a.vor + LB};
       cplx* ys[1] = {a.s1 + L};
-      deriv_forward<1, false, 8>(o2, a.c.rh2, in, P, ys, a.PS, gld(a.shm_old + i));
-      VorSink sk{a.rhg + L, a.vor + L, a.vorwk + L, P, fac, k2, kx, dt, c_gama[ir], c_zeta[ir],
+      deriv_forward<1, false, 8>(o2, a.c.rh2, in, FS, ys, a.PS, gld(a.shm_old + i));
+      VorSink sk{a.rhg + LB, a.vor + LB, a.vorwk + LB, FS, fac, k2, kx, dt, c_gama[ir], c_zeta[ir],
                  f1, f2, a.g.dy, a.g.Ly, rk, L == 0};
       deriv_backward<1, 4>(o2, ys, a.PS, sk);
-      d1_one<8>(a, o1n, gld(a.shm_new + i), a.vor + L, P, a.s1 + L, a.dome + L, P);
+      d1_one<8>(a, o1n, gld(a.shm_new + i), a.vor + LB, FS, a.s1 + L, a.dome + LB, FS);
     } else {
-      deriv_forward_dual<true, false, 8>(o1, a.c.rh1, o2, a.c.rh2, a.rdome + L, a.phi + L, P,
+      deriv_forward_dual<true, false, 8>(o1, a.c.rh1, o2, a.c.rh2, a.rdome + LB, a.phi + LB, FS,
                                          a.s2 + L, a.s3 + L, a.PS, gld(a.shm_old + i));
-      PhiSink sk{a.rhv + L, a.phi + L, a.phiwk + L, P, fac, k2, kx, dt, c_gama[ir], c_zeta[ir],
+      PhiSink sk{a.rhv + LB, a.phi + LB, a.phiwk + LB, FS, fac, k2, kx, dt, c_gama[ir], c_zeta[ir],
                  f1, f2, a.g.dy, a.g.Ly, rk, L == 0};
       deriv_backward_dual<4>(o1, o2, a.s2 + L, a.s3 + L, a.PS, sk);
       if (L != 0)
-        helm_line(a, L, k2, gld(a.shp_new + i), gld(a.shm_new + i), a.phi + L, P, a.s2 + L, a.v + L, P);
-      d1_one<8>(a, o1n, gld(a.shm_new + i), a.v + L, P, a.s2 + L, a.dv + L, P);
+        helm_line(a, L, k2, gld(a.shp_new + i), gld(a.shm_new + i), a.phi + LB, FS, a.s2 + L, a.v + LB, FS);
+      d1_one<8>(a, o1n, gld(a.shm_new + i), a.v + LB, FS, a.s2 + L, a.dv + LB, FS);
     }
   } else {
     const long L = P + (t - 2 * P);
@@ -491,21 +493,22 @@ k_y_implicit(YArgs a, int rk) {
     int i, k;
     double kx, k2;
     line_geom(a.g, L, i, k, kx, k2);
+    const long LB = sline(a.g, i, k), FS = SNF_CW;   // the line in the fields
     {
       const DOp o = make_dop(a.b1, a.tw_old + i, gld(a.shp_old + i), N);
-      const cplx* in[1] = {a.rdome + L};
+      const cplx* in[1] = {a.rdome + LB};
       cplx* ys[1] = {a.s1 + L};
-      deriv_forward<1, true, 4>(o, a.c.rh1, in, P, ys, a.PS, gld(a.shm_old + i));
-      SubSink sk{a.rhv + L, P};
+      deriv_forward<1, true, 4>(o, a.c.rh1, in, FS, ys, a.PS, gld(a.shm_old + i));
+      SubSink sk{a.rhv + LB, FS};
       deriv_backward<1, 4>(o, ys, a.PS, sk);
     }
     {
       const DOp o = make_dop(a.b2, a.tw_old + a.g.kxn + i, gld(a.shp_old + i), N);
-      const cplx* in[2] = {a.vor + L, a.phi + L};
+      const cplx* in[2] = {a.vor + LB, a.phi + LB};
       cplx* ys[2] = {a.s1 + L, a.s2 + L};
-      deriv_forward<2, false, 4>(o, a.c.rh2, in, P, ys, a.PS, gld(a.shm_old + i));
-      ImplicitRKSink sk{a.rhg + L, a.rhv + L, a.vor + L, a.phi + L, a.vorwk + L, a.phiwk + L,
-                        P, fac, facb, rdt, k2, kx, c_gama[ir], f2, a.g.dy, a.g.Ly, rk, L == 0};
+      deriv_forward<2, false, 4>(o, a.c.rh2, in, FS, ys, a.PS, gld(a.shm_old + i));
+      ImplicitRKSink sk{a.rhg + LB, a.rhv + LB, a.vor + LB, a.phi + LB, a.vorwk + LB, a.phiwk + LB,
+                        FS, fac, facb, rdt, k2, kx, c_gama[ir], f2, a.g.dy, a.g.Ly, rk, L == 0};
       deriv_backward<2, 2>(o, ys, a.PS, sk);
     }
     // (8) Helmholtz solves for vor and phi with k2 + dtr1      (cross.f90:898-928)
@@ -514,28 +517,28 @@ k_y_implicit(YArgs a, int rk) {
       lapmat(a.c, dtr1 + k2, lm);
       HScratch hs{a.hs.hr + L, a.hs.hc + L};
       HTail tail;
-      const cplx* in[2] = {a.vorwk + L, a.phiwk + L};
+      const cplx* in[2] = {a.vorwk + LB, a.phiwk + LB};
       cplx* ys[2] = {a.s1 + L, a.s2 + L};
-      helm_forward<2, 4>(lm, a.c.pr2, N, gld(a.shp_new + i), gld(a.shm_new + i), in, P, ys, a.PS, hs, tail);
-      cplx* outs[2] = {a.vor + L, a.phi + L};
-      StoreSink<2> sk{outs, P};
+      helm_forward<2, 4>(lm, a.c.pr2, N, gld(a.shp_new + i), gld(a.shm_new + i), in, FS, ys, a.PS, hs, tail);
+      cplx* outs[2] = {a.vor + LB, a.phi + LB};
+      StoreSink<2> sk{outs, FS};
       helm_backward<2, 4>(N, ys, a.PS, hs, tail, sk);
     }
     // (9) zeta * [hg, hv] -> [vorwk, phiwk] with the phase map (cross.f90:931-944)
     if (rk != 3) {
       const double zet = c_zeta[ir];
       for (int j = 0; j < N; ++j) {
-        const long o = (long)j * P + L;
+        const long o = LB + (long)j * FS;
         const double yj = (double)j * a.g.dy - 0.5 * a.g.Ly;
         const cplx m1 = mframe(kx, yj, f1);
         gst(a.vorwk + o, cmul(crmul(zet, gld(a.rhg + o)), m1));
         gst(a.phiwk + o, cmul(crmul(zet, gld(a.rhv + o)), m1));
       }
     }
-    if (L != 0) helm_line(a, L, k2, gld(a.shp_new + i), gld(a.shm_new + i), a.phi + L, P, a.s1 + L, a.v + L, P);
+    if (L != 0) helm_line(a, L, k2, gld(a.shp_new + i), gld(a.shm_new + i), a.phi + LB, FS, a.s1 + L, a.v + LB, FS);
     const DOp o1n = make_dop(a.b1, a.tw_new + i, gld(a.shp_new + i), N);
-    d1_one<8>(a, o1n, gld(a.shm_new + i), a.vor + L, P, a.s1 + L, a.dome + L, P);
-    d1_one<8>(a, o1n, gld(a.shm_new + i), a.v + L, P, a.s2 + L, a.dv + L, P);
+    d1_one<8>(a, o1n, gld(a.shm_new + i), a.vor + LB, FS, a.s1 + L, a.dome + LB, FS);
+    d1_one<8>(a, o1n, gld(a.shm_new + i), a.v + LB, FS, a.s2 + L, a.dv + LB, FS);
   } else {
     cplx* u = (L == P) ? a.u00c : a.w00c;
     cplx* wk = (L == P) ? a.u00wk : a.w00wk;
@@ -601,11 +604,14 @@ k_op_helm(int N, long nl, YCoef c, const double* k2, const cplx* shp,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_energy_part(Geom g, const cplx* vor, const cplx* phi, double* part) {
-  const long n = (long)g.my * g.P;
+  // element e of the kz-chunk layout: kz chunk t / my of kx i, kz % SNF_CW = e % SNF_CW
+  const long n = sfield_len(g), chunk = (long)g.nkzb * g.my * SNF_CW;
   double sa = 0.0, sb = 0.0;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-    const long L = e % g.P;
-    const double wt = (L < g.mz) ? 1.0 : 2.0;   // kx index 0 <=> L < mz
+    const long r = e % chunk;   // within kx = e / chunk
+    const int k = (int)(r / ((long)g.my * SNF_CW)) * SNF_CW + (int)(e & (SNF_CW - 1));
+    if (k >= g.mz) continue;
+    const double wt = (e < chunk) ? 1.0 : 2.0;   // kx = 0 counted once
     sa += wt * cabs2(gld(vor + e));
     sb += wt * cabs2(gld(phi + e));
   }
