@@ -362,7 +362,8 @@ void launch_xzw_any(snf_handle* h, int which, int rk) {
 template <class H>
 void launch_xz_generic(snf_handle* h, int which, int rk) {
   const Geom& g = h->g;
-  dim3 gA(g.my, g.kxn);
+  const int rb = xz_rows_per_block(g.mgalz);
+  dim3 gA((g.my + rb - 1) / rb, g.kxn);
   H* hin = reinterpret_cast<H*>(h->hyb_in);
   H* hout = reinterpret_cast<H*>(h->hyb_out);
   if (which == 0) {
@@ -476,6 +477,9 @@ void with_yplan(int my, bool mfma, F&& f) {
     case 192: f(PlanTag<YFFT<8, 8, 3>>{}); break;
     case 256: f(PlanTag<YFFT<8, 8, 4>>{}); break;
     case 385: f(PlanTag<YFFT<11, 7, 5>>{}); break;
+    case 97: f(PlanTag<YRader<97, 5, 8, 4, 3>>{}); break;
+    case 193: f(PlanTag<YRader<193, 5, 8, 8, 3>>{}); break;
+    case 257: f(PlanTag<YRader<257, 3, 8, 8, 4>>{}); break;
     default:
       if (mfma) {
         switch ((my + 63) / 64) {
@@ -507,6 +511,70 @@ int y_plan_radices(int my, int* R) {
       return np;
     }
   return 0;
+}
+// Prime line lengths with a Rader plan (YRader in with_yplan): the primitive
+// root g and the radices of the P - 1 point plan; 0 if my has none.
+int y_rader_plan(int my, int* g, int* R) {
+  static const int plans[][5] = {{97, 5, 8, 4, 3}, {193, 5, 8, 8, 3}, {257, 3, 8, 8, 4}};
+  for (const auto& pl : plans)
+    if (pl[0] == my) {
+      *g = pl[1];
+      int np = 0;
+      while (np < 3 && pl[2 + np]) {
+        R[np] = pl[2 + np];
+        ++np;
+      }
+      return np;
+    }
+  return 0;
+}
+// pass s > 0 of the plan R[0..np-1]: exp(-2 pi i r k / (Ns R)) at [r - 1][k]
+void y_pass_twiddles(const int* R, int np, cplx* tp) {
+  int Ns = R[0], off = 0;
+  for (int s = 1; s < np; ++s) {
+    for (int r = 1; r < R[s]; ++r)
+      for (int k = 0; k < Ns; ++k) {
+        const long double a =
+            -2.0L * 3.14159265358979323846264338327950288L * r * k / ((long double)Ns * R[s]);
+        tp[off + (r - 1) * Ns + k] = mkc((double)cosl(a), (double)sinl(a));
+      }
+    off += (R[s] - 1) * Ns;
+    Ns *= R[s];
+  }
+}
+// The table of YRader<P, g, R...> (yf_rader): pass twiddles of the M = P - 1
+// point plan, DFT_M(v) / M for both signs with v[s] = exp(sign 2 pi i g^-s / P),
+// and the index maps (g^q, g^-q) mod P.  False if g is not a primitive root.
+bool y_rader_table(int P, int g, const int* R, int np, std::vector<cplx>& tp) {
+  const int M = P - 1;
+  std::vector<int> gp(M), gm(M);
+  std::vector<char> seen(P, 0);
+  long e = 1;
+  for (int q = 0; q < M; ++q) {
+    gp[q] = (int)e;
+    if (seen[e]) return false;
+    seen[e] = 1;
+    e = e * g % P;
+  }
+  for (int q = 0; q < M; ++q) gm[q] = gp[(M - q) % M];   // g^-q = g^(M - q)
+  tp.assign(4 * (size_t)M, mkc(0.0, 0.0));
+  y_pass_twiddles(R, np, tp.data());
+  const long double pi2 = 2.0L * 3.14159265358979323846264338327950288L;
+  for (int sg = 0; sg < 2; ++sg) {
+    const long double sign = sg == 0 ? -1.0L : 1.0L;
+    for (int m = 0; m < M; ++m) {
+      long double re = 0.0L, im = 0.0L;
+      for (int s2 = 0; s2 < M; ++s2) {
+        // v[s2] exp(-2 pi i m s2 / M)
+        const long double a = sign * pi2 * gm[s2] / P - pi2 * (long double)((long)m * s2 % M) / M;
+        re += cosl(a);
+        im += sinl(a);
+      }
+      tp[(size_t)(1 + sg) * M + m] = mkc((double)(re / M), (double)(im / M));
+    }
+  }
+  for (int q = 0; q < M; ++q) tp[3 * (size_t)M + q] = mkc((double)gp[q], (double)gm[q]);
+  return true;
 }
 
 // the spectral y stage of one substep: explicit or implicit viscous terms
@@ -648,13 +716,13 @@ template <class H>
 void tab_A(const snf_handle* h, const BxXzA* t, int n, hipStream_t s) {
   lds_attr((const void*)k_xz_A_tab<H>, h->ldsA, h->dev);
   if (n == 0) return;   // the LDS limit only (set before a graph capture)
-  k_xz_A_tab<H><<<dim3(h->g.my, h->g.kxn, n), 256, h->ldsA, s>>>(t);
+  k_xz_A_tab<H><<<dim3((h->g.my + xz_rows_per_block(h->g.mgalz) - 1) / xz_rows_per_block(h->g.mgalz), h->g.kxn, n), 256, h->ldsA, s>>>(t);
 }
 template <class H>
 void tab_C(const snf_handle* h, const BxXzC* t, int n, hipStream_t s) {
   lds_attr((const void*)k_xz_C_tab<H>, h->ldsC, h->dev);
   if (n == 0) return;
-  k_xz_C_tab<H><<<dim3(h->g.my, h->g.kxn, n), 256, h->ldsC, s>>>(t);
+  k_xz_C_tab<H><<<dim3((h->g.my + xz_rows_per_block(h->g.mgalz) - 1) / xz_rows_per_block(h->g.mgalz), h->g.kxn, n), 256, h->ldsC, s>>>(t);
 }
 template <int NT, class H>
 void tab_B_nt(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
@@ -936,8 +1004,8 @@ const char* snf_last_error(void) { return g_err.c_str(); }
 // DFT where its my^2 work per line still beats the banded sweeps (measured:
 // 3.2x faster at my = 32, 1.8x at 129, 2.2x slower at 193)
 static bool y_spectral(int my) {
-  int R[4];
-  return y_plan_radices(my, R) > 0 || my <= Y_DIRECT_MAX;
+  int R[4], g;
+  return y_plan_radices(my, R) > 0 || y_rader_plan(my, &g, R) > 0 || my <= Y_DIRECT_MAX;
 }
 
 int snf_create(const snf_params* p, snf_handle** out) {
@@ -1083,7 +1151,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   TRY(A(&h->pplane, (size_t)my * 3));
   TRY(A(&h->upart, (size_t)UMAX_SLOTS * 4));
   TRY(A(&h->twy, (size_t)my));
-  TRY(A(&h->tpy, (size_t)my));
+  TRY(A(&h->tpy, 4 * (size_t)my));   // plan tables: up to 4 (my - 1) entries (Rader)
   for (int q = 0; q < 2; ++q) TRY(A(&h->ysym[q], (size_t)g.kxn * my * YS));
   TRY(A(&h->stats, (size_t)my * SNF_NSTAT));
   TRY(A(&h->stats2, (size_t)my * SNF_NSTAT));
@@ -1141,29 +1209,30 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
     TRY(hipMemcpy(h->twy, ty.data(), ty.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y twiddles"));
-    // pass s > 0 of the plan: exp(-2 pi i r k / (Ns R)) at [r - 1][k]
+    // the plan's table: pass twiddles of a radix plan, or a Rader table
     std::vector<cplx> tp(my, mkc(0.0, 0.0));
-    int R[4];
+    int R[4], rg = 0;
     const int np = y_plan_radices(my, R);
     if (np > 0) {
-      if (yf_twiddle_count(R, np) > my) return fail(SNF_BAD_PARAM, "y twiddle table larger than my");
-      int Ns = R[0], off = 0;
-      for (int s = 1; s < np; ++s) {
-        for (int r = 1; r < R[s]; ++r)
-          for (int k = 0; k < Ns; ++k) {
-            const long double a =
-                -2.0L * 3.14159265358979323846264338327950288L * r * k / ((long double)Ns * R[s]);
-            tp[off + (r - 1) * Ns + k] = mkc((double)cosl(a), (double)sinl(a));
-          }
-        off += (R[s] - 1) * Ns;
-        Ns *= R[s];
+      if (yf_twiddle_count(R, np) > my) {
+        snf_destroy(h);
+        return fail(SNF_BAD_PARAM, "y twiddle table larger than my");
+      }
+      y_pass_twiddles(R, np, tp.data());
+    } else if (const int nr = y_rader_plan(my, &rg, R)) {
+      if (!y_rader_table(my, rg, R, nr, tp)) {
+        snf_destroy(h);
+        return fail(SNF_BAD_PARAM, "y Rader plan: not a primitive root");
       }
     }
     TRY(hipMemcpy(h->tpy, tp.data(), tp.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y pass twiddles"));
-    // tile, a column per wave, roots, two multiplier rows, four phase tables
-    // (1472 B at my = 385: 81552 B, two blocks still fit one CU's 160 KiB)
-    h->ldsY = ((size_t)my * (2 * YF_LINES + 5) + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
+    // tile, a column per wave, the plan's table, two multiplier rows, four
+    // phase tables (1472 B at my = 385: 81552 B, two blocks still fit one
+    // CU's 160 KiB)
+    size_t twn = (size_t)my;
+    with_yplan(my, h->ymfma, [&](auto tag) { twn = (size_t)decltype(tag)::type::tw_len(my); });
+    h->ldsY = ((size_t)my * (2 * YF_LINES + 4) + twn + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
     with_yplan(my, h->ymfma, [&](auto tag) {
       typedef typename decltype(tag)::type PL;
       for (const void* ky : {(const void*)k_yf_explicit<PL, 0>, (const void*)k_yf_implicit<PL, 0>,
@@ -1239,9 +1308,11 @@ int snf_create(const snf_params* p, snf_handle** out) {
   }
   // Stockham x-z kernels: elements of es bytes (FP32 in the mixed mode)
   const size_t es = h->mixed ? sizeof(cplxf) : sizeof(cplx);
-  h->ldsA = 2 * 3 * (size_t)mgalz * es;   // ping-pong, 3 sequences
+  // ping-pong buffers of 3 sequences per row, xz_rows_per_block rows, and
+  // the FP64 z roots
+  h->ldsA = 2 * 3 * (size_t)xz_rows_per_block(mgalz) * mgalz * es + (size_t)mgalz * sizeof(cplx);
   h->ldsB = 3 * (size_t)h->zb * mgalx * es;
-  h->ldsC = 2 * 3 * (size_t)mgalz * es;
+  h->ldsC = h->ldsA;
   lds_attr((const void*)k_ydealias, 3 * (size_t)my * sizeof(cplx), h->dev);
   for (const void* k : {(const void*)k_xz_A<cplx>, (const void*)k_xz_A<cplxf>})
     lds_attr(k, h->ldsA, h->dev);

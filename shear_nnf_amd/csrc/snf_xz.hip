@@ -4,12 +4,12 @@
 //
 // Data movement per substep (Nz = mgalz, Mx = mgalx):
 //   A  spectral (5 inputs, kz-chunk layout) -> hybrid [f][y][kx][z], 6 fields,
-//      z-inverse DFT per (kx, y) row               (grid my x kxn)
+//      z-inverse DFT per (kx, y) row      (grid my / rows x kxn)
 //   B  hybrid rows (all kx at one z)   -> physical x rows in LDS,
 //      u x omega, x-forward DFT -> hybrid [f][y][kx][z], 3 fields
 //                                                  (grid Nz/2 x my)
 //   C  hybrid columns -> z-forward DFT -> 3 spectral outputs
-//                                                  (grid my x kxn)
+//                                         (grid my / rows x kxn)
 // The kx=0 real-to-complex convention of a c2r transform (imaginary part
 // of the kx=0 coefficient ignored) and the 3/2-rule mode map of
 // fill00/pack00 (fou3D_thread.f:75-161) are reproduced exactly.
@@ -86,15 +86,29 @@ __device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i,
 }
 
 // ---------------------------------------------------------------------------
-// A: spectral -> hybrid (inverse z DFT) of u,v,w,o1,o2,o3; one (kx, y) row per
-// block (grid my x kxn), two groups of three sequences (ping-pong LDS buffers)
+// A: spectral -> hybrid (inverse z DFT) of u,v,w,o1,o2,o3; xz_rows_per_block
+// (kx, y) rows per block (grid my / rows x kxn), two groups of three sequences
+// per row (ping-pong LDS buffers)
 // ---------------------------------------------------------------------------
 #define XZ_MAXV 12       // values per thread per FFT pass in the x-row kernel
 #define XZ_B_MAXW 2      // (row pair, x) product items per thread
 
 
+// Rows (kx, y) per block of the Stockham A and C kernels: enough that the
+// block's 3 * rows sequences of Nz points keep its 256 threads busy (short
+// planes), at most 8.  The host sizes the grid (my / rows blocks per kx)
+// and the LDS (2 * 3 * rows * Nz elements and Nz FP64 roots) with the same
+// function.
+__host__ __device__ __forceinline__ int xz_rows_per_block(int Nz) {
+  const int r = 1536 / (3 * Nz);
+  return r < 1 ? 1 : (r > 8 ? 8 : r);
+}
+
 // H: element type of the transforms and hybrid planes (cplx, or cplxf in the
-// mixed precision mode); the spectral fields and the reconstruction are FP64
+// mixed precision mode); the spectral fields and the reconstruction are FP64.
+// A block takes rows y = j0 .. j0 + rows - 1 of one kx; its loads run over
+// (kz chunk, row, kz % SNF_CW), so consecutive threads read consecutive
+// elements of the kz-chunk layout.
 template <class H>
 __device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
        const cplx* __restrict__ hg, const cplx* __restrict__ hv,
@@ -105,28 +119,34 @@ __device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor
   typedef typename wf_real<H>::T R;
   if (sc->halted) return;
   extern __shared__ double4 xz_smem[];
-  const int Nz = g.mgalz;
+  const int Nz = g.mgalz, RB = xz_rows_per_block(Nz);
   H* a = reinterpret_cast<H*>(xz_smem);
-  H* b = a + 3 * Nz;
-  const int j = blockIdx.x, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
-  const double u00j = u00[j], w00j = w00[j], dudyj = dudy[j], dwdyj = dwdy[j];
+  H* b = a + 3 * RB * Nz;
+  cplx* tw = reinterpret_cast<cplx*>(b + 3 * RB * Nz);   // the z roots, staged once
+  for (int t = threadIdx.x; t < Nz; t += blockDim.x) tw[t] = gld(twz + t);
+  const int j0 = blockIdx.x * RB, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
+  const int nr = min(RB, g.my - j0);                 // rows of this block
   const long fstride = (long)g.my * g.kxn * Nz;
+  const int nload = g.nkzb * nr * SNF_CW;
   for (int grp = 0; grp < 2; ++grp) {
-    for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) a[t] = wf_mk((R)0, (R)0);
+    for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) a[t] = wf_mk((R)0, (R)0);
     __syncthreads();
-    for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
+    for (int t = threadIdx.x; t < nload; t += blockDim.x) {
+      const int c = t & (SNF_CW - 1), r = (t / SNF_CW) % nr, k = (t / (SNF_CW * nr)) * SNF_CW + c;
+      if (k >= g.mz) continue;
+      const int j = j0 + r;
       PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
       cplx d[6];
-      derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, d);
+      derived6(g, q, i, k, gldd(u00 + j), gldd(w00 + j), gldd(dudy + j), gldd(dwdy + j), vbulk, d);
       const int pos = kz_pos(k, g.nz, g.mz, Nz);
 #pragma unroll
-      for (int f = 0; f < 3; ++f) a[f * Nz + pos] = to_h<H>(d[grp * 3 + f]);
+      for (int f = 0; f < 3; ++f) a[(3 * r + f) * Nz + pos] = to_h<H>(d[grp * 3 + f]);
     }
     __syncthreads();
-    H* r = fft_lds(a, b, 3, pz, twz, +1);
-    for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
-      const int f = t / Nz, z = t - f * Nz;
-      hyb[(grp * 3 + f) * fstride + ((long)j * g.kxn + i) * Nz + z] = r[t];
+    H* res = fft_lds(a, b, 3 * nr, pz, tw, +1);
+    for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) {
+      const int rf = t / Nz, z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
+      hyb[(grp * 3 + f) * fstride + ((long)(j0 + r) * g.kxn + i) * Nz + z] = res[t];
     }
     __syncthreads();
   }
@@ -321,25 +341,33 @@ __device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, 
        const DevScalars* sc) {
   if (sc->halted) return;
   extern __shared__ double4 xz_smem[];
-  const int Nz = g.mgalz;
+  const int Nz = g.mgalz, RB = xz_rows_per_block(Nz);
   H* a = reinterpret_cast<H*>(xz_smem);
-  H* b = a + 3 * Nz;
-  const int j = blockIdx.x, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
+  H* b = a + 3 * RB * Nz;
+  cplx* tw = reinterpret_cast<cplx*>(b + 3 * RB * Nz);   // the z roots, staged once
+  for (int t = threadIdx.x; t < Nz; t += blockDim.x) tw[t] = gld(twz + t);
+  const int j0 = blockIdx.x * RB, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
+  const int nr = min(RB, g.my - j0);
   const long fstride = (long)g.my * g.kxn * Nz;
-  for (int t = threadIdx.x; t < 3 * Nz; t += blockDim.x) {
-    const int f = t / Nz, z = t - f * Nz;
-    a[t] = hout[f * fstride + ((long)j * g.kxn + i) * Nz + z];
+  for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) {
+    const int rf = t / Nz, z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
+    a[t] = hout[f * fstride + ((long)(j0 + r) * g.kxn + i) * Nz + z];
   }
   __syncthreads();
-  H* r = fft_lds(a, b, 3, pz, twz, -1);
+  H* res = fft_lds(a, b, 3 * nr, pz, tw, -1);
   const double norm = 1.0 / ((double)g.mgalx * (double)g.mgalz);
   const double kx = g.alp * (double)i;
   const double kx2 = kx * kx;
-  for (int k = threadIdx.x; k < g.mz; k += blockDim.x) {
+  const int nst = g.nkzb * nr * SNF_CW;
+  for (int t = threadIdx.x; t < nst; t += blockDim.x) {
+    const int c = t & (SNF_CW - 1), r = (t / SNF_CW) % nr, k = (t / (SNF_CW * nr)) * SNF_CW + c;
+    if (k >= g.mz) continue;
+    const int j = j0 + r;
+    const H* rr = res + 3 * r * Nz;
     const int pos = kz_pos(k, g.nz, g.mz, Nz);
-    const cplx h1 = crmul(norm, to_d(r[pos]));
-    const cplx h2 = crmul(norm, to_d(r[Nz + pos]));
-    const cplx h3 = crmul(norm, to_d(r[2 * Nz + pos]));
+    const cplx h1 = crmul(norm, to_d(rr[pos]));
+    const cplx h2 = crmul(norm, to_d(rr[Nz + pos]));
+    const cplx h3 = crmul(norm, to_d(rr[2 * Nz + pos]));
     const double kz = kz_of(k, g.nz, g.mz, g.gam);
     const long idx = sidx(g, j, i, k);
     const cplx v = ld_io(vel, idx, g.io32 & 1);

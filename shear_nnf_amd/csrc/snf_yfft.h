@@ -321,15 +321,90 @@ __device__ __forceinline__ void yf_dft_mfma(cplx* T, Col col, const cplx* rt, in
   __syncthreads();   // the columns of the other waves are written
 }
 
+// A column viewed from offset o: element j is element j + o of col.
+template <class Col> struct ShiftCol {
+  Col c;
+  int o;
+  __device__ __forceinline__ int operator()(int j) const { return c(j + o); }
+};
+
+// DFT of prime length P in place on column col (natural order in and out,
+// unnormalised) by Rader's algorithm: with g a primitive root mod P and
+// M = P - 1, the outputs X[g^-p] - x[0] are the cyclic convolution over M
+// points of u[q] = x[g^q] with v[s] = exp(SIGN 2 pi i g^-s / P), formed as
+// IDFT_M(DFT_M(u) . DFT_M(v) / M) with the mixed-radix plan Rs of M; x[0]
+// enters as the zero mode of the product, and X[0] = x[0] + DFT_M(u)[0].
+// The table tw (4 M entries, staged in LDS): [0, M) the pass twiddles of
+// the M-point plan, [M, 2M) DFT_M(v) / M for SIGN = -1, [2M, 3M) the same for
+// SIGN = +1, [3M, 4M) (g^q mod P, g^-q mod P) as doubles.  u and the
+// convolution live in elements 1..M of the column.
+template <int SIGN, int P, class Col, int... Rs>
+__device__ __forceinline__ void yf_rader(cplx* T, Col col, const cplx* tw) {
+  constexpr int M = P - 1, QM = (M + 63) / 64;
+  static_assert(YPlanT<Rs...>::N == M, "the convolution plan has P - 1 points");
+  const int lane = threadIdx.x & 63;
+  const ShiftCol<Col> sc{col, 1};
+  const cplx* perm = tw + 3 * M;
+  wf_sync();   // the column was just written lane by lane
+  const cplx x0 = T[col(0)];
+  cplx u[QM];
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    const int q = lane + 64 * t;
+    if (q < M) u[t] = T[col((int)perm[q].x)];
+  }
+  wf_sync();
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    const int q = lane + 64 * t;
+    if (q < M) T[sc(q)] = u[t];
+  }
+  yf_fft<-1, ShiftCol<Col>, Rs...>(T, sc, tw);
+  const cplx* V = tw + (SIGN < 0 ? M : 2 * M);
+  cplx X0 = x0;
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    const int m = lane + 64 * t;
+    if (m < M) {
+      const cplx U = T[sc(m)];
+      cplx W = cmul(U, V[m]);
+      if (m == 0) {
+        X0 = cadd(x0, U);
+        W = cadd(W, x0);
+      }
+      T[sc(m)] = W;
+    }
+  }
+  yf_fft<1, ShiftCol<Col>, Rs...>(T, sc, tw);
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    const int p = lane + 64 * t;
+    if (p < M) u[t] = T[sc(p)];
+  }
+  wf_sync();
+#pragma unroll
+  for (int t = 0; t < QM; ++t) {
+    const int p = lane + 64 * t;
+    if (p < M) T[col((int)perm[p].y)] = u[t];
+  }
+  if (lane == 0) T[col(0)] = X0;
+  wf_sync();
+}
+
 // Plans of the y-line transforms, the template argument of the y kernels.
 //   YFFT<Rs...>  mixed-radix Stockham passes, N = prod Rs at compile time,
 //                twiddles from the pass-major table;
+//   YRader<P, G, Rs...>  prime N = P by Rader's algorithm over the plan Rs
+//                of P - 1 (G a primitive root mod P), its table of 4 (P - 1)
+//                entries in place of the twiddles;
 //   YDirect<QM>  the direct DFT, N = my <= 64 QM at run time, twiddles from
 //                the root table exp(-2 pi i t/N).
+// tw_len(N): entries of the plan's table that a kernel stages in LDS.
 template <int... Rs> struct YFFT {
   static constexpr int Q = (YPlanT<Rs...>::N + 63) / 64;
   static constexpr bool direct = false;
   static constexpr int NC = YPlanT<Rs...>::N;   // line length (compile time)
+  __host__ __device__ static constexpr int tw_len(int n) { return n; }
   // active = false: this wave's column stays as it is (the call keeps the
   // waves of a block in step for the block-collective plan)
   template <int SIGN, class Col>
@@ -342,10 +417,26 @@ template <int... Rs> struct YFFT {
     fft<SIGN>(T, c, tw, n);
   }
 };
+template <int P, int G, int... Rs> struct YRader {
+  static constexpr int Q = (P + 63) / 64;
+  static constexpr bool direct = false;
+  static constexpr int NC = P;
+  __host__ __device__ static constexpr int tw_len(int) { return 4 * (P - 1); }
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* tw, int,
+                                             bool active = true) {
+    if (active) yf_rader<SIGN, P, Col, Rs...>(T, c, tw);
+  }
+  template <int SIGN, class Col>
+  __device__ __forceinline__ static void fft_wave(cplx* T, Col c, const cplx* tw, int n) {
+    fft<SIGN>(T, c, tw, n);
+  }
+};
 template <int QM> struct YDirect {
   static constexpr int Q = QM;
   static constexpr bool direct = true;
   static constexpr int NC = 0;                  // line length: my, at run time
+  __host__ __device__ static constexpr int tw_len(int n) { return n; }
   template <int SIGN, class Col>
   __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N,
                                              bool active = true) {
@@ -362,6 +453,7 @@ template <int QM> struct YMfma {
   static constexpr int Q = QM;
   static constexpr bool direct = true;
   static constexpr int NC = 0;
+  __host__ __device__ static constexpr int tw_len(int n) { return n; }
   template <int SIGN, class Col>
   __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N,
                                              bool active = true) {

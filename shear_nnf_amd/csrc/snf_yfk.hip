@@ -240,10 +240,11 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   const int nv = min(YF_LINES, g.mz - YF_LINES * ch);   // lines of this block
   cplx pv[Q];
   if (blockIdx.x != nbl) yf_issue(pv, a.vor, P, L0, nv, N);
-  // LDS: tile [N][YF_LINES], one column per wave, the N-th roots, the
-  // multiplier rows of this block's kx at the old and new twist
+  // LDS: tile [N][YF_LINES], one column per wave, the plan's table (the
+  // N-th roots or twiddles), the multiplier rows of this block's kx at the
+  // old and new twist
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
+  for (int t = threadIdx.x; t < PL::tw_len(N); t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
   const int ir = rk - 1;
   double dt, f1, f2;
@@ -318,7 +319,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [2][m]
+  cplx* syo = rt + PL::tw_len(N);          // d/dy, d2/dy2 at the old twist: [2][m]
   cplx* syn = syo + 2 * N;     // ... at the new twist
   for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
     const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
@@ -610,7 +611,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   cplx pv[Q];
   if (blockIdx.x != nbl) yf_issue(pv, a.vor, P, L0, nv, N);
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
+  for (int t = threadIdx.x; t < PL::tw_len(N); t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
   const int ir = rk - 1;
   double dt, f1, f2;
@@ -694,7 +695,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  cplx* syo = rt + N;          // d/dy, d2/dy2 at the old twist: [2][m]
+  cplx* syo = rt + PL::tw_len(N);          // d/dy, d2/dy2 at the old twist: [2][m]
   cplx* syn = syo + 2 * N;     // ... at the new twist
   for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
     const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
@@ -1013,7 +1014,7 @@ k_yf_first(YArgs a) {
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
+  for (int t = threadIdx.x; t < PL::tw_len(N); t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
   const double invN = 1.0 / (double)N;
   const TileCol tc{w};
@@ -1056,7 +1057,7 @@ k_yf_first(YArgs a) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i);
   const double tho = atan2(so.y, so.x) * invN;
-  cplx* sy = rt + N;           // d/dy, d2/dy2 at the current twist: [2][m]
+  cplx* sy = rt + PL::tw_len(N);           // d/dy, d2/dy2 at the current twist: [2][m]
   for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
     const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
     sy[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
@@ -1170,7 +1171,7 @@ k_yf_filter(YArgs a) {
   const int nch = (g.mz + YF_LINES - 1) / YF_LINES;
   const long nbl = (long)g.kxn * nch;
   cplx* rt = T + 2 * YF_LINES * N;
-  for (int t = threadIdx.x; t < N; t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
+  for (int t = threadIdx.x; t < PL::tw_len(N); t += blockDim.x) rt[t] = gld((PL::direct ? a.twy : a.tpy) + t);
   __syncthreads();
   const double invN = 1.0 / (double)N;
   const TileCol tc{w};

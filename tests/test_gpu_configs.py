@@ -5,10 +5,10 @@
   ``tests/golden/config2_192x193x192.npz`` (made by
   ``tests/golden/make_config2.py``): the relative L2 error per field, estimated
   through 64 seeded complex Gaussian projections, <= 1e-12; U, W exactly as
-  fields <= 1e-12; every UNER record to rtol 1e-11.  my = 193 has no radix
-  plan, so this is the banded twisted-cyclic y stage.
-* 32x193x32, explicit and implicit, 10 steps against the live oracle (the
-  banded y stage at my = 193 in both viscous modes).
+  fields <= 1e-12; every UNER record to rtol 1e-11.  my = 193 is prime: the
+  spectral y stage transforms its lines by Rader's algorithm.
+* 32x193x32, explicit and implicit, 10 steps against the live oracle, on
+  the Rader y stage and on the banded twisted-cyclic sweeps.
 * 48x49x48, Re_z = 1000, 100 steps against the live oracle.
 * 512x513x512, Re_z = 8000 (too large for the oracle): the wave x-z kernels
   against the Stockham ones over 2 steps (<= 1e-12), the device energy
@@ -66,10 +66,12 @@ def test_config2_192x193x192_10_steps():
     np.testing.assert_allclose(np.array(box.history), ref["history"], rtol=1e-11, atol=1e-14)
 
 
+@pytest.mark.parametrize("path", ["spectral", "banded"])
 @pytest.mark.parametrize("explicit", [1, 0])
-def test_banded_y_my193_10_steps(explicit):
+def test_y_my193_10_steps(explicit, path, monkeypatch):
     grid, Re = (32, 193, 32), 4000.0
     ic = box_ic(*grid)
+    monkeypatch.setenv("SNF_Y_PATH", path)
     box = _box(grid, Re, explicit=explicit, nhist=1)
     box.set_state(*ic)
     st = box.cross(10)

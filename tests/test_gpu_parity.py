@@ -285,7 +285,7 @@ def test_spectra_and_reset(lib):
 
 # line lengths without a radix plan (my <= 144) take the direct-DFT spectral
 # y stage: against the oracle's compact solves and against the banded sweeps
-@pytest.mark.parametrize("grid", [(32, 32, 32), (48, 40, 48), (64, 65, 64), (32, 97, 32),
+@pytest.mark.parametrize("grid", [(32, 32, 32), (48, 40, 48), (64, 65, 64), (32, 101, 32),
                                   (32, 129, 32), (32, 144, 32), (32, 145, 32)])
 def test_cross_direct_dft_y_3_steps(lib, grid):
     st, (vor, phi, u00, w00), box, c = _run_pair(1, 3, grid=grid)
@@ -368,8 +368,8 @@ def test_radix_and_banded_y_paths_agree(lib, monkeypatch, grid):
 # the direct DFT of line lengths without a radix plan on the matrix cores
 # (v_mfma_f64_16x16x4f64, the block's four lines transformed together):
 # same bar against the oracle as the per-wave sums it can replace
-@pytest.mark.parametrize("grid,explicit,ydealias", [((48, 40, 48), 1, 0), ((32, 97, 32), 0, 0),
-                                                    ((64, 97, 64), 1, 0), ((32, 129, 32), 1, 3)])
+@pytest.mark.parametrize("grid,explicit,ydealias", [((48, 40, 48), 1, 0), ((32, 101, 32), 0, 0),
+                                                    ((64, 99, 64), 1, 0), ((32, 129, 32), 1, 3)])
 def test_cross_mfma_dft_y_3_steps(lib, monkeypatch, grid, explicit, ydealias):
     monkeypatch.setenv("SNF_Y_DFT", "mfma")
     st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 3, grid=grid, ydealias=ydealias)
@@ -378,3 +378,36 @@ def test_cross_mfma_dft_y_3_steps(lib, monkeypatch, grid, explicit, ydealias):
     assert rel_l2(u00, c.u00) < STEP_TOL
     assert rel_l2(w00, c.w00) < STEP_TOL
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+# prime line lengths by Rader's algorithm (a cyclic convolution over the
+# radix plan of my - 1: 97 over 8*4*3, 193 over 8*8*3, 257 over 8*8*4),
+# explicit, implicit and with the compact filter, against the oracle
+@pytest.mark.parametrize("grid,explicit,ydealias", [((32, 97, 32), 1, 0), ((32, 97, 32), 0, 0),
+                                                    ((48, 97, 48), 1, 3), ((32, 193, 32), 1, 0),
+                                                    ((32, 193, 32), 0, 0), ((32, 257, 32), 1, 0)])
+def test_cross_rader_y_plans_3_steps(lib, grid, explicit, ydealias):
+    st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 3, grid=grid, ydealias=ydealias)
+    assert rel_l2(vor, c.vor) < STEP_TOL
+    assert rel_l2(phi, c.phi) < STEP_TOL
+    assert rel_l2(u00, c.u00) < STEP_TOL
+    assert rel_l2(w00, c.w00) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("grid,precision", [((64, 193, 64), "fp64"), ((96, 97, 96), "fp64"),
+                                            ((96, 97, 96), "mixed")])
+def test_rader_and_banded_y_paths_agree(lib, monkeypatch, grid, precision):
+    """A Rader plan against the banded sweeps over 5 steps (FP64: to the
+    parity bar; mixed: the FP32 inter-stage fields of the spectral stage
+    against the banded stage's FP64 ones, to the mixed mode's bound)."""
+    vor, phi, u00, w00 = box_ic(*grid)
+    outs = []
+    for path in ("spectral", "banded"):
+        monkeypatch.setenv("SNF_Y_PATH", path)
+        box = _box(*grid, explicit=1, precision=precision)
+        box.set_state(vor, phi, u00, w00)
+        box.cross(5)
+        outs.append(box.get_state())
+    for a, b in zip(*outs):
+        assert rel_l2(a, b) < (STEP_TOL if precision == "fp64" else 1e-5)
