@@ -397,7 +397,8 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
       }
     }
   }
-  yf_flush_blk(T, vwk_blk, N);
+  // the third substep's accumulators are never read (a step starts afresh)
+  if (rk != 3) yf_flush_blk(T, vwk_blk, N);
   {
     YPh<true> ph(phn, lane);
 #pragma unroll
@@ -506,7 +507,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
       }
     }
   }
-  yf_flush_blk(T, pwk_blk, N);
+  if (rk != 3) yf_flush_blk(T, pwk_blk, N);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
