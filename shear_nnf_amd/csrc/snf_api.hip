@@ -690,7 +690,7 @@ unsigned ylines_blocks(const snf_handle* h) { return (unsigned)((h->P + 2 + 127)
 // its LDS and per-thread plans
 bool xz_b_fits(const snf_handle* h, int nt, int zb) {
   const int mgalx = h->g.mgalx, mgalz = h->g.mgalz;
-  return zb >= 2 && mgalz % zb == 0 && 3 * (size_t)zb * mgalx * sizeof(cplx) <= 156 * 1024 &&
+  return zb >= 2 && mgalz % zb == 0 && xz_lds_B(zb, mgalx, sizeof(cplx)) <= 156 * 1024 &&
          (zb / 2) * mgalx <= XZ_B_MAXW * nt && fft_max_values(h->px, 3 * zb, nt) <= XZ_MAXV;
 }
 
@@ -922,7 +922,7 @@ int set_b_shape(const std::vector<snf_handle*>& hs, int nt, int zb) {
     if (h->bthreads == nt && h->zb == zb) continue;
     h->bthreads = nt;
     h->zb = zb;
-    h->ldsB = 3 * (size_t)zb * h->g.mgalx * (h->mixed ? sizeof(cplxf) : sizeof(cplx));
+    h->ldsB = xz_lds_B(zb, h->g.mgalx, h->mixed ? sizeof(cplxf) : sizeof(cplx));
     for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
                            (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
                            (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>})
@@ -1311,7 +1311,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
   // ping-pong buffers of 3 sequences per row, xz_rows_per_block rows, and
   // the FP64 z roots
   h->ldsA = 2 * 3 * (size_t)xz_rows_per_block(mgalz) * mgalz * es + (size_t)mgalz * sizeof(cplx);
-  h->ldsB = 3 * (size_t)h->zb * mgalx * es;
+  h->ldsB = xz_lds_B(h->zb, mgalx, es);
   h->ldsC = h->ldsA;
   lds_attr((const void*)k_ydealias, 3 * (size_t)my * sizeof(cplx), h->dev);
   for (const void* k : {(const void*)k_xz_A<cplx>, (const void*)k_xz_A<cplxf>})

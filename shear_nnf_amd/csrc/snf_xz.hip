@@ -171,6 +171,12 @@ __device__ __forceinline__ unsigned long long dbits(double v) {
   return (unsigned long long)__double_as_longlong(v);
 }
 
+// LDS of the Stockham x-row kernel: 3 ZB sequences of mgalx elements of es
+// bytes, then the mgalx FP64 x roots
+inline size_t xz_lds_B(int zb, int mgalx, size_t es) {
+  return 3 * (size_t)zb * mgalx * es + (size_t)mgalx * sizeof(cplx);
+}
+
 // NT threads per block (256, 512 or 1024), ZB z-rows per block
 template <int NT, class H>
 __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H* __restrict__ hout, FFTPlan px,
@@ -188,6 +194,8 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   const long fstride = (long)g.my * kxn * Nz;
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
   const int nseq = 3 * ZB;    // sequence s = p*ZB + r (pair p, row r)
+  cplx* tw = reinterpret_cast<cplx*>(lds + nseq * Mx);   // the x roots, staged once
+  for (int t = threadIdx.x; t < Mx; t += blockDim.x) tw[t] = gld(twx + t);
   for (int t = threadIdx.x; t < nseq * Mx; t += blockDim.x) lds[t] = wf_mk((R)0, (R)0);
   __syncthreads();
   // Z_pair = G_a + i G_b with the Hermitian extension in kx
@@ -207,7 +215,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
     }
   }
   __syncthreads();
-  fft_lds_ip<XZ_MAXV>(lds, nseq, px, twx, +1);
+  fft_lds_ip<XZ_MAXV>(lds, nseq, px, tw, +1);
   // u x omega: each (row pair, x) handled by one thread; results go to
   // sequences 0..ZB-1 (H1 + i H2 per row) and ZB..3ZB/2-1 (H3 of two rows)
   // products and maxima in the element type, the 'p' sums in FP64
@@ -302,7 +310,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
     }
   }
   __syncthreads();
-  fft_lds_ip<XZ_MAXV>(lds, ZB + npair, px, twx, -1);
+  fft_lds_ip<XZ_MAXV>(lds, ZB + npair, px, tw, -1);
   // unpack the pairs, z fastest among consecutive threads
   for (int t = threadIdx.x; t < kxn * ZB; t += blockDim.x) {
     const int r = t % ZB, k = t / ZB;
