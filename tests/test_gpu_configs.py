@@ -9,6 +9,8 @@
   spectral y stage transforms its lines by Rader's algorithm.
 * 32x193x32, explicit and implicit, 10 steps against the live oracle, on
   the Rader y stage and on the banded twisted-cyclic sweeps.
+* 48x513x48, explicit and implicit, 5 steps against the live oracle (the
+  banded y stage at the 512^3 grid's line length).
 * 48x49x48, Re_z = 1000, 100 steps against the live oracle.
 * 512x513x512, Re_z = 8000 (too large for the oracle): the wave x-z kernels
   against the Stockham ones over 2 steps (<= 1e-12), the device energy
@@ -79,6 +81,25 @@ def test_y_my193_10_steps(explicit, path, monkeypatch):
     c.set_state(*ic)
     c.run(10)
     assert st.steps_done == 10
+    for a, b in zip(box.get_state(), (c.vor, c.phi, c.u00, c.w00)):
+        assert rel_l2(a, b) < STEP_TOL
+    np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("explicit", [1, 0])
+def test_y_my513_5_steps(explicit):
+    """my = 513 (the 512x513x512 grid's line length, 3^3 * 19: no spectral
+    plan, the banded twisted-cyclic sweeps) on a 48 x 48 x-z plane, 5 steps
+    against the oracle (about 17 s of oracle time per case)."""
+    grid, Re = (48, 513, 48), 8000.0
+    ic = box_ic(*grid)
+    box = _box(grid, Re, explicit=explicit, nhist=1)
+    box.set_state(*ic)
+    st = box.cross(5)
+    c = _oracle(grid, Re, explicit=explicit, nhist=1)
+    c.set_state(*ic)
+    c.run(5)
+    assert st.steps_done == 5
     for a, b in zip(box.get_state(), (c.vor, c.phi, c.u00, c.w00)):
         assert rel_l2(a, b) < STEP_TOL
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
