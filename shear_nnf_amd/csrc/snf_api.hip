@@ -22,7 +22,7 @@
 #include "snf_xzw.hip"
 #include "snf_scalar.hip"
 #include "snf_ykern.hip"
-#include "snf_yfk.hip"
+#include "snf_yplan.h"
 #include "snf_ydeal.hip"
 #include "snf_batch.hip"
 
@@ -35,6 +35,19 @@
 #define SNF_ENERGY_BLOCKS 1024
 #define SNF_SCRATCH_DOUBLES (2 * SNF_ENERGY_BLOCKS + 4)
 static __device__ __align__(256) double scratch[SNF_SCRATCH_DOUBLES];
+
+// hipFuncAttributeMaxDynamicSharedMemorySize is a per-kernel setting of the
+// process, shared by every handle on a device: raise it to the largest LDS
+// any live handle launches the kernel with, never lower it.
+void lds_attr(const void* k, size_t bytes, int dev) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> set;
+  std::lock_guard<std::mutex> lk(mu);
+  size_t& cur = set[std::make_pair(k, dev)];
+  if (bytes <= cur) return;
+  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
+    cur = bytes;
+}
 
 namespace {
 
@@ -66,18 +79,6 @@ struct DevBuf {
   operator T*() const { return p; }
 };
 
-// hipFuncAttributeMaxDynamicSharedMemorySize is a per-kernel setting of the
-// process, shared by every handle on a device: raise it to the largest LDS
-// any live handle launches the kernel with, never lower it.
-void lds_attr(const void* k, size_t bytes, int dev) {
-  static std::mutex mu;
-  static std::map<std::pair<const void*, int>, size_t> set;
-  std::lock_guard<std::mutex> lk(mu);
-  size_t& cur = set[std::make_pair(k, dev)];
-  if (bytes <= cur) return;
-  if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess)
-    cur = bytes;
-}
 
 // The device address of `scratch` on each device (each GPU holds its own
 // copy at its own address), looked up once with that device current, and a
@@ -187,7 +188,10 @@ struct snf_handle {
   cplx* twy = nullptr;
   cplx* tpy = nullptr;                  // pass-major twiddles of the y plan
   cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
+  const YPlanOps* yops = nullptr;       // the spectral y plan of my
+  const YPlanOps* yfops = nullptr;      // the direct-DFT filter (no spectral plan)
   size_t ldsY = 0;
+  size_t ldsYF = 0;
   double cflx = 0, cfly = 0, cflz = 0, cflvv = 0;
   // kernel timing
   bool timing = false;
@@ -457,50 +461,55 @@ ScalarArgs sargs(snf_handle* h) {
 
 // ---- y-transform plans
 // The radix plan of each line length with one (the template arguments of the
-// spectral y kernels, pass order), and the direct DFT for other line
-// lengths up to Y_DIRECT_MAX.  with_yplan calls f with a tag of the plan of
-// my; y_plan_radices gives the same radices to the host twiddle table.
-template <class PL> struct PlanTag { typedef PL type; };
+// spectral y kernels, pass order, in snf_yplans_*.hip), a Rader plan for the
+// primes 97, 193 and 257, and the direct DFT for other line lengths up to
+// Y_DIRECT_MAX.  yplan_ops finds the plan's entry points; y_plan_radices and
+// y_rader_plan give the same radices to the host tables.
 #define Y_DIRECT_MAX 144
 #define Y_DFT_MFMA_DEFAULT false
-template <class F>
-void with_yplan(int my, bool mfma, F&& f) {
+}  // namespace
+const YPlanOps *yplan_r32(), *yplan_r33(), *yplan_r48(), *yplan_r49(), *yplan_r64(),
+    *yplan_r65(), *yplan_r96(), *yplan_r128(), *yplan_r192(), *yplan_r256(), *yplan_r385(),
+    *yplan_r513(), *yplan_rader97(), *yplan_rader193(), *yplan_rader257(), *yplan_direct1(),
+    *yplan_direct2(), *yplan_direct3(), *yplan_mfma1(), *yplan_mfma2(), *yplan_mfma3(),
+    *yplan_fdirect5(), *yplan_fdirect9(), *yplan_fdirect17();
+const YPlanOps* yplan_ops(int my, bool mfma) {
   switch (my) {
-    case 32: f(PlanTag<YFFT<8, 4>>{}); break;
-    case 33: f(PlanTag<YFFT<11, 3>>{}); break;
-    case 48: f(PlanTag<YFFT<4, 4, 3>>{}); break;
-    case 49: f(PlanTag<YFFT<7, 7>>{}); break;
-    case 64: f(PlanTag<YFFT<8, 8>>{}); break;
-    case 65: f(PlanTag<YFFT<13, 5>>{}); break;
-    case 96: f(PlanTag<YFFT<8, 4, 3>>{}); break;
-    case 128: f(PlanTag<YFFT<8, 4, 4>>{}); break;
-    case 192: f(PlanTag<YFFT<8, 8, 3>>{}); break;
-    case 256: f(PlanTag<YFFT<8, 8, 4>>{}); break;
-    case 385: f(PlanTag<YFFT<11, 7, 5>>{}); break;
-    case 97: f(PlanTag<YRader<97, 5, 8, 4, 3>>{}); break;
-    case 193: f(PlanTag<YRader<193, 5, 8, 8, 3>>{}); break;
-    case 257: f(PlanTag<YRader<257, 3, 8, 8, 4>>{}); break;
-    default:
-      if (mfma) {
-        switch ((my + 63) / 64) {
-          case 1: f(PlanTag<YMfma<1>>{}); break;
-          case 2: f(PlanTag<YMfma<2>>{}); break;
-          default: f(PlanTag<YMfma<3>>{}); break;
-        }
-      } else {
-        switch ((my + 63) / 64) {
-          case 1: f(PlanTag<YDirect<1>>{}); break;
-          case 2: f(PlanTag<YDirect<2>>{}); break;
-          default: f(PlanTag<YDirect<3>>{}); break;
-        }
-      }
+    case 32: return yplan_r32();
+    case 33: return yplan_r33();
+    case 48: return yplan_r48();
+    case 49: return yplan_r49();
+    case 64: return yplan_r64();
+    case 65: return yplan_r65();
+    case 96: return yplan_r96();
+    case 128: return yplan_r128();
+    case 192: return yplan_r192();
+    case 256: return yplan_r256();
+    case 385: return yplan_r385();
+    case 513: return yplan_r513();
+    case 97: return yplan_rader97();
+    case 193: return yplan_rader193();
+    case 257: return yplan_rader257();
+    default: break;
+  }
+  if (my > Y_DIRECT_MAX) return nullptr;
+  switch ((my + 63) / 64) {
+    case 1: return mfma ? yplan_mfma1() : yplan_direct1();
+    case 2: return mfma ? yplan_mfma2() : yplan_direct2();
+    default: return mfma ? yplan_mfma3() : yplan_direct3();
   }
 }
+const YPlanOps* yplan_filter_direct_ops(int my) {
+  if (my <= 320) return yplan_fdirect5();
+  if (my <= 576) return yplan_fdirect9();
+  return yplan_fdirect17();
+}
+namespace {
 int y_plan_radices(int my, int* R) {
   static const int plans[][5] = {{32, 8, 4, 0, 0},   {33, 11, 3, 0, 0}, {48, 4, 4, 3, 0},
                                  {49, 7, 7, 0, 0},   {64, 8, 8, 0, 0},  {65, 13, 5, 0, 0},
                                  {96, 8, 4, 3, 0},   {128, 8, 4, 4, 0}, {192, 8, 8, 3, 0},
-                                 {256, 8, 8, 4, 0},  {385, 11, 7, 5, 0}};
+                                 {256, 8, 8, 4, 0},  {385, 11, 7, 5, 0}, {513, 9, 3, 19, 0}};
   for (const auto& pl : plans)
     if (pl[0] == my) {
       int np = 0;
@@ -512,7 +521,7 @@ int y_plan_radices(int my, int* R) {
     }
   return 0;
 }
-// Prime line lengths with a Rader plan (YRader in with_yplan): the primitive
+// Prime line lengths with a Rader plan (YRader in yplan_ops): the primitive
 // root g and the radices of the P - 1 point plan; 0 if my has none.
 int y_rader_plan(int my, int* g, int* R) {
   static const int plans[][5] = {{97, 5, 8, 4, 3}, {193, 5, 8, 8, 3}, {257, 3, 8, 8, 4}};
@@ -577,44 +586,26 @@ bool y_rader_table(int P, int g, const int* R, int np, std::vector<cplx>& tp) {
   return true;
 }
 
-// the spectral y stage of one substep: explicit or implicit viscous terms
-// IO: the storage of the inter-stage fields (Geom::io32: 0 FP64, 3 FP32)
-template <class PL, int IO>
-void launch_yf_plan_io(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
-  if (h->p.explicit_visc == 1)
-    k_yf_explicit<PL, IO><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
-  else
-    k_yf_implicit<PL, IO><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a, rk);
+// launches of the spectral y stage through the plan's entry points
+YLaunch ylaunch(const snf_handle* h, size_t lds, hipStream_t s) {
+  YLaunch c;
+  c.stream = s;
+  c.lds = lds;
+  c.explicit_visc = h->p.explicit_visc;
+  c.io32 = h->g.io32;
+  return c;
 }
-template <class PL>
-void launch_yf_plan(snf_handle* h, int rk, unsigned nb, const YArgs& a) {
-  if (h->g.io32)
-    launch_yf_plan_io<PL, 3>(h, rk, nb, a);
-  else
-    launch_yf_plan_io<PL, 0>(h, rk, nb, a);
+// y line blocks: one kz chunk of one kx each, and one for U, W
+unsigned yf_blocks(const snf_handle* h) {
+  return (unsigned)((long)h->g.kxn * ((h->g.mz + SNF_CW - 1) / SNF_CW) + 1);
 }
 // start of a cross call: v, dv, dome (and dU/dy, dW/dy) at the current twist
-template <class PL>
-void launch_yf_first_plan(snf_handle* h, unsigned nb, const YArgs& a) {
-  if (h->g.io32)
-    k_yf_first<PL, 3><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
-  else
-    k_yf_first<PL, 0><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
-}
 void launch_yf_first(snf_handle* h) {
-  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
-  const YArgs a = yargs(h);
-  with_yplan(h->g.my, h->ymfma, [&](auto tag) {
-    launch_yf_first_plan<typename decltype(tag)::type>(h, nb, a);
-  });
+  h->yops->first(ylaunch(h, h->ldsY, h->stream), yf_blocks(h), yargs(h));
 }
-
+// the spectral y stage of one substep: explicit or implicit viscous terms
 void launch_yf(snf_handle* h, int rk) {
-  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
-  const YArgs a = yargs(h);
-  with_yplan(h->g.my, h->ymfma, [&](auto tag) {
-    launch_yf_plan<typename decltype(tag)::type>(h, rk, nb, a);
-  });
+  h->yops->substep(ylaunch(h, h->ldsY, h->stream), yf_blocks(h), yargs(h), rk);
 }
 
 // iydealiasing=3: compact filter of vor, phi, U, W at the current twist
@@ -642,31 +633,12 @@ void launch_ydealias(snf_handle* h) {
 
 // Line lengths without a spectral y stage (the banded sweeps run the
 // substeps) filter through the direct DFT of up to 64 Q points, in its own
-// LDS plan: the transfer tile, a column per wave and the roots (9 my complex).
-template <class F>
-void with_filter_direct(int my, F&& f) {
-  if (my <= 320)
-    f(PlanTag<YDirect<5>>{});
-  else if (my <= 576)
-    f(PlanTag<YDirect<9>>{});
-  else
-    f(PlanTag<YDirect<17>>{});
-}
-inline size_t filter_direct_lds(int my) { return (size_t)my * (2 * YF_LINES + 1) * sizeof(cplx); }
-
+// LDS plan (yplan_filter_direct_ops).
 void launch_yfilter(snf_handle* h) {
-  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
-  const YArgs a = yargs(h);
-  if (h->yfft) {
-    with_yplan(h->g.my, h->ymfma, [&](auto tag) {
-      k_yf_filter<typename decltype(tag)::type><<<nb, 64 * YF_LINES, h->ldsY, h->stream>>>(a);
-    });
-  } else {
-    const size_t lds = filter_direct_lds(h->g.my);
-    with_filter_direct(h->g.my, [&](auto tag) {
-      k_yf_filter<typename decltype(tag)::type><<<nb, 64 * YF_LINES, lds, h->stream>>>(a);
-    });
-  }
+  if (h->yfft)
+    h->yops->filter(ylaunch(h, h->ldsY, h->stream), yf_blocks(h), yargs(h));
+  else
+    h->yfops->filter(ylaunch(h, h->ldsYF, h->stream), yf_blocks(h), yargs(h));
 }
 
 // tables at the twist shp[which]: the banded factors' tails (tails = false
@@ -739,28 +711,8 @@ void tab_B(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
   else
     tab_B_nt<1024, H>(h, t, n, rk, s);
 }
-template <class PL, int IO>
-void tab_y_plan_io(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
-  const unsigned nb = (unsigned)((long)h->g.kxn * ((h->g.mz + YF_LINES - 1) / YF_LINES) + 1);
-  if (h->p.explicit_visc == 1) {
-    lds_attr((const void*)k_yf_explicit_tab<PL, IO>, h->ldsY, h->dev);
-    if (n == 0) return;
-    k_yf_explicit_tab<PL, IO><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
-  } else {
-    lds_attr((const void*)k_yf_implicit_tab<PL, IO>, h->ldsY, h->dev);
-    if (n == 0) return;
-    k_yf_implicit_tab<PL, IO><<<dim3(nb, 1, n), 64 * YF_LINES, h->ldsY, s>>>(t, rk);
-  }
-}
-template <class PL>
-void tab_y_plan(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
-  if (h->g.io32)
-    tab_y_plan_io<PL, 3>(h, t, n, rk, s);
-  else
-    tab_y_plan_io<PL, 0>(h, t, n, rk, s);
-}
 void tab_y(const snf_handle* h, const YArgs* t, int n, int rk, hipStream_t s) {
-  with_yplan(h->g.my, h->ymfma, [&](auto tag) { tab_y_plan<typename decltype(tag)::type>(h, t, n, rk, s); });
+  h->yops->table(ylaunch(h, h->ldsY, s), yf_blocks(h), t, n, rk, h->dev);
 }
 
 // One RK3 step (st = 0 or 1 of the pair the tables hold) of every box of a
@@ -1219,6 +1171,15 @@ int snf_create(const snf_params* p, snf_handle** out) {
         return fail(SNF_BAD_PARAM, "y twiddle table larger than my");
       }
       y_pass_twiddles(R, np, tp.data());
+      const int RL = R[np - 1];
+      if (yf_spread_radix(RL)) {
+        // roots exp(-2 pi i t / RL) of a spread last pass, after the N entries
+        tp.resize((size_t)my + RL);
+        for (int t = 0; t < RL; ++t) {
+          const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / RL;
+          tp[(size_t)my + t] = mkc((double)cosl(a), (double)sinl(a));
+        }
+      }
     } else if (const int nr = y_rader_plan(my, &rg, R)) {
       if (!y_rader_table(my, rg, R, nr, tp)) {
         snf_destroy(h);
@@ -1227,24 +1188,21 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
     TRY(hipMemcpy(h->tpy, tp.data(), tp.size() * sizeof(cplx), hipMemcpyHostToDevice) == hipSuccess
             ? SNF_OK : fail(SNF_DEVICE_ERROR, "copy y pass twiddles"));
-    // tile, a column per wave, the plan's table, two multiplier rows, four
-    // phase tables (1472 B at my = 385: 81552 B, two blocks still fit one
-    // CU's 160 KiB)
-    size_t twn = (size_t)my;
-    with_yplan(my, h->ymfma, [&](auto tag) { twn = (size_t)decltype(tag)::type::tw_len(my); });
-    h->ldsY = ((size_t)my * (2 * YF_LINES + 4) + twn + 4 * YPH_N((my + 63) / 64)) * sizeof(cplx);
-    with_yplan(my, h->ymfma, [&](auto tag) {
-      typedef typename decltype(tag)::type PL;
-      for (const void* ky : {(const void*)k_yf_explicit<PL, 0>, (const void*)k_yf_implicit<PL, 0>,
-                             (const void*)k_yf_first<PL, 0>, (const void*)k_yf_explicit<PL, 3>,
-                             (const void*)k_yf_implicit<PL, 3>, (const void*)k_yf_first<PL, 3>,
-                             (const void*)k_yf_filter<PL>})
-        lds_attr(ky, h->ldsY, h->dev);
-    });
-    if (!h->yfft && p->ydealias == 3)
-      with_filter_direct(my, [&](auto tag) {
-        lds_attr((const void*)k_yf_filter<typename decltype(tag)::type>, filter_direct_lds(my), h->dev);
-      });
+    // the plan's entry points and LDS limits
+    h->yops = yplan_ops(my, h->ymfma);
+    if (h->yops) {
+      h->ldsY = h->yops->lds_bytes(my);
+      h->yops->lds_attrs(h->ldsY, h->dev);
+    }
+    if (!h->yfft && p->ydealias == 3) {
+      h->yfops = yplan_filter_direct_ops(my);
+      h->ldsYF = h->yfops->lds_bytes(my);
+      h->yfops->lds_attrs(h->ldsYF, h->dev);
+    }
+    if (h->yfft && !h->yops) {
+      snf_destroy(h);
+      return fail(SNF_BAD_PARAM, "no spectral y plan for my");
+    }
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
   // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning).

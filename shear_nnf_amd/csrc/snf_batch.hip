@@ -137,13 +137,4 @@ __global__ void __launch_bounds__(256) k_ysym_tab(const BxYsym* __restrict__ t) 
   const BxYsym& a = t[blockIdx.z];
   k_ysym_body(a.N, a.c, a.shp, a.rt, a.out, a.sc);
 }
-template <class PL, int IO>
-__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
-k_yf_explicit_tab(const YArgs* __restrict__ t, int rk) {
-  k_yf_explicit_body<PL, IO>(t[blockIdx.z], rk);
-}
-template <class PL, int IO>
-__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
-k_yf_implicit_tab(const YArgs* __restrict__ t, int rk) {
-  k_yf_implicit_body<PL, IO>(t[blockIdx.z], rk);
-}
+

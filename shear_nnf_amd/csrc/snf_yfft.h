@@ -14,8 +14,9 @@
 // (x -> X_m = sum_j x_j exp(-i w_m j)), multiplied, and transformed back.
 //
 // Transforms: one wave per line, mixed-radix Stockham passes (radices 2, 3,
-// 4, 5, 7, 8, 11, 13) in place on the line's column of an LDS tile, every pass a
-// read-all / write-all of the wave's own column (no block barrier).
+// 4, 5, 7, 8, 9, 11, 13, 19) in place on the line's column of an LDS tile,
+// every pass a read-all / write-all of the wave's own column (no block
+// barrier).
 // Twiddles from a pass-major table staged in LDS: pass s > 0 (stride Ns,
 // radix R) holds exp(-2 pi i r k / (Ns R)) at [r - 1][k], r < R, k < Ns, so
 // that the lanes of a pass read consecutive entries (no bank conflicts);
@@ -90,6 +91,15 @@ template <> struct OddRoots<7> {
                                   0.4338837391175581204758, -0.4338837391175581204758,
                                   -0.9749279121818236070181, -0.7818314824680298087084};
 };
+template <> struct OddRoots<9> {
+  static constexpr double c[9] = {1.0, 0.7660444431189780352024, 0.1736481776669303488517, -0.5,
+                                  -0.9396926207859083840541, -0.9396926207859083840541, -0.5,
+                                  0.1736481776669303488517, 0.7660444431189780352024};
+  static constexpr double s[9] = {0.0, 0.6427876096865393263226, 0.9848077530122080593667,
+                                  0.8660254037844386467637, 0.3420201433256687330441,
+                                  -0.3420201433256687330441, -0.8660254037844386467637,
+                                  -0.9848077530122080593667, -0.6427876096865393263226};
+};
 template <> struct OddRoots<11> {
   static constexpr double c[11] = {1.0, 0.8412535328311811688618, 0.4154150130018864255293,
                                    -0.1423148382732851404438, -0.6548607339452850640569,
@@ -116,6 +126,23 @@ template <> struct OddRoots<13> {
                                    -0.2393156642875577671488, -0.6631226582407952023768,
                                    -0.9350162426854148234398, -0.9927088740980539928008,
                                    -0.8229838658936563945796, -0.4647231720437685456560};
+};
+
+template <> struct OddRoots<19> {
+  static constexpr double c[19] = {
+      1.0, 0.9458172417006346790197, 0.7891405093963935992190, 0.5469481581224268747118,
+      0.2454854871407991489223, -0.08257934547233232460034, -0.4016954246529694575168,
+      -0.6772815716257410747622, -0.8794737512064890713909, -0.9863613034027223736025,
+      -0.9863613034027223736025, -0.8794737512064890713909, -0.6772815716257410747622,
+      -0.4016954246529694575168, -0.08257934547233232460034, 0.2454854871407991489223,
+      0.5469481581224268747118, 0.7891405093963935992190, 0.9458172417006346790197};
+  static constexpr double s[19] = {
+      0.0, 0.3246994692046834874076, 0.6142127126896678174443, 0.8371664782625285748061,
+      0.9694002659393304167361, 0.9965844930066698498194, 0.9157733266550574399193,
+      0.7357239106731316247742, 0.4759473930370735444314, 0.1645945902807338941437,
+      -0.1645945902807338941437, -0.4759473930370735444314, -0.7357239106731316247742,
+      -0.9157733266550574399193, -0.9965844930066698498194, -0.9694002659393304167361,
+      -0.8371664782625285748061, -0.6142127126896678174443, -0.3246994692046834874076};
 };
 
 template <int R, int SIGN>
@@ -151,16 +178,18 @@ __host__ __device__ __forceinline__ void yf_dft_odd(cplx* v) {
 
 template <int R, int SIGN>
 __host__ __device__ __forceinline__ void yf_dft(cplx* v) {
-  if (R == 5 || R == 7 || R == 11 || R == 13)
-    yf_dft_odd<(R == 5 ? 5 : (R == 7 ? 7 : (R == 11 ? 11 : 13))), SIGN>(v);
+  if constexpr (R % 2 == 1 && R >= 5)
+    yf_dft_odd<R, SIGN>(v);
   else
-    wf_dft<(R == 2 ? 2 : (R == 3 ? 3 : (R == 4 ? 4 : 8))), SIGN>(v);
+    wf_dft<R, SIGN>(v);
 }
 
 // one Stockham pass on column w of the tile: inputs x[b + r nb], outputs
 // (b/Ns) Ns R + b%Ns + r Ns, twiddle exp(SIGN 2 pi i r (b%Ns) / (Ns R)) read
 // from tw[OFF + (r - 1) Ns + b%Ns]
-template <int R, int SIGN, int N, int Ns, int OFF, class Col>
+// PT >= 0: each output p >= PNs is multiplied by tw[PT + p - PNs], the
+// twiddle of the next pass (a spread pass, whose nb is PNs) at its input p.
+template <int R, int SIGN, int N, int Ns, int OFF, int PT = -1, int PNs = 0, class Col>
 __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
   constexpr int nb = N / R;
   constexpr int MB = (nb + 63) / 64;       // butterflies per lane
@@ -188,19 +217,112 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
       yf_dft<R, SIGN>(v[q]);
       const int o = (b / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) T[col(o + r * Ns)] = v[q][r];
+      for (int r = 0; r < R; ++r) {
+        const int p = o + r * Ns;
+        cplx y = v[q][r];
+        if (PT >= 0 && p >= PNs) y = wf_tw<SIGN>(y, tw[PT + p - PNs]);
+        T[col(p)] = y;
+      }
     }
   }
   wf_sync();
 }
+
+// The last pass of a plan whose last radix R is a large odd prime (17 and
+// up), with the outputs spread over the lanes.  A butterfly takes H + 1 =
+// (R + 1) / 2 lanes: lane k = 0 forms y_0, lane k = 1..H forms the pair
+// y_k, y_(R-k) from the symmetric sums a_j = x_j + x_(R-j), b_j = x_j -
+// x_(R-j) (each input one LDS broadcast to the lanes of its butterfly), so
+// that 64 / (H + 1) butterflies run at once (60 of 64 lanes at R = 19,
+// against 27 butterflies on 27 lanes for a butterfly per lane), and a lane
+// holds its H multipliers, not the R inputs and 2H sums of a whole
+// butterfly.  The multipliers exp(-2 pi i t / R), t < R, sit at tw[N + t].
+// In the last Stockham pass (Ns = nb) butterfly b reads and writes the same
+// slots b + r nb, so each round writes as soon as its butterflies are read.
+// The pass's twiddles are applied by the previous pass as it writes
+// (yf_pass with PT >= 0): input b + r nb takes tw[OFF + (r - 1) nb + b] =
+// tw[OFF + (b + r nb) - nb].
+template <int R, int SIGN, int N, class Col>
+__device__ __forceinline__ void yf_pass_spread(cplx* T, Col col, const cplx* tw) {
+  constexpr int H = (R - 1) / 2, L = H + 1;
+  constexpr int G = 64 / L;                  // butterflies per round
+  constexpr int nb = N / R;
+  constexpr int NR = (nb + G - 1) / G;       // rounds
+  const int lane = threadIdx.x & 63;
+  const int g = lane / L, k = lane - g * L;
+#ifdef YSPREAD_REGS
+  double cs[H], sn[H];
+#pragma unroll
+  for (int j = 1; j <= H; ++j) {
+    const cplx r = tw[N + (j * k) % R];     // exp(-2 pi i j k / R)
+    cs[j - 1] = r.x;
+    sn[j - 1] = -r.y;
+  }
+#endif
+#pragma unroll
+  for (int t = 0; t < NR; ++t) {
+    const int b = g + G * t;
+    const bool act = g < G && b < nb;
+    cplx ylo = mkc(0.0, 0.0), yhi = mkc(0.0, 0.0);
+    if (act) {
+      const cplx x0 = T[col(b)];
+      cplx re = x0, im = mkc(0.0, 0.0), sum = x0;
+#pragma unroll
+      for (int j = 1; j <= H; ++j) {
+        const cplx xp = T[col(b + j * nb)], xm = T[col(b + (R - j) * nb)];
+        const cplx aj = wf_add(xp, xm), bj = wf_sub(xp, xm);
+        sum = wf_add(sum, aj);
+#ifdef YSPREAD_REGS
+        const double c = cs[j - 1], sj = sn[j - 1];
+#else
+        const cplx rr = tw[N + (j * k) % R];
+        const double c = rr.x, sj = -rr.y;
+#endif
+        re.x += c * aj.x;
+        re.y += c * aj.y;
+        im.x += sj * bj.x;
+        im.y += sj * bj.y;
+      }
+      const cplx si = wf_si<SIGN>(im);      // SIGN * i * im
+      ylo = k == 0 ? sum : wf_add(re, si);
+      yhi = wf_sub(re, si);
+    }
+    wf_sync();   // the butterflies of this round are read
+    if (act) {
+      T[col(b + k * nb)] = ylo;
+      if (k > 0) T[col(b + (R - k) * nb)] = yhi;
+    }
+  }
+  wf_sync();
+}
+// radix of a last pass that runs spread over the lanes
+__host__ __device__ constexpr bool yf_spread_radix(int R) { return R >= 17 && R % 2 == 1; }
+template <int... Rs> struct YLast;
+template <> struct YLast<> {
+  static constexpr int value = 0;
+};
+template <int R> struct YLast<R> {
+  static constexpr int value = R;
+};
+template <int R, int R2, int... Rs> struct YLast<R, R2, Rs...> {
+  static constexpr int value = YLast<R2, Rs...>::value;
+};
 
 // transform of column w in place (natural order in and out), unnormalised
 template <int SIGN, int N, int Ns, int OFF, class Col>
 __device__ __forceinline__ void yf_passes(cplx*, Col, const cplx*) {}
 template <int SIGN, int N, int Ns, int OFF, class Col, int R, int... Rs>
 __device__ __forceinline__ void yf_passes(cplx* T, Col col, const cplx* tw) {
-  yf_pass<R, SIGN, N, Ns, OFF>(T, col, tw);
-  yf_passes<SIGN, N, Ns * R, (Ns > 1 ? OFF + (R - 1) * Ns : 0), Col, Rs...>(T, col, tw);
+  constexpr int NOFF = Ns > 1 ? OFF + (R - 1) * Ns : 0;   // the next pass's twiddles
+  if constexpr (yf_spread_radix(R)) {
+    static_assert(sizeof...(Rs) == 0, "a spread pass runs last");
+    yf_pass_spread<R, SIGN, N>(T, col, tw);
+  } else if constexpr (sizeof...(Rs) == 1 && yf_spread_radix(YLast<Rs...>::value)) {
+    yf_pass<R, SIGN, N, Ns, OFF, NOFF, Ns * R>(T, col, tw);   // twiddles the spread pass's inputs
+  } else {
+    yf_pass<R, SIGN, N, Ns, OFF>(T, col, tw);
+  }
+  yf_passes<SIGN, N, Ns * R, NOFF, Col, Rs...>(T, col, tw);
 }
 template <int SIGN, class Col, int... Rs>
 __device__ __forceinline__ void yf_fft(cplx* T, Col col, const cplx* tw) {
@@ -404,7 +526,13 @@ template <int... Rs> struct YFFT {
   static constexpr int Q = (YPlanT<Rs...>::N + 63) / 64;
   static constexpr bool direct = false;
   static constexpr int NC = YPlanT<Rs...>::N;   // line length (compile time)
-  __host__ __device__ static constexpr int tw_len(int n) { return n; }
+  // multiplier rows read from the global table: beyond 448 points the rows
+  // staged in LDS would leave one block per CU
+  static constexpr bool sym_gl = NC > 448;
+  // the pass twiddles (at most N entries), then the R-th roots of a spread
+  // last pass of radix R
+  static constexpr int RS = yf_spread_radix(YLast<Rs...>::value) ? YLast<Rs...>::value : 0;
+  __host__ __device__ static constexpr int tw_len(int n) { return n + RS; }
   // active = false: this wave's column stays as it is (the call keeps the
   // waves of a block in step for the block-collective plan)
   template <int SIGN, class Col>
@@ -421,6 +549,7 @@ template <int P, int G, int... Rs> struct YRader {
   static constexpr int Q = (P + 63) / 64;
   static constexpr bool direct = false;
   static constexpr int NC = P;
+  static constexpr bool sym_gl = false;
   __host__ __device__ static constexpr int tw_len(int) { return 4 * (P - 1); }
   template <int SIGN, class Col>
   __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* tw, int,
@@ -436,6 +565,7 @@ template <int QM> struct YDirect {
   static constexpr int Q = QM;
   static constexpr bool direct = true;
   static constexpr int NC = 0;                  // line length: my, at run time
+  static constexpr bool sym_gl = false;
   __host__ __device__ static constexpr int tw_len(int n) { return n; }
   template <int SIGN, class Col>
   __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N,
@@ -453,6 +583,7 @@ template <int QM> struct YMfma {
   static constexpr int Q = QM;
   static constexpr bool direct = true;
   static constexpr int NC = 0;
+  static constexpr bool sym_gl = false;
   __host__ __device__ static constexpr int tw_len(int n) { return n; }
   template <int SIGN, class Col>
   __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* rt, int N,

@@ -18,6 +18,8 @@
 // Fields read: vor rhg vorwk rdome phi rhv phiwk; written: vor vorwk dome phi
 // phiwk v dv -- each once.  The real x-z means U, W (00 lines, m=1 calls)
 // take one extra block with one wave each.
+#pragma once
+#include "snf_yargs.h"
 #include "snf_yfft.h"
 
 #define YS 3   // multipliers per mode in the tables: d/dy, d2/dy2, filter
@@ -66,6 +68,40 @@ __device__ __forceinline__ cplx yph_entry(int t, int Q, const double* a, const d
                    : (r < 16 ? a[b] * (double)(8 * (r - 8)) : a[b] * (double)(64 * (r - 16)) + a0[b]);
   return yf_expi(ang);
 }
+
+// The multiplier rows d/dy, d2/dy2 of a block's kx at one twist.  Staged in
+// LDS as [2][m], or, for plans whose LDS would then admit one block per CU
+// (PL::sym_gl), read from the global table [kx][m][YS], where the blocks of
+// one kx (kept on one XCD) find them in L2.
+template <bool GL> struct YSymRow;
+template <> struct YSymRow<false> {
+  const cplx* p;
+  int N;
+  __device__ __forceinline__ cplx d1(int m) const { return p[m]; }
+  __device__ __forceinline__ cplx d2(int m) const { return p[N + m]; }
+};
+template <> struct YSymRow<true> {
+  const cplx* p;
+  int N;
+  __device__ __forceinline__ cplx d1(int m) const { return gld(p + YS * m); }
+  __device__ __forceinline__ cplx d2(int m) const { return gld(p + YS * m + 1); }
+};
+// row of kx i of the table tab: staged into the LDS slot dst (2 N entries,
+// block-collective; read after the next block barrier) or pointed at in place
+template <class PL>
+__device__ __forceinline__ YSymRow<PL::sym_gl> ys_row(cplx* dst, const cplx* tab, int i, int N) {
+  if constexpr (PL::sym_gl) {
+    return YSymRow<true>{tab + (long)i * N * YS, N};
+  } else {
+    for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
+      const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
+      dst[(t & 1) * N + (t >> 1)] = gld(tab + src);
+    }
+    return YSymRow<false>{dst, N};
+  }
+}
+// LDS entries of one staged row
+template <class PL> __host__ __device__ constexpr int ys_lds(int N) { return PL::sym_gl ? 0 : 2 * N; }
 
 // block-collective transfers of field F (y stride P) for the nv lines from L0:
 // element (j, c) at F + j P + L0 + c.  In the kz-chunk layout P = SNF_CW and
@@ -190,6 +226,67 @@ __device__ __forceinline__ void yf_flush_blk(const cplx* T, cplx* W, int N) {
   __syncthreads();
 }
 
+__device__ __forceinline__ void yf_fill_blk(cplx* T, const cplx* W, int N) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    T[TileCol{c}(j)] = gld(W + e);
+  }
+  __syncthreads();
+}
+
+// A second field transfer split into issue and commit.  LIVE: as yf_issue /
+// yf_commit, the loads in flight in registers in between.  Otherwise the
+// issue only notes the source and the commit loads it (yf_fill): for plans
+// of 9 rows per lane, whose registers do not hold a second field in flight
+// beside the transforms.
+template <int Q, bool LIVE> struct YPf;
+template <int Q> struct YPf<Q, true> {
+  cplx r[Q];
+  __device__ __forceinline__ void issue(const cplx* F, long P, long L0, int nv, int N) {
+    yf_issue(r, F, P, L0, nv, N);
+  }
+  __device__ __forceinline__ void issue_io(const cplx* F, long P, long L0, int nv, int N, int io32) {
+    yf_issue_io(r, F, P, L0, nv, N, io32);
+  }
+  __device__ __forceinline__ void issue_blk(const cplx* W, int N) { yf_issue_blk(r, W, N); }
+  __device__ __forceinline__ void commit(cplx* T, int N) { yf_commit(T, r, N); }
+  __device__ __forceinline__ void commit_io(cplx* T, int N, int io32) { yf_commit_io(T, r, N, io32); }
+};
+template <int Q> struct YPf<Q, false> {
+  const cplx* F;
+  long P, L0;
+  int nv;
+  bool blk;
+  __device__ __forceinline__ void issue(const cplx* F_, long P_, long L0_, int nv_, int) {
+    F = F_;
+    P = P_;
+    L0 = L0_;
+    nv = nv_;
+    blk = false;
+  }
+  __device__ __forceinline__ void issue_io(const cplx* F_, long P_, long L0_, int nv_, int N, int) {
+    issue(F_, P_, L0_, nv_, N);
+  }
+  __device__ __forceinline__ void issue_blk(const cplx* W, int) {
+    F = W;
+    blk = true;
+  }
+  __device__ __forceinline__ void commit(cplx* T, int N) {
+    if (blk)
+      yf_fill_blk(T, F, N);
+    else
+      yf_fill(T, F, P, L0, nv, N);
+  }
+  __device__ __forceinline__ void commit_io(cplx* T, int N, int io32) {
+    yf_fill_io(T, F, P, L0, nv, N, io32);
+  }
+};
+// whether a plan's kernels keep the second field transfer in flight
+template <class PL> struct YPfLive {
+  static constexpr bool value = PL::Q <= 8;
+};
+
 // exp(i w_m) of mode m: exp(2 pi i m/N) exp(i theta/N)
 __device__ __forceinline__ cplx yf_mode(const cplx* rt, int m, cplx eth) {
   return cmul(cconj(gld(rt + m)), eth);
@@ -211,11 +308,6 @@ __device__ __forceinline__ void k_ysym_body(int N, YCoef c, const cplx* __restri
   gst(o, yf_div(yf_symbol(rh1, e1), yf_symbol(pr1, e1)));
   gst(o + 1, yf_div(yf_symbol(rh2, e1), yf_symbol(pr2, e1)));
   gst(o + 2, mkc(yf_filter_symbol(e1), 0.0));
-}
-__global__ void __launch_bounds__(256)
-k_ysym(int N, YCoef c, const cplx* __restrict__ shp, const cplx* __restrict__ rt,
-       cplx* __restrict__ out, const DevScalars* sc) {
-  k_ysym_body(N, c, shp, rt, out, sc);
 }
 
 template <class PL, int IO>
@@ -319,15 +411,13 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  cplx* syo = rt + PL::tw_len(N);          // d/dy, d2/dy2 at the old twist: [2][m]
-  cplx* syn = syo + 2 * N;     // ... at the new twist
-  for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
-    const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
-    syo[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
-    syn[(t & 1) * N + (t >> 1)] = gld(a.ysym_new + src);
-  }
+  // d/dy, d2/dy2 at the old and the new twist
+  cplx* syo_s = rt + PL::tw_len(N);
+  cplx* syn_s = syo_s + ys_lds<PL>(N);
+  const auto syo = ys_row<PL>(syo_s, a.ysym_old, i, N);
+  const auto syn = ys_row<PL>(syn_s, a.ysym_new, i, N);
   // phase tables: old twist, new twist, the two moving-frame maps
-  cplx* pho = syn + 2 * N;
+  cplx* pho = syn_s + ys_lds<PL>(N);
   cplx* phn = pho + YPH_N(Q);
   cplx* pm1 = phn + YPH_N(Q);
   cplx* pm2 = pm1 + YPH_N(Q);
@@ -338,12 +428,13 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   }
 
   // ===== omega_y chain =====
-  // two fields' loads stay in flight (pfa, pfb) while the block transforms;
+  // two fields' loads stay in flight (pfa, pfb; pfb where YPfLive) while the block transforms;
   // each is issued about two stages before the tile needs it
-  cplx pfa[Q], pfb[Q];
+  cplx pfa[Q];
+  YPf<Q, YPfLive<PL>::value> pfb;
   yf_commit(T, pv, N);                     // vor
   yf_issue_io(pfa, a.rhg, P, L0, nv, N, IO & 2);
-  if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
+  if (rk > 1) pfb.issue_blk(vwk_blk, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -361,7 +452,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = syo[N + m];
+      const cplx s2 = syo.d2(m);
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
@@ -378,8 +469,8 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hg
     }
   }
-  if (rk > 1) yf_commit(T, pfb, N);       // vorwk
-  yf_issue(pfb, a.phi, P, L0, nv, N);
+  if (rk > 1) pfb.commit(T, N);       // vorwk
+  pfb.issue(a.phi, P, L0, nv, N);
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -417,7 +508,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = syn[m];
+      const cplx s1 = syn.d1(m);
       T[ac(m)] = cmul(T[ac(m)], crmul(invN, s1));
     }
   }
@@ -450,12 +541,12 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = syo[m];
+      const cplx s1 = syo.d1(m);
       T[ac(m)] = cmul(T[ac(m)], crmul(-invN, s1));
     }
   }
-  yf_commit(T, pfb, N);                    // phi
-  if (rk > 1) yf_issue_blk(pfb, pwk_blk, N);
+  pfb.commit(T, N);                    // phi
+  if (rk > 1) pfb.issue_blk(pwk_blk, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -473,7 +564,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = syo[N + m];
+      const cplx s2 = syo.d2(m);
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fac * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cadd(T[ac(m)], cmul(T[tc(m)], mul));
     }
@@ -489,7 +580,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], cmul(T[ac(j)], e));     // hv
     }
   }
-  if (rk > 1) yf_commit(T, pfb, N);       // phiwk
+  if (rk > 1) pfb.commit(T, N);       // phiwk
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -537,7 +628,7 @@ __device__ __forceinline__ void k_yf_explicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = syn[m], s2 = syn[N + m];
+      const cplx s1 = syn.d1(m), s2 = syn.d2(m);
       if (!is00) {
         const cplx H = yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y));   // lapcdy
         const cplx vh = cmul(T[ac(m)], H);
@@ -696,14 +787,12 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  cplx* syo = rt + PL::tw_len(N);          // d/dy, d2/dy2 at the old twist: [2][m]
-  cplx* syn = syo + 2 * N;     // ... at the new twist
-  for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
-    const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
-    syo[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
-    syn[(t & 1) * N + (t >> 1)] = gld(a.ysym_new + src);
-  }
-  cplx* pho = syn + 2 * N;
+  // d/dy, d2/dy2 at the old and the new twist
+  cplx* syo_s = rt + PL::tw_len(N);
+  cplx* syn_s = syo_s + ys_lds<PL>(N);
+  const auto syo = ys_row<PL>(syo_s, a.ysym_old, i, N);
+  const auto syn = ys_row<PL>(syn_s, a.ysym_new, i, N);
+  cplx* pho = syn_s + ys_lds<PL>(N);
   cplx* phn = pho + YPH_N(Q);
   cplx* pm1 = phn + YPH_N(Q);
   cplx* pm2 = pm1 + YPH_N(Q);
@@ -714,15 +803,16 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   }
   // Helmholtz multiplier 1/(R2/B2 - k2 - dtr1) / N at the new twist
   auto helm = [&](int m) {
-    const cplx s2 = syn[N + m];
+    const cplx s2 = syn.d2(m);
     return yf_div(mkc(invN, 0.0), mkc(s2.x - (k2 + dtr1), s2.y));
   };
 
   // ===== omega_y chain =====
-  cplx pfa[Q], pfb[Q];
+  cplx pfa[Q];
+  YPf<Q, YPfLive<PL>::value> pfb;
   yf_commit(T, pv, N);                     // vor
   yf_issue_io(pfa, a.rhg, P, L0, nv, N, IO & 2);
-  if (rk > 1) yf_issue_blk(pfb, vwk_blk, N);
+  if (rk > 1) pfb.issue_blk(vwk_blk, N);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -740,14 +830,14 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = syo[N + m];
+      const cplx s2 = syo.d2(m);
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fva * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
   }
   PL::template fft<1>(T, ac, rt, N);
   // own: alpha/Re (D2 - k2) vor + vor/dt [+ vorwk]
-  if (rk > 1) yf_commit(T, pfb, N);       // vorwk
+  if (rk > 1) pfb.commit(T, N);       // vorwk
   {
     YPh<false> ph(pho, lane);
 #pragma unroll
@@ -763,7 +853,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   }
   yf_commit_io(T, pfa, N, IO & 2);  // rhg
   yf_issue_io(pfa, a.rdome, P, L0, nv, N, IO & 2);
-  yf_issue(pfb, a.phi, P, L0, nv, N);
+  pfb.issue(a.phi, P, L0, nv, N);
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -788,7 +878,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
     if (m < N) {
       const cplx vh = is00 ? crmul(invN, T[ac(m)]) : cmul(T[ac(m)], helm(m));
       T[ac(m)] = vh;
-      T[tc(m)] = cmul(vh, syn[m]);
+      T[tc(m)] = cmul(vh, syn.d1(m));
     }
   }
   PL::template fft<1>(T, ac, rt, N);
@@ -814,8 +904,8 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   yf_flush(T, a.vor, P, L0, nv, N);
 
   // ===== phi chain =====
-  yf_commit(T, pfb, N);                    // phi
-  yf_issue_io(pfb, a.rhv, P, L0, nv, N, IO & 2);
+  pfb.commit(T, N);                    // phi
+  pfb.issue_io(a.rhv, P, L0, nv, N, IO & 2);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -833,7 +923,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s2 = syo[N + m];
+      const cplx s2 = syo.d2(m);
       const cplx mul = is00 ? mkc(0.0, 0.0) : crmul(fva * invN, mkc(s2.x - k2, s2.y));
       T[ac(m)] = cmul(T[ac(m)], mul);
     }
@@ -864,7 +954,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
-    if (m < N) T[tc(m)] = cmul(T[tc(m)], crmul(invN, syo[m]));
+    if (m < N) T[tc(m)] = cmul(T[tc(m)], crmul(invN, syo.d1(m)));
   }
   PL::template fft<1>(T, tc, rt, N);
   // pfb holds rhv; D1(rdome) goes to registers d, the tile turns over
@@ -886,7 +976,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
       if (j < N) T[ac(j)] = cadd(T[tc(j)], T[ac(j)]);
     }
   }
-  yf_commit_io(T, pfb, N, IO & 2);  // rhv
+  pfb.commit_io(T, N, IO & 2);  // rhv
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -937,7 +1027,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        const cplx s1 = syn[m], s2 = syn[N + m];
+        const cplx s1 = syn.d1(m), s2 = syn.d2(m);
         // own holds Helm'(pwk) / N with the 1/N already applied once: the
         // lapcdy multiplier B2/(R2 - k2 B2) without a second 1/N
         const cplx vh = yf_div(T[ac(m)], mkc(s2.x - k2, s2.y));
@@ -965,7 +1055,7 @@ __device__ __forceinline__ void k_yf_implicit_body(YArgs a, int rk) {
     for (int q = 0; q < Q; ++q) {
       const int m = lane + 64 * q;
       if (m < N) {
-        T[tc(m)] = cmul(T[ac(m)], crmul(invN, syn[m]));
+        T[tc(m)] = cmul(T[ac(m)], crmul(invN, syn.d1(m)));
         T[ac(m)] = crmul(invN, T[ac(m)]);
       }
     }
@@ -1058,12 +1148,9 @@ k_yf_first(YArgs a) {
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i);
   const double tho = atan2(so.y, so.x) * invN;
-  cplx* sy = rt + PL::tw_len(N);           // d/dy, d2/dy2 at the current twist: [2][m]
-  for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
-    const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
-    sy[(t & 1) * N + (t >> 1)] = gld(a.ysym_old + src);
-  }
-  cplx* pho = sy + 2 * N;
+  cplx* sy_s = rt + PL::tw_len(N);         // d/dy, d2/dy2 at the current twist
+  const auto sy = ys_row<PL>(sy_s, a.ysym_old, i, N);
+  cplx* pho = sy_s + ys_lds<PL>(N);
   {
     const double pa[1] = {tho}, pa0[1] = {0.0};
     for (int t = threadIdx.x; t < YPH_N(Q); t += blockDim.x) pho[t] = yph_entry(t, Q, pa, pa0);
@@ -1084,7 +1171,7 @@ k_yf_first(YArgs a) {
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
-    if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, sy[m]));
+    if (m < N) T[ac(m)] = cmul(T[ac(m)], crmul(invN, sy.d1(m)));
   }
   PL::template fft<1>(T, ac, rt, N);
   {
@@ -1124,7 +1211,7 @@ k_yf_first(YArgs a) {
   for (int q = 0; q < Q; ++q) {
     const int m = lane + 64 * q;
     if (m < N) {
-      const cplx s1 = sy[m], s2 = sy[N + m];
+      const cplx s1 = sy.d1(m), s2 = sy.d2(m);
       const cplx vh = is00 ? crmul(invN, T[ac(m)])
                            : cmul(T[ac(m)], yf_div(mkc(invN, 0.0), mkc(s2.x - k2, s2.y)));
       T[tc(m)] = cmul(vh, s1);
@@ -1254,4 +1341,16 @@ k_yf_filter(YArgs a) {
     }
     yf_flush(T, F, P, L0, nv, N);
   }
+}
+
+// box-table forms (batched ensembles): block z takes the arguments of box z
+template <class PL, int IO>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_explicit_tab(const YArgs* __restrict__ t, int rk) {
+  k_yf_explicit_body<PL, IO>(t[blockIdx.z], rk);
+}
+template <class PL, int IO>
+__global__ void __launch_bounds__(64 * YF_LINES) __attribute__((amdgpu_waves_per_eu(2)))
+k_yf_implicit_tab(const YArgs* __restrict__ t, int rk) {
+  k_yf_implicit_body<PL, IO>(t[blockIdx.z], rk);
 }

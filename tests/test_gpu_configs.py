@@ -9,11 +9,13 @@
   spectral y stage transforms its lines by Rader's algorithm.
 * 32x193x32, explicit and implicit, 10 steps against the live oracle, on
   the Rader y stage and on the banded twisted-cyclic sweeps.
-* 48x513x48, explicit and implicit, 5 steps against the live oracle (the
-  banded y stage at the 512^3 grid's line length).
+* 48x513x48, explicit and implicit, 5 steps against the live oracle, on
+  the spectral y stage and on the banded sweeps (the 512^3 grid's line
+  length).
 * 48x49x48, Re_z = 1000, 100 steps against the live oracle.
 * 512x513x512, Re_z = 8000 (too large for the oracle): the wave x-z kernels
-  against the Stockham ones over 2 steps (<= 1e-12), the device energy
+  against the Stockham ones and the spectral y stage against the banded
+  sweeps over 2 steps (<= 1e-12), the device energy
   against the oracle's energy of the downloaded state (<= 1e-12), and the
   mixed precision mode against FP64 over 3 steps (<= 1e-5).
 * 48x49x48 Newton workload, mixed precision: 16 JVP directions of one batch,
@@ -86,13 +88,16 @@ def test_y_my193_10_steps(explicit, path, monkeypatch):
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
 
+@pytest.mark.parametrize("path", ["spectral", "banded"])
 @pytest.mark.parametrize("explicit", [1, 0])
-def test_y_my513_5_steps(explicit):
-    """my = 513 (the 512x513x512 grid's line length, 3^3 * 19: no spectral
-    plan, the banded twisted-cyclic sweeps) on a 48 x 48 x-z plane, 5 steps
-    against the oracle (about 17 s of oracle time per case)."""
+def test_y_my513_5_steps(explicit, path, monkeypatch):
+    """my = 513 (the 512x513x512 grid's line length, 3^3 * 19) on a 48 x 48
+    x-z plane, 5 steps against the oracle (about 17 s of oracle time per
+    case): the spectral y stage (radix 9, 3 and a radix-19 pass spread over
+    the lanes) and the banded twisted-cyclic sweeps."""
     grid, Re = (48, 513, 48), 8000.0
     ic = box_ic(*grid)
+    monkeypatch.setenv("SNF_Y_PATH", path)
     box = _box(grid, Re, explicit=explicit, nhist=1)
     box.set_state(*ic)
     st = box.cross(5)
@@ -150,6 +155,14 @@ def test_512_xz_paths_agree(ic512, monkeypatch):
     st_g, gen, _ = _advance512(ic512, 2, monkeypatch, {"SNF_XZ_PATH": "generic"})
     assert st_w.steps_done == st_g.steps_done == 2
     for a, b in zip(wave, gen):
+        assert rel_l2(a, b) < STEP_TOL
+
+
+def test_512_y_paths_agree(ic512, monkeypatch):
+    st_s, spec, _ = _advance512(ic512, 2, monkeypatch, {"SNF_Y_PATH": "spectral"})
+    st_b, band, _ = _advance512(ic512, 2, monkeypatch, {"SNF_Y_PATH": "banded"})
+    assert st_s.steps_done == st_b.steps_done == 2
+    for a, b in zip(spec, band):
         assert rel_l2(a, b) < STEP_TOL
 
 

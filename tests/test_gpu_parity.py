@@ -177,12 +177,18 @@ def test_cross_spectral_y_3_steps(lib, grid):
 
 
 # iydealiasing=3: compact filter of vor, phi, U, W at every substep; line
-# lengths without a spectral plan (145, 193, 513: banded substeps) filter
+# lengths without a spectral plan (145; 513 on the banded substeps) filter
 # through the direct DFT
-@pytest.mark.parametrize("grid,explicit", [((32, 33, 32), 1), ((48, 49, 48), 1), ((48, 49, 48), 0),
-                                           ((32, 145, 32), 1), ((32, 193, 32), 1),
-                                           ((32, 193, 32), 0), ((24, 513, 24), 1)])
-def test_cross_compact_filter(lib, grid, explicit):
+@pytest.mark.parametrize("grid,explicit,path", [((32, 33, 32), 1, "spectral"),
+                                                ((48, 49, 48), 1, "spectral"),
+                                                ((48, 49, 48), 0, "spectral"),
+                                                ((32, 145, 32), 1, "spectral"),
+                                                ((32, 193, 32), 1, "spectral"),
+                                                ((32, 193, 32), 0, "spectral"),
+                                                ((24, 513, 24), 1, "spectral"),
+                                                ((24, 513, 24), 1, "banded")])
+def test_cross_compact_filter(lib, grid, explicit, path, monkeypatch):
+    monkeypatch.setenv("SNF_Y_PATH", path)
     st, (vor, phi, u00, w00), box, c = _run_pair(explicit, 4, grid=grid, ydealias=3)
     assert rel_l2(vor, c.vor) < STEP_TOL
     assert rel_l2(phi, c.phi) < STEP_TOL
