@@ -1173,12 +1173,9 @@ int snf_create(const snf_params* p, snf_handle** out) {
       y_pass_twiddles(R, np, tp.data());
       const int RL = R[np - 1];
       if (yf_spread_radix(RL)) {
-        // roots exp(-2 pi i t / RL) of a spread last pass, after the N entries
-        tp.resize((size_t)my + RL);
-        for (int t = 0; t < RL; ++t) {
-          const long double a = -2.0L * 3.14159265358979323846264338327950288L * t / RL;
-          tp[(size_t)my + t] = mkc((double)cosl(a), (double)sinl(a));
-        }
+        // the multipliers of a spread last pass, after the N entries
+        tp.resize((size_t)my + yf_spread_entries(RL));
+        yf_spread_table(RL, tp.data() + my);
       }
     } else if (const int nr = y_rader_plan(my, &rg, R)) {
       if (!y_rader_table(my, rg, R, nr, tp)) {

@@ -229,71 +229,83 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
 }
 
 // The last pass of a plan whose last radix R is a large odd prime (17 and
-// up), with the outputs spread over the lanes.  A butterfly takes H + 1 =
-// (R + 1) / 2 lanes: lane k = 0 forms y_0, lane k = 1..H forms the pair
-// y_k, y_(R-k) from the symmetric sums a_j = x_j + x_(R-j), b_j = x_j -
-// x_(R-j) (each input one LDS broadcast to the lanes of its butterfly), so
-// that 64 / (H + 1) butterflies run at once (60 of 64 lanes at R = 19,
-// against 27 butterflies on 27 lanes for a butterfly per lane), and a lane
-// holds its H multipliers, not the R inputs and 2H sums of a whole
-// butterfly.  The multipliers exp(-2 pi i t / R), t < R, sit at tw[N + t].
-// In the last Stockham pass (Ns = nb) butterfly b reads and writes the same
-// slots b + r nb, so each round writes as soon as its butterflies are read.
-// The pass's twiddles are applied by the previous pass as it writes
-// (yf_pass with PT >= 0): input b + r nb takes tw[OFF + (r - 1) nb + b] =
-// tw[OFF + (b + r nb) - nb].
+// up), each butterfly on two lanes.  With a_j = x_j + x_(R-j), b_j = x_j -
+// x_(R-j), j = 1..H, H = (R - 1) / 2, output pair k is
+//   y_k, y_(R-k) = x_0 + sum_j a_j cos(2 pi j k / R) +- SIGN i sum_j b_j sin(2 pi j k / R).
+// Lane h of a butterfly forms the pairs k = 1..K (h = 0) or K + 1..H and
+// y_0 (h = 1), K = (H + 1) / 2, walking j once and keeping only its K pair
+// sums, not the R inputs and 2H sums of a butterfly per lane (which spill
+// the y kernels at R = 19).  Both lanes read each input (an LDS broadcast).
+// The multipliers exp(-2 pi i j k / R) sit at tw[N + (h K + k') H + j - 1],
+// k' < K the lane's pair, j = 1..H (yf_spread_table).
+// In the last Stockham pass (Ns = nb) butterfly b reads and writes the
+// same slots b + r nb.  The pass's twiddles are applied by the previous
+// pass as it writes (yf_pass with PT >= 0): input b + r nb takes
+// tw[OFF + (r - 1) nb + b] = tw[OFF + (b + r nb) - nb].
 template <int R, int SIGN, int N, class Col>
 __device__ __forceinline__ void yf_pass_spread(cplx* T, Col col, const cplx* tw) {
-  constexpr int H = (R - 1) / 2, L = H + 1;
-  constexpr int G = 64 / L;                  // butterflies per round
+  constexpr int H = (R - 1) / 2, K = (H + 1) / 2;
   constexpr int nb = N / R;
-  constexpr int NR = (nb + G - 1) / G;       // rounds
+  constexpr int NR = (nb + 31) / 32;         // rounds of 32 butterflies
   const int lane = threadIdx.x & 63;
-  const int g = lane / L, k = lane - g * L;
-#ifdef YSPREAD_REGS
-  double cs[H], sn[H];
-#pragma unroll
-  for (int j = 1; j <= H; ++j) {
-    const cplx r = tw[N + (j * k) % R];     // exp(-2 pi i j k / R)
-    cs[j - 1] = r.x;
-    sn[j - 1] = -r.y;
-  }
-#endif
+  const int g = lane >> 1, h = lane & 1;
+  const cplx* mt = tw + N + h * K * H;       // this lane's multipliers [k'][j]
 #pragma unroll
   for (int t = 0; t < NR; ++t) {
-    const int b = g + G * t;
-    const bool act = g < G && b < nb;
-    cplx ylo = mkc(0.0, 0.0), yhi = mkc(0.0, 0.0);
+    const int b = g + 32 * t;
+    const bool act = b < nb;
+    cplx re[K], im[K], sum = mkc(0.0, 0.0), x0 = mkc(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < K; ++q) re[q] = im[q] = mkc(0.0, 0.0);
     if (act) {
-      const cplx x0 = T[col(b)];
-      cplx re = x0, im = mkc(0.0, 0.0), sum = x0;
+      x0 = T[col(b)];
 #pragma unroll
       for (int j = 1; j <= H; ++j) {
         const cplx xp = T[col(b + j * nb)], xm = T[col(b + (R - j) * nb)];
         const cplx aj = wf_add(xp, xm), bj = wf_sub(xp, xm);
         sum = wf_add(sum, aj);
-#ifdef YSPREAD_REGS
-        const double c = cs[j - 1], sj = sn[j - 1];
-#else
-        const cplx rr = tw[N + (j * k) % R];
-        const double c = rr.x, sj = -rr.y;
-#endif
-        re.x += c * aj.x;
-        re.y += c * aj.y;
-        im.x += sj * bj.x;
-        im.y += sj * bj.y;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+          const cplx w = mt[q * H + j - 1];     // exp(-2 pi i j k / R)
+          re[q].x += w.x * aj.x;
+          re[q].y += w.x * aj.y;
+          im[q].x -= w.y * bj.x;
+          im[q].y -= w.y * bj.y;
+        }
       }
-      const cplx si = wf_si<SIGN>(im);      // SIGN * i * im
-      ylo = k == 0 ? sum : wf_add(re, si);
-      yhi = wf_sub(re, si);
     }
     wf_sync();   // the butterflies of this round are read
     if (act) {
-      T[col(b + k * nb)] = ylo;
-      if (k > 0) T[col(b + (R - k) * nb)] = yhi;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const int k = h * K + q + 1;
+        if (k <= H) {
+          const cplx base = wf_add(x0, re[q]), si = wf_si<SIGN>(im[q]);   // SIGN * i * im
+          T[col(b + k * nb)] = wf_add(base, si);
+          T[col(b + (R - k) * nb)] = wf_sub(base, si);
+        }
+      }
+      if (h == 1) T[col(b)] = wf_add(x0, sum);
     }
   }
   wf_sync();
+}
+// entries of the multiplier table of a spread pass of radix R
+__host__ __device__ constexpr int yf_spread_entries(int R) {
+  return 2 * (((R - 1) / 2 + 1) / 2) * ((R - 1) / 2);
+}
+// the multiplier table of a spread pass of radix R: exp(-2 pi i j k / R) at
+// [h][k'][j - 1], k = h K + k' + 1 (zero beyond H)
+inline void yf_spread_table(int R, cplx* out) {
+  const int H = (R - 1) / 2, K = (H + 1) / 2;
+  for (int h = 0; h < 2; ++h)
+    for (int q = 0; q < K; ++q)
+      for (int j = 1; j <= H; ++j) {
+        const int k = h * K + q + 1;
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)((j * k) % R) / R;
+        out[(h * K + q) * H + j - 1] = k <= H ? make_double2((double)cosl(a), (double)sinl(a))
+                                              : make_double2(0.0, 0.0);
+      }
 }
 // radix of a last pass that runs spread over the lanes
 __host__ __device__ constexpr bool yf_spread_radix(int R) { return R >= 17 && R % 2 == 1; }
@@ -529,9 +541,10 @@ template <int... Rs> struct YFFT {
   // multiplier rows read from the global table: beyond 448 points the rows
   // staged in LDS would leave one block per CU
   static constexpr bool sym_gl = NC > 448;
-  // the pass twiddles (at most N entries), then the R-th roots of a spread
-  // last pass of radix R
-  static constexpr int RS = yf_spread_radix(YLast<Rs...>::value) ? YLast<Rs...>::value : 0;
+  // the pass twiddles (at most N entries), then the multipliers of a spread
+  // last pass
+  static constexpr int RS =
+      yf_spread_radix(YLast<Rs...>::value) ? yf_spread_entries(YLast<Rs...>::value) : 0;
   __host__ __device__ static constexpr int tw_len(int n) { return n + RS; }
   // active = false: this wave's column stays as it is (the call keeps the
   // waves of a block in step for the block-collective plan)
