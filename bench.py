@@ -35,13 +35,17 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="precision of the main timed leg (mixed: no separate mixed leg)")
+    ap.add_argument("--grid2", type=str, default="512,513,512",
+                    help="second grid of the mixed-precision configuration ('' to skip)")
+    ap.add_argument("--grid2-steps", type=int, default=20,
+                    help="timed steps on the second grid (FP64 and mixed)")
     ap.add_argument("--cmp-steps", type=int, default=2000,
                     help="steps of the FP64-vs-mixed statistics comparison (2000: the "
                          "length the mixed-precision workload is specified over)")
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="oracle processes of the CPU baseline (capped at the host's CPUs)")
     ap.add_argument("--cpu-grid", type=str, default="",
-                    help="grid of the CPU baseline sample (default: the workload grid)")
+                    help="grid of the CPU baseline sample (default: half the workload grid in x, y, z)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run the streaming-copy probe on this many GiB and exit")
     return ap.parse_args()
@@ -206,6 +210,25 @@ def mixed_error(snf, ic, grid, Re, explicit, nstep):
     return err
 
 
+def grid2_leg(snf, make_ic, grid, Re, explicit, steps, warmup, cmp_steps, seed):
+    """The mixed-precision configuration's second grid (512x513x512 by
+    default): FP64 and mixed steps/s over `steps` timed steps each, the
+    per-launch kernel times, and the mixed mode's statistics error against
+    FP64 over cmp_steps steps from the same seeded state."""
+    ic = make_ic(*grid, math.pi, 2 * math.pi, 1.0, seed=seed)
+    res = {"grid": list(grid), "steps": steps}
+    for prec in ("fp64", "mixed"):
+        st, wall, timing, (_, _, nc) = timed_leg(snf, ic, grid, Re, explicit, steps, warmup, prec)
+        sps = st.steps_done / wall
+        res[prec] = {"value": sps, "unit": "steps/s",
+                     "kernel_ms_per_launch": {k: ms / cnt for k, (ms, cnt) in timing.items() if cnt},
+                     "step_roofline_frac": step_bytes(nc, prec) * sps / 1e9 / HBM_PEAK_GBS}
+    res["mixed"]["speedup_vs_fp64"] = res["mixed"]["value"] / res["fp64"]["value"]
+    res["mixed"]["rel_err_vs_fp64"] = (mixed_error(snf, ic, grid, Re, explicit, cmp_steps)
+                                       if cmp_steps > 0 else None)
+    return res
+
+
 def batch_leg(snf, make_ic, nbox, steps, seed):
     """Ensemble of independent boxes (grids n^3 and n x (n+1) x n, n cycling
     over 32, 48, 64, 96; Re_z uniform in [500, 3000]; seeded fields): box-steps
@@ -351,6 +374,9 @@ def main():
             "rel_err_vs_fp64": mixed_error(snf, ic, grid, a.Re, a.explicit, a.cmp_steps)
             if a.cmp_steps > 0 else None,
         }
+    if a.grid2 and not a.no_mixed and a.precision == "fp64":
+        out["grid2"] = grid2_leg(snf, make_ic, tuple(int(x) for x in a.grid2.split(",")), a.Re,
+                                 a.explicit, a.grid2_steps, a.warmup, a.cmp_steps, seed)
     if a.batch > 0 and a.precision == "fp64":
         out["ensemble"] = batch_leg(snf, make_ic, a.batch, a.batch_steps, seed)
     if cpu is not None:
