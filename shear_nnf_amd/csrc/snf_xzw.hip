@@ -221,15 +221,29 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     for (int q = 0; q < 8; ++q) {
       const R u = zx[0][q].x, v = zx[0][q].y, w = zx[1][q].x, o1 = zx[1][q].y;
       const R o2 = zx[2][q].x, o3 = zx[2][q].y;
-      m0 = fmax(m0, fabs(u));
-      m1 = fmax(m1, fabs(u + sy));
-      m2 = fmax(m2, fabs(v));
-      m3 = fmax(m3, fabs(w));
-      p0 += (double)(v * (u * u));
-      p1 += (double)(v * (w * w));
-      p2 += (double)(v * v * v);
       z12[q] = wf_mk(v * (o3 + c2) - w * o2, w * o1 - u * (o3 + c2));
       z3[q] = wf_mk(u * o2 - v * o1, (R)0);
+    }
+    // the maxima only at rk 1, the 'p' sums only on a statistics step
+    // (uniform branches: the other substeps skip the work)
+    if (rk == 1) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const R u = zx[0][q].x, v = zx[0][q].y, w = zx[1][q].x;
+        m0 = fmax(m0, fabs(u));
+        m1 = fmax(m1, fabs(u + sy));
+        m2 = fmax(m2, fabs(v));
+        m3 = fmax(m3, fabs(w));
+      }
+    }
+    if (ihist) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const R u = zx[0][q].x, v = zx[0][q].y, w = zx[1][q].x;
+        p0 += (double)(v * (u * u));
+        p1 += (double)(v * (w * w));
+        p2 += (double)(v * v * v);
+      }
     }
   }
   // --- forward x transforms -> lane layout bins ---
