@@ -1695,6 +1695,8 @@ static int class_enqueue(ClassRun& r, const std::vector<int>& resume, int nstep)
   return SNF_OK;
 }
 
+static int batch_enqueue(snf_handle* const* hs, int n, const AdvanceMark* m, int nstep);
+
 int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_status* st) {
   if (!hs || n < 1 || nstep < 0) return fail(SNF_BAD_PARAM, "bad argument");
   for (int i = 0; i < n; ++i) {
@@ -1707,8 +1709,18 @@ int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_s
   int rc;
   for (int i = 0; i < n; ++i)
     if ((rc = advance_begin(hs[i], false, nstep, m[i]))) return rc;
-  // size classes of two or more table-eligible boxes; SNF_BATCH_TABLES=0
-  // keeps every box on its own stream
+  if ((rc = batch_enqueue(hs, n, m.data(), nstep))) return rc;
+  for (int i = 0; i < n; ++i)
+    if ((rc = advance_finish(hs[i], m[i], st ? st + i : nullptr))) return rc;
+  return SNF_OK;
+}
+
+// nstep steps of n begun advances (advance_begin): the size classes of two
+// or more table-eligible boxes one launch per kernel from box tables, the
+// other boxes on their own streams; SNF_BATCH_TABLES=0 keeps every box on
+// its own stream.  The caller finishes the advances (advance_finish).
+static int batch_enqueue(snf_handle* const* hs, int n, const AdvanceMark* m, int nstep) {
+  int rc;
   const char* et = getenv("SNF_BATCH_TABLES");
   const bool tables = !(et && std::string(et) == "0");
   std::map<std::vector<int>, std::vector<int>> cls;
@@ -1747,11 +1759,9 @@ int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_s
       return rc;
     }
   }
-  for (int i = 0; i < n; ++i)
-    if ((rc = advance_finish(hs[i], m[i], st ? st + i : nullptr))) {
-      cleanup();
-      return rc;
-    }
+  // the class runs' device tables are freed once their steps are done
+  for (ClassRun& r : runs)
+    for (snf_handle* h : r.hs) (void)hipStreamSynchronize(h->stream);
   cleanup();
   return SNF_OK;
 }
@@ -1767,7 +1777,9 @@ int snf_state_len(snf_handle* h, int64_t* n) {
 // starts a new run from its state x[q] at (t0, xoffb0, xofft0) with its
 // statistics restarted and stops at etime = t0 + T (cross.f90:1680-1712).
 // The cross calls run in chunks of steps that continue them; every chunk of
-// every unfinished handle is in flight before any is waited for.
+// every unfinished handle is in flight before any is waited for, those of a
+// size class from box tables (a handle that reaches etime inside a chunk is
+// halted on the device and its kernels in the table launches do nothing).
 static int shoot_many(snf_handle* const* hs, int n, const double* const* x, double t0,
                       double xoffb0, double xofft0, double T, double* const* y,
                       snf_run_status* st) {
@@ -1798,10 +1810,17 @@ static int shoot_many(snf_handle* const* hs, int n, const double* const* x, doub
     if (c > (1 << 22) / chunk) return fail(SNF_DEVICE_ERROR, "shooting did not reach t0 + T");
     // the begins read device scalars with blocking copies: all of them
     // before any handle's steps are in flight
+    std::vector<snf_handle*> lh;
+    std::vector<AdvanceMark> lm;
     for (int q = 0; q < n; ++q)
-      if (live[q] && (rc = advance_begin(hs[q], c > 0, chunk, m[q]))) return rc;
-    for (int q = 0; q < n; ++q)
-      if (live[q] && (rc = advance_enqueue(hs[q], chunk, m[q].resume))) return rc;
+      if (live[q]) {
+        if ((rc = advance_begin(hs[q], c > 0, chunk, m[q]))) return rc;
+        lh.push_back(hs[q]);
+        lm.push_back(m[q]);
+      }
+    // the handles of one size class advance from box tables, one launch per
+    // kernel for all of them (the others each on its own stream)
+    if ((rc = batch_enqueue(lh.data(), (int)lh.size(), lm.data(), chunk))) return rc;
     for (int q = 0; q < n; ++q)
       if (live[q]) {
         if ((rc = advance_finish(hs[q], m[q], &r[q]))) return rc;
