@@ -14,14 +14,22 @@
  *   vor, phi : complex(0:my-1, 0:mx/2-1, 0:mz-1) stored as interleaved
  *              real/imag doubles, y fastest, then kx, then kz
  *   u00, w00 : real(0:my-1)
- * Device storage is [y][kx][kz] (kz fastest); the handle transposes on
- * upload/download.  All arithmetic is IEEE binary64 unless the handle was
- * created with SNF_PREC_MIXED.
+ * Device storage is kz-chunk-major, [kx][kz/4][y][kz%4]; the handle
+ * transposes on upload/download (on the device).  All arithmetic is IEEE
+ * binary64 unless the handle was created with SNF_PREC_MIXED.
+ *
+ * The library exports these functions only (built with hidden visibility).
  */
 #ifndef SHEAR_RK3_H
 #define SHEAR_RK3_H
 
 #include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define SNF_API __attribute__((visibility("default")))
+#else
+#define SNF_API
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -104,27 +112,27 @@ typedef struct snf_run_status {
 /* ---- lifetime ---------------------------------------------------------
  * Replaces initcr(0) grid setup + derivadas/ffti (initcr.f90:186-306,
  * diffy_shear_hepta.f90:535-573, fou3D_thread.f:1-19).                  */
-int snf_create(const snf_params* p, snf_handle** out);
-int snf_destroy(snf_handle* h);
-const char* snf_last_error(void);
+SNF_API int snf_create(const snf_params* p, snf_handle** out);
+SNF_API int snf_destroy(snf_handle* h);
+SNF_API const char* snf_last_error(void);
 
 /* sizes: kxn = mx/2 complex kx modes, mz kz modes, nc = my*kxn*mz */
-int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc);
+SNF_API int snf_sizes(const snf_handle* h, int32_t* kxn, int32_t* mz, int64_t* nc);
 
 /* ---- state transfer (host jik layout, FP64) ---------------------------
  * Replaces the argument passing of cross (vor, phi, u00, w00) and the
  * module scalars time (running) and xoffb/xofft (bcs).                   */
-int snf_state_upload(snf_handle* h, const double* vor, const double* phi,
+SNF_API int snf_state_upload(snf_handle* h, const double* vor, const double* phi,
                      const double* u00, const double* w00,
                      double time, double xoffb, double xofft);
-int snf_state_download(snf_handle* h, double* vor, double* phi,
+SNF_API int snf_state_download(snf_handle* h, double* vor, double* phi,
                        double* u00, double* w00,
                        double* time, double* xoffb, double* xofft);
 /* same, in the device layout [y][kx][kz] (interleaved complex)           */
-int snf_state_upload_ykxkz(snf_handle* h, const double* vor, const double* phi,
+SNF_API int snf_state_upload_ykxkz(snf_handle* h, const double* vor, const double* phi,
                            const double* u00, const double* w00,
                            double time, double xoffb, double xofft);
-int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi,
+SNF_API int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi,
                              double* u00, double* w00);
 
 /* ---- the time advance -------------------------------------------------
@@ -135,16 +143,16 @@ int snf_state_download_ykxkz(snf_handle* h, double* vor, double* phi,
  * than nstep steps done): it continues that run, as the reference's loop
  * continues after writing the dump.  snf_state_upload* starts a new run.
  * UNER records are kept on the device; fetch them with snf_history.      */
-int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st);
+SNF_API int snf_advance(snf_handle* h, int32_t nstep, snf_run_status* st);
 /* ---- shooting (the Newton/Arnoldi glue; the Krylov packing stays with
  * the caller) -------------------------------------------------------------
  * A state vector is vor, phi in the layout [y][kx][kz] (interleaved complex,
  * 2*my*kxn*mz doubles each) followed by u00, w00 (my doubles each).      */
-int snf_state_len(snf_handle* h, int64_t* n);
+SNF_API int snf_state_len(snf_handle* h, int64_t* n);
 /* the shooting map of calc_dns_disk3d (gmres_shear.f90:2951-3120): from x
  * at time t0 with offsets xoffb0, xofft0, restart the statistics and run one
  * cross call to t0 + T (the etime stop); y receives the final state.    */
-int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double xofft0,
+SNF_API int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double xofft0,
               double T, double* y, snf_run_status* st);
 /* forward-difference Jacobian-vector products of the shooting map
  * (matvec_dns_disk3d with iopt_matvec=1, gmres_shear.f90:2666-2730):
@@ -154,46 +162,46 @@ int snf_shoot(snf_handle* h, const double* x, double t0, double xoffb0, double x
  * first need while half the free device memory holds them, kept until
  * snf_destroy(h)), each bitwise the same as a separate snf_shoot.  h is left
  * holding the state of one of the shootings.                             */
-int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
+SNF_API int snf_jvp_batch(snf_handle* h, const double* x0, const double* y0, const double* dirs,
                   int32_t ndir, const double* eps, double t0, double xoffb0, double xofft0,
                   double T, double* out);
 /* ensembles: advances n independent handles by up to nstep steps each, as
  * n snf_advance calls would, with every handle's steps in flight on its own
  * stream before any is waited for (no counterpart in the reference, which
  * runs one box per MPI job).  st: n statuses (or null).                  */
-int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_status* st);
+SNF_API int snf_advance_batch(snf_handle* const* hs, int32_t n, int32_t nstep, snf_run_status* st);
 /* copies up to max_records records (SNF_UNER_LEN doubles each) produced
  * since the last call; *n receives the count                             */
-int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n);
+SNF_API int snf_history(snf_handle* h, double* out, int32_t max_records, int32_t* n);
 /* per-plane statistics stats(1:23, 0:my-1) and stats2 (cross.f90:1798-2045),
  * column-major like the reference: out[s + 23*j]                          */
-int snf_statistics(snf_handle* h, double* stats, double* stats2,
+SNF_API int snf_statistics(snf_handle* h, double* stats, double* stats2,
                    int32_t* nacum, double* total_dt);
 /* spectra of the statistics sp, sp2 (cross.f90:1970-2016): out[q + 8*(i +
  * kxn*k)] = sp(q+1, i, k), the sums over samples and planes of |u|^2,
  * |v|^2, |w|^2, Re(u v*), |o1|^2, |o2|^2, |o3|^2, Re(o1 o2*) with kx > 0
  * counted twice (sp2: each sample times its step)                        */
-int snf_spectra(snf_handle* h, double* sp, double* sp2);
+SNF_API int snf_spectra(snf_handle* h, double* sp, double* sp2);
 /* zero stats, stats2, sp, sp2, nacum and total_dt, as escru does after
  * writing them (readwrite.f90:1322-1334)                                 */
-int snf_reset_statistics(snf_handle* h);
+SNF_API int snf_reset_statistics(snf_handle* h);
 
 /* ---- operator entry points for parity tests (host arrays, [y][line]) --
  * derivyc (diffy_shear_hepta.f90:333-450): order 1 or 2 on nl lines of
  * length my; line l uses twist shp[l], shm[l].                           */
-int snf_op_derivy(snf_handle* h, const double* u, double* du, int32_t nl,
+SNF_API int snf_op_derivy(snf_handle* h, const double* u, double* du, int32_t nl,
                   const double* shp, const double* shm, int32_t order);
 /* lapcdy (diffy_shear_hepta.f90:466-525): (d_yy - k2[l]) p = u           */
-int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl,
+SNF_API int snf_op_helmholtz(snf_handle* h, const double* u, double* p, int32_t nl,
                      const double* k2, const double* shp, const double* shm);
 /* getv (operators.f90:735-793) on fields in the device layout [y][kx][kz]
  * (interleaved complex, my*kxn*mz values) with the twist of the handle's
  * current offsets: dir=+1 v = lapcdy(phi) (phi -> v), dir=-1 phi =
  * d2v/dy2 - k2 v (v -> phi).  The (0,0) line of out is not written.     */
-int snf_op_getv(snf_handle* h, const double* in, double* out, int32_t dir);
+SNF_API int snf_op_getv(snf_handle* h, const double* in, double* out, int32_t dir);
 /* energy (operators.f90:870-924) of the device state: out[0..3] = sums of
  * |vor|^2, |phi|^2 (kx > 0 counted twice), U^2, W^2.                    */
-int snf_energy(snf_handle* h, double* out);
+SNF_API int snf_energy(snf_handle* h, double* out);
 /* The same sums as snf_energy, from the same kernels (bitwise equal
  * results), with no device memory allocated or freed by the call: no
  * counterpart in the reference; for callers that cannot allocate (code
@@ -204,16 +212,16 @@ int snf_energy(snf_handle* h, double* out);
  * box exceeds it.  Calls on one GPU share that array: the library
  * serialises them within a process, and every partial is written before it
  * is read, so a call depends on no earlier call's contents.            */
-int snf_energy_noalloc(snf_handle* h, double* out);
+SNF_API int snf_energy_noalloc(snf_handle* h, double* out);
 /* fourxz (fou3D_thread.f:22-73) on nplanes planes: dir=+1 spectral
  * (kxn x mz complex, [kx][kz]) -> physical (mgalx x mgalz, [x][z]);
  * dir=-1 the reverse with the 1/(mgalx mgalz) scaling.                   */
-int snf_op_fourxz(snf_handle* h, const double* in, double* out,
+SNF_API int snf_op_fourxz(snf_handle* h, const double* in, double* out,
                   int32_t nplanes, int32_t dir);
 /* one hvhg evaluation (cross.f90:1127-1765) on the uploaded state with the
  * given v, dv/dy and d(omega_y)/dy fields (device layout); returns the
  * three outputs (ome1c -> dvordy, rhgc -> hg, rhvc -> hv) and rf0u, rf0w. */
-int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv,
+SNF_API int snf_op_hvhg(snf_handle* h, const double* hg_v, const double* hv_dv,
                 const double* dvordy, double* out_dvordy, double* out_hg,
                 double* out_hv, double* rf0u, double* rf0w);
 
@@ -230,14 +238,14 @@ enum snf_kernel_class {
   SNF_K_SMALL = 4,     /* scalar/CFL, twist tails, hist, step control    */
   SNF_K_NCLASS = 5
 };
-int snf_set_timing(snf_handle* h, int32_t on);
-int snf_get_timing(snf_handle* h, double* ms, int32_t* counts, int32_t nclass);
+SNF_API int snf_set_timing(snf_handle* h, int32_t on);
+SNF_API int snf_get_timing(snf_handle* h, double* ms, int32_t* counts, int32_t nclass);
 
 /* Streaming-copy probe (no counterpart in the reference): copies nbytes
  * between two device buffers iters times with a plain grid-stride kernel
  * and returns the mean kernel time in *ms.  Used to measure attainable HBM
  * bandwidth and to calibrate the FETCH_SIZE / WRITE_SIZE counters.        */
-int snf_copy_probe(int32_t device, int64_t nbytes, int32_t iters, double* ms);
+SNF_API int snf_copy_probe(int32_t device, int64_t nbytes, int32_t iters, double* ms);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ out="${1:-$here/../libshear_rk3.so}"
 jobs="${SNF_BUILD_JOBS:-$(nproc)}"
 tmp="$(mktemp -d)"
 trap 'rm -rf "$tmp"' EXIT
-flags=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC
+flags=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC -fvisibility=hidden
        -Xclang -target-feature -Xclang -packed-fp32-ops -Wno-unused-result)
 srcs=("$here/snf_api.hip" "$here"/snf_yplans_*.hip)
 printf '%s\n' "${srcs[@]}" | xargs -P "$jobs" -I{} sh -c \
