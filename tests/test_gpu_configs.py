@@ -1,8 +1,8 @@
 """GPU parity at the benchmark configurations of the path.
 
-* 192x193x192, Re_z = 4000, explicit, 10 RK3 steps (the drop-in parity
-  workload): against the committed oracle fixture
-  ``tests/golden/config2_192x193x192.npz`` (made by
+* 192x193x192, Re_z = 4000, explicit and implicit, 10 RK3 steps (the drop-in
+  parity workload): against the committed oracle fixtures
+  ``tests/golden/config2_192x193x192{,_implicit}.npz`` (made by
   ``tests/golden/make_config2.py``): the relative L2 error per field, estimated
   through 64 seeded complex Gaussian projections, <= 1e-12; U, W exactly as
   fields <= 1e-12; every UNER record to rtol 1e-11.  my = 193 is prime: the
@@ -12,7 +12,8 @@
 * 48x513x48, explicit and implicit, 5 steps against the live oracle, on
   the spectral y stage and on the banded sweeps (the 512^3 grid's line
   length).
-* 48x49x48, Re_z = 1000, 100 steps against the live oracle.
+* 48x49x48, Re_z = 1000, 100 steps against the live oracle, explicit and
+  implicit.
 * 512x513x512, Re_z = 8000 (too large for the oracle): the wave x-z kernels
   against the Stockham ones and the spectral y stage against the banded
   sweeps over 2 steps (<= 1e-12), the device energy
@@ -24,7 +25,6 @@
 """
 import gc
 import math
-import os
 
 import numpy as np
 import pytest
@@ -34,7 +34,6 @@ from tests.boxes import ALP, GAM, LY, box_ic, rel_l2
 pytestmark = pytest.mark.gpu
 
 STEP_TOL = 1e-12
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _box(grid, Re, **kw):
@@ -49,11 +48,12 @@ def _oracle(grid, Re, explicit=1, **kw):
     return Cross(g, explicit=explicit, **kw)
 
 
-def test_config2_192x193x192_10_steps():
-    from tests.golden.make_config2 import GRID, NSTEP, RE, SEED, sketch
-    ref = np.load(os.path.join(GOLDEN, "config2_192x193x192.npz"))
-    print(f"config 2: {GRID} Re_z={RE} {NSTEP} steps vs fixture", flush=True)
-    box = _box(GRID, RE, explicit=1, nhist=1)
+@pytest.mark.parametrize("explicit", [1, 0])
+def test_config2_192x193x192_10_steps(explicit):
+    from tests.golden.make_config2 import GRID, NSTEP, RE, SEED, fixture_path, sketch
+    ref = np.load(fixture_path(explicit))
+    print(f"config 2: {GRID} Re_z={RE} explicit={explicit} {NSTEP} steps vs fixture", flush=True)
+    box = _box(GRID, RE, explicit=explicit, nhist=1)
     box.set_state(*box_ic(*GRID, seed=SEED))
     st = box.cross(NSTEP)
     assert st.steps_done == NSTEP
@@ -110,18 +110,19 @@ def test_y_my513_5_steps(explicit, path, monkeypatch):
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
 
-def test_config1_48x49x48_100_steps():
+@pytest.mark.parametrize("explicit", [1, 0])
+def test_config1_48x49x48_100_steps(explicit):
     grid, Re = (48, 49, 48), 1000.0
     ic = box_ic(*grid)
-    box = _box(grid, Re, explicit=1, nhist=1)
+    box = _box(grid, Re, explicit=explicit, nhist=1)
     box.set_state(*ic)
     st = box.cross(100)
-    c = _oracle(grid, Re, explicit=1, nhist=1)
+    c = _oracle(grid, Re, explicit=explicit, nhist=1)
     c.set_state(*ic)
     c.run(100)
     assert st.steps_done == 100
     errs = [rel_l2(a, b) for a, b in zip(box.get_state(), (c.vor, c.phi, c.u00, c.w00))]
-    print("config 1, 100 steps: rel L2 per field", errs)
+    print(f"config 1, explicit={explicit}, 100 steps: rel L2 per field", errs)
     assert max(errs) < STEP_TOL
     np.testing.assert_allclose(np.array(box.history), np.array(c.history), rtol=1e-11, atol=1e-14)
 
