@@ -44,8 +44,10 @@ def parse():
                          "length the mixed-precision workload is specified over)")
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="oracle processes of the CPU baseline (capped at the host's CPUs)")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="RK3 steps of each CPU baseline process")
     ap.add_argument("--cpu-grid", type=str, default="",
-                    help="grid of the CPU baseline sample (default: half the workload grid in x, y, z)")
+                    help="grid of the CPU baseline sample (default: the workload grid)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run the streaming-copy probe on this many GiB and exit")
     return ap.parse_args()
@@ -110,7 +112,10 @@ def _cpu_sample(args):
     c = Cross(g, explicit=explicit, nhist=10 ** 9)
     c.set_state(vor, phi, u00, w00)
     t0 = time.perf_counter()
-    c.run(nstep)
+    for _ in range(nstep):
+        c.run(1)
+        print(f"cpu baseline: one {mgalx}x{my}x{mgalz} step of the oracle done", file=sys.stderr,
+              flush=True)
     t1 = time.perf_counter()
     return (t1 - t0) / nstep, t0, t1
 
@@ -129,18 +134,17 @@ def host_cpu():
     return model, os.cpu_count()
 
 
-def cpu_baseline(grid, sample_grid, Re, explicit, seed, procs):
+def cpu_baseline(grid, sample_grid, Re, explicit, seed, procs, nstep=2):
     """The numpy oracle on the host cores: procs processes at once, each
-    running 2 RK3 steps of the same seeded sample (one numpy thread each;
+    running nstep RK3 steps of the same seeded state (one numpy thread each;
     the oracle has no threading of its own), so every core works on the
-    reference algorithm.  The aggregate rate at the sample grid is scaled
-    to the workload grid by the number of spectral coefficients (the
-    oracle's cost is linear in Nc up to the log factor of the FFTs).  One
-    384^3 step of the oracle takes ~100 s on one core, so the sample is
-    192x193x192.  Runs before anything initialises the GPU (the workers
-    are fresh processes)."""
+    reference algorithm.  By default the sample is the workload itself, one
+    step per process (about 37 s a step at 384x385x384 with 16 processes,
+    10 GB each); a smaller sample grid is scaled to the workload grid by
+    the number of spectral coefficients (the oracle's cost is linear in Nc
+    up to the log factor of the FFTs).  Runs before anything initialises
+    the GPU (the workers are fresh processes)."""
     import multiprocessing as mp
-    nstep = 2
     ctx = mp.get_context("spawn")
     with ctx.Pool(procs) as pool:
         res = pool.map(_cpu_sample, [(grid, sample_grid, Re, explicit, seed, nstep)] * procs)
@@ -153,7 +157,7 @@ def cpu_baseline(grid, sample_grid, Re, explicit, seed, procs):
     g = Grid(*sample_grid, Re=Re, alp=math.pi, gam=2 * math.pi, Ly=1.0)
     scale = g.nc / G.nc
     model, ncpu = host_cpu()
-    return rate * scale, (f"{procs} processes at once, each {nstep} RK3 steps of the numpy oracle "
+    return rate * scale, (f"{procs} processes at once, each {nstep} RK3 step(s) of the numpy oracle "
                           f"at {sample_grid[0]}x{sample_grid[1]}x{sample_grid[2]} "
                           f"({min(per_step):.2f}-{max(per_step):.2f} s per step per process, "
                           f"{span:.1f} s for all); aggregate {rate:.3f} steps/s scaled to "
@@ -295,12 +299,10 @@ def main():
     cpu = None
     if not a.no_cpu and a.copy_probe <= 0:
         # before anything touches the GPU: the workers are fresh processes
-        if a.cpu_grid:
-            cg = tuple(int(x) for x in a.cpu_grid.split(","))
-        else:
-            cg = (max(mgalx // 2, 32), max((my - 1) // 2 + 1, 33), max(mgalz // 2, 32))
+        cg = tuple(int(x) for x in a.cpu_grid.split(",")) if a.cpu_grid else (mgalx, my, mgalz)
         procs = max(1, min(a.cpu_procs, os.cpu_count() or 1))
-        v, sample = cpu_baseline((mgalx, my, mgalz), cg, a.Re, a.explicit, seed, procs)
+        v, sample = cpu_baseline((mgalx, my, mgalz), cg, a.Re, a.explicit, seed, procs,
+                                 a.cpu_steps)
         cpu = {"value": v, "unit": "steps/s", "cores": procs, "kind": "port",
                "sample": sample + " (numpy FFT and line solves)"}
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
