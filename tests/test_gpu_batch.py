@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 CASES = [((48, 385, 48), 1500.0, {}), ((32, 33, 32), 500.0, {}), ((48, 49, 48), 3000.0, {"explicit": 0}),
          ((64, 65, 64), 1200.0, {}), ((96, 97, 96), 2000.0, {}),
          ((128, 49, 128), 1000.0, {"precision": "mixed"}), ((48, 49, 48), 800.0, {"ydealias": 3}),
-         ((32, 150, 32), 700.0, {}), ((64, 64, 64), 900.0, {"explicit": 0, "precision": "mixed"})]
+         ((32, 150, 32), 700.0, {}), ((64, 64, 64), 900.0, {"explicit": 0, "precision": "mixed"}),
+         ((48, 513, 48), 1000.0, {"precision": "mixed"})]
 
 
 def _boxes():

@@ -29,7 +29,8 @@ def _state(grid, seed=77):
 
 
 @pytest.mark.parametrize("variant", [1, 2])
-@pytest.mark.parametrize("grid,explicit", [((32, 33, 32), 1), ((48, 49, 48), 0), ((32, 40, 32), 1)])
+@pytest.mark.parametrize("grid,explicit", [((32, 33, 32), 1), ((48, 49, 48), 0), ((32, 40, 32), 1),
+                                           ((24, 385, 24), 1), ((24, 513, 24), 0)])
 def test_cross_ydealias(variant, grid, explicit):
     import shear_nnf_amd as snf
     from oracle.cross import Cross

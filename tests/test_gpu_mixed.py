@@ -37,7 +37,8 @@ def _stats(hist):
 
 
 @pytest.mark.parametrize("grid,explicit", [((128, 49, 128), 1), ((192, 33, 128), 1),
-                                           ((128, 49, 256), 0), ((512, 33, 512), 1)])
+                                           ((128, 49, 256), 0), ((512, 33, 512), 1),
+                                           ((128, 513, 128), 1), ((128, 513, 128), 0)])
 def test_mixed_agrees_with_fp64(grid, explicit):
     f64, h64, st64 = _run(grid, "fp64", 10, explicit=explicit)
     fmx, hmx, stmx = _run(grid, "mixed", 10, explicit=explicit)
