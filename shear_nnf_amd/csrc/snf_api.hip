@@ -185,6 +185,7 @@ struct snf_handle {
   bool yfft = false;
   bool ysym_on = false;   // multiplier tables maintained (spectral y or filter)
   bool ymfma = false;     // line lengths without a radix plan: DFT on the matrix cores
+  bool y32 = false;       // mixed mode: explicit y substeps with FP32 line transforms
   cplx* twy = nullptr;
   cplx* tpy = nullptr;                  // pass-major twiddles of the y plan
   cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
@@ -595,6 +596,7 @@ YLaunch ylaunch(const snf_handle* h, size_t lds, hipStream_t s) {
   c.lds = lds;
   c.explicit_visc = h->p.explicit_visc;
   c.io32 = h->g.io32;
+  c.y32 = h->y32;
   return c;
 }
 // y line blocks: one kz chunk of one kx each, and one for U, W
@@ -1150,6 +1152,10 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // the mixed precision mode with a spectral y stage stores the six
     // inter-stage fields as FP32 (the banded sweeps read them as FP64)
     h->g.io32 = (h->mixed && h->yfft) ? 3 : 0;
+    // and transforms the lines of its explicit substeps in FP32 (radix
+    // plans; SNF_Y32=0 keeps them FP64)
+    const char* e32 = getenv("SNF_Y32");
+    h->y32 = h->g.io32 && !(e32 && std::string(e32) == "0");
     // the direct DFT of line lengths without a radix plan: on the matrix
     // cores (YMfma) or as per-wave sums (YDirect); SNF_Y_DFT=mfma|direct
     const char* ed = getenv("SNF_Y_DFT");

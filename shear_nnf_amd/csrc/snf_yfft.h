@@ -145,39 +145,40 @@ template <> struct OddRoots<19> {
       -0.8371664782625285748061, -0.6142127126896678174443, -0.3246994692046834874076};
 };
 
-template <int R, int SIGN>
-__host__ __device__ __forceinline__ void yf_dft_odd(cplx* v) {
+template <int R, int SIGN, class C>
+__host__ __device__ __forceinline__ void yf_dft_odd(C* v) {
+  typedef typename wf_real<C>::T Rt;
   constexpr int H = (R - 1) / 2;
-  cplx a[H + 1], b[H + 1];
-  cplx y0 = v[0];
+  C a[H + 1], b[H + 1];
+  C y0 = v[0];
 #pragma unroll
   for (int j = 1; j <= H; ++j) {
     a[j] = wf_add(v[j], v[R - j]);
     b[j] = wf_sub(v[j], v[R - j]);
     y0 = wf_add(y0, a[j]);
   }
-  const cplx x0 = v[0];
+  const C x0 = v[0];
   v[0] = y0;
 #pragma unroll
   for (int k = 1; k <= H; ++k) {
-    cplx re = x0, im = make_double2(0.0, 0.0);
+    C re = x0, im = wf_mk((Rt)0, (Rt)0);
 #pragma unroll
     for (int j = 1; j <= H; ++j) {
       const int t = (j * k) % R;
-      const double cc = OddRoots<R>::c[t], ss = OddRoots<R>::s[t];
+      const Rt cc = (Rt)OddRoots<R>::c[t], ss = (Rt)OddRoots<R>::s[t];
       re.x += cc * a[j].x;
       re.y += cc * a[j].y;
       im.x += ss * b[j].x;
       im.y += ss * b[j].y;
     }
-    const cplx si = wf_si<SIGN>(im);   // SIGN * i * im
+    const C si = wf_si<SIGN>(im);   // SIGN * i * im
     v[k] = wf_add(re, si);
     v[R - k] = wf_sub(re, si);
   }
 }
 
-template <int R, int SIGN>
-__host__ __device__ __forceinline__ void yf_dft(cplx* v) {
+template <int R, int SIGN, class C>
+__host__ __device__ __forceinline__ void yf_dft(C* v) {
   if constexpr (R % 2 == 1 && R >= 5)
     yf_dft_odd<R, SIGN>(v);
   else
@@ -189,12 +190,12 @@ __host__ __device__ __forceinline__ void yf_dft(cplx* v) {
 // from tw[OFF + (r - 1) Ns + b%Ns]
 // PT >= 0: each output p >= PNs is multiplied by tw[PT + p - PNs], the
 // twiddle of the next pass (a spread pass, whose nb is PNs) at its input p.
-template <int R, int SIGN, int N, int Ns, int OFF, int PT = -1, int PNs = 0, class Col>
-__device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
+template <int R, int SIGN, int N, int Ns, int OFF, int PT = -1, int PNs = 0, class Col, class C>
+__device__ __forceinline__ void yf_pass(C* T, Col col, const C* tw) {
   constexpr int nb = N / R;
   constexpr int MB = (nb + 63) / 64;       // butterflies per lane
   const int lane = threadIdx.x & 63;
-  cplx v[MB][R];
+  C v[MB][R];
 #pragma unroll
   for (int q = 0; q < MB; ++q) {
     const int b = lane + 64 * q;
@@ -202,7 +203,7 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
       const int k = b % Ns;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        cplx x = T[col(b + r * nb)];
+        C x = T[col(b + r * nb)];
         if (Ns > 1 && r > 0) x = wf_tw<SIGN>(x, tw[OFF + (r - 1) * Ns + k]);
         v[q][r] = x;
       }
@@ -219,7 +220,7 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int p = o + r * Ns;
-        cplx y = v[q][r];
+        C y = v[q][r];
         if (PT >= 0 && p >= PNs) y = wf_tw<SIGN>(y, tw[PT + p - PNs]);
         T[col(p)] = y;
       }
@@ -242,31 +243,33 @@ __device__ __forceinline__ void yf_pass(cplx* T, Col col, const cplx* tw) {
 // same slots b + r nb.  The pass's twiddles are applied by the previous
 // pass as it writes (yf_pass with PT >= 0): input b + r nb takes
 // tw[OFF + (r - 1) nb + b] = tw[OFF + (b + r nb) - nb].
-template <int R, int SIGN, int N, class Col>
-__device__ __forceinline__ void yf_pass_spread(cplx* T, Col col, const cplx* tw) {
+template <int R, int SIGN, int N, class Col, class C>
+__device__ __forceinline__ void yf_pass_spread(C* T, Col col, const C* tw) {
+  typedef typename wf_real<C>::T Rt;
   constexpr int H = (R - 1) / 2, K = (H + 1) / 2;
   constexpr int nb = N / R;
   constexpr int NR = (nb + 31) / 32;         // rounds of 32 butterflies
   const int lane = threadIdx.x & 63;
   const int g = lane >> 1, h = lane & 1;
-  const cplx* mt = tw + N + h * K * H;       // this lane's multipliers [k'][j]
+  const C* mt = tw + N + h * K * H;          // this lane's multipliers [k'][j]
+  const C zero = wf_mk((Rt)0, (Rt)0);
 #pragma unroll
   for (int t = 0; t < NR; ++t) {
     const int b = g + 32 * t;
     const bool act = b < nb;
-    cplx re[K], im[K], sum = mkc(0.0, 0.0), x0 = mkc(0.0, 0.0);
+    C re[K], im[K], sum = zero, x0 = zero;
 #pragma unroll
-    for (int q = 0; q < K; ++q) re[q] = im[q] = mkc(0.0, 0.0);
+    for (int q = 0; q < K; ++q) re[q] = im[q] = zero;
     if (act) {
       x0 = T[col(b)];
 #pragma unroll
       for (int j = 1; j <= H; ++j) {
-        const cplx xp = T[col(b + j * nb)], xm = T[col(b + (R - j) * nb)];
-        const cplx aj = wf_add(xp, xm), bj = wf_sub(xp, xm);
+        const C xp = T[col(b + j * nb)], xm = T[col(b + (R - j) * nb)];
+        const C aj = wf_add(xp, xm), bj = wf_sub(xp, xm);
         sum = wf_add(sum, aj);
 #pragma unroll
         for (int q = 0; q < K; ++q) {
-          const cplx w = mt[q * H + j - 1];     // exp(-2 pi i j k / R)
+          const C w = mt[q * H + j - 1];        // exp(-2 pi i j k / R)
           re[q].x += w.x * aj.x;
           re[q].y += w.x * aj.y;
           im[q].x -= w.y * bj.x;
@@ -280,7 +283,7 @@ __device__ __forceinline__ void yf_pass_spread(cplx* T, Col col, const cplx* tw)
       for (int q = 0; q < K; ++q) {
         const int k = h * K + q + 1;
         if (k <= H) {
-          const cplx base = wf_add(x0, re[q]), si = wf_si<SIGN>(im[q]);   // SIGN * i * im
+          const C base = wf_add(x0, re[q]), si = wf_si<SIGN>(im[q]);   // SIGN * i * im
           T[col(b + k * nb)] = wf_add(base, si);
           T[col(b + (R - k) * nb)] = wf_sub(base, si);
         }
@@ -321,10 +324,10 @@ template <int R, int R2, int... Rs> struct YLast<R, R2, Rs...> {
 };
 
 // transform of column w in place (natural order in and out), unnormalised
-template <int SIGN, int N, int Ns, int OFF, class Col>
-__device__ __forceinline__ void yf_passes(cplx*, Col, const cplx*) {}
-template <int SIGN, int N, int Ns, int OFF, class Col, int R, int... Rs>
-__device__ __forceinline__ void yf_passes(cplx* T, Col col, const cplx* tw) {
+template <int SIGN, int N, int Ns, int OFF, class Col, class C>
+__device__ __forceinline__ void yf_passes(C*, Col, const C*) {}
+template <int SIGN, int N, int Ns, int OFF, class Col, class C, int R, int... Rs>
+__device__ __forceinline__ void yf_passes(C* T, Col col, const C* tw) {
   constexpr int NOFF = Ns > 1 ? OFF + (R - 1) * Ns : 0;   // the next pass's twiddles
   if constexpr (yf_spread_radix(R)) {
     static_assert(sizeof...(Rs) == 0, "a spread pass runs last");
@@ -334,12 +337,13 @@ __device__ __forceinline__ void yf_passes(cplx* T, Col col, const cplx* tw) {
   } else {
     yf_pass<R, SIGN, N, Ns, OFF>(T, col, tw);
   }
-  yf_passes<SIGN, N, Ns * R, NOFF, Col, Rs...>(T, col, tw);
+  yf_passes<SIGN, N, Ns * R, NOFF, Col, C, Rs...>(T, col, tw);
 }
-template <int SIGN, class Col, int... Rs>
-__device__ __forceinline__ void yf_fft(cplx* T, Col col, const cplx* tw) {
+// element type C: cplx, or cplxf (with an FP32 table tw)
+template <int SIGN, class Col, class C, int... Rs>
+__device__ __forceinline__ void yf_fft(C* T, Col col, const C* tw) {
   wf_sync();   // the column was just written lane by lane
-  yf_passes<SIGN, YPlanT<Rs...>::N, 1, 0, Col, Rs...>(T, col, tw);
+  yf_passes<SIGN, YPlanT<Rs...>::N, 1, 0, Col, C, Rs...>(T, col, tw);
 }
 
 // Direct DFT of column col in place (natural order in and out,
@@ -493,7 +497,7 @@ __device__ __forceinline__ void yf_rader(cplx* T, Col col, const cplx* tw) {
     const int q = lane + 64 * t;
     if (q < M) T[sc(q)] = u[t];
   }
-  yf_fft<-1, ShiftCol<Col>, Rs...>(T, sc, tw);
+  yf_fft<-1, ShiftCol<Col>, cplx, Rs...>(T, sc, tw);
   const cplx* V = tw + (SIGN < 0 ? M : 2 * M);
   cplx X0 = x0;
 #pragma unroll
@@ -509,7 +513,7 @@ __device__ __forceinline__ void yf_rader(cplx* T, Col col, const cplx* tw) {
       T[sc(m)] = W;
     }
   }
-  yf_fft<1, ShiftCol<Col>, Rs...>(T, sc, tw);
+  yf_fft<1, ShiftCol<Col>, cplx, Rs...>(T, sc, tw);
 #pragma unroll
   for (int t = 0; t < QM; ++t) {
     const int p = lane + 64 * t;
@@ -537,6 +541,7 @@ __device__ __forceinline__ void yf_rader(cplx* T, Col col, const cplx* tw) {
 template <int... Rs> struct YFFT {
   static constexpr int Q = (YPlanT<Rs...>::N + 63) / 64;
   static constexpr bool direct = false;
+  static constexpr bool f32 = true;   // transforms also on FP32 columns (cplxf)
   static constexpr int NC = YPlanT<Rs...>::N;   // line length (compile time)
   // multiplier rows read from the global table: beyond 448 points the rows
   // staged in LDS would leave one block per CU
@@ -548,19 +553,20 @@ template <int... Rs> struct YFFT {
   __host__ __device__ static constexpr int tw_len(int n) { return n + RS; }
   // active = false: this wave's column stays as it is (the call keeps the
   // waves of a block in step for the block-collective plan)
-  template <int SIGN, class Col>
-  __device__ __forceinline__ static void fft(cplx* T, Col c, const cplx* tw, int,
+  template <int SIGN, class Col, class C>
+  __device__ __forceinline__ static void fft(C* T, Col c, const C* tw, int,
                                              bool active = true) {
-    if (active) yf_fft<SIGN, Col, Rs...>(T, c, tw);
+    if (active) yf_fft<SIGN, Col, C, Rs...>(T, c, tw);
   }
-  template <int SIGN, class Col>   // one wave on its own (the 00 lines)
-  __device__ __forceinline__ static void fft_wave(cplx* T, Col c, const cplx* tw, int n) {
+  template <int SIGN, class Col, class C>   // one wave on its own (the 00 lines)
+  __device__ __forceinline__ static void fft_wave(C* T, Col c, const C* tw, int n) {
     fft<SIGN>(T, c, tw, n);
   }
 };
 template <int P, int G, int... Rs> struct YRader {
   static constexpr int Q = (P + 63) / 64;
   static constexpr bool direct = false;
+  static constexpr bool f32 = false;
   static constexpr int NC = P;
   static constexpr bool sym_gl = false;
   __host__ __device__ static constexpr int tw_len(int) { return 4 * (P - 1); }
@@ -577,6 +583,7 @@ template <int P, int G, int... Rs> struct YRader {
 template <int QM> struct YDirect {
   static constexpr int Q = QM;
   static constexpr bool direct = true;
+  static constexpr bool f32 = false;
   static constexpr int NC = 0;                  // line length: my, at run time
   static constexpr bool sym_gl = false;
   __host__ __device__ static constexpr int tw_len(int n) { return n; }
@@ -595,6 +602,7 @@ template <int QM> struct YDirect {
 template <int QM> struct YMfma {
   static constexpr int Q = QM;
   static constexpr bool direct = true;
+  static constexpr bool f32 = false;
   static constexpr int NC = 0;
   static constexpr bool sym_gl = false;
   __host__ __device__ static constexpr int tw_len(int n) { return n; }

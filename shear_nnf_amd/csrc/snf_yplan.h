@@ -14,6 +14,7 @@ struct YLaunch {
   size_t lds;          // dynamic LDS of the plan's kernels (bytes per block)
   int explicit_visc;   // 1: explicit viscous terms, 0: implicit
   int io32;            // storage of the inter-stage fields (Geom::io32)
+  int y32;             // mixed mode: explicit substeps with FP32 line transforms
 };
 
 struct YPlanOps {

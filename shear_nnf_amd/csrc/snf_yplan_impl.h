@@ -2,6 +2,7 @@
 // by the plan translation units snf_yplans_*.hip through SNF_YPLAN.
 #pragma once
 #include "snf_yfk.h"
+#include "snf_yfk32.h"
 #include "snf_yplan.h"
 
 template <class PL>
@@ -21,7 +22,18 @@ struct YPlanImpl {
     else
       k_yf_implicit<PL, IO><<<nb, 64 * YF_LINES, c.lds, c.stream>>>(a, rk);
   }
+  // the mixed mode's explicit substep with FP32 transforms (radix plans)
+  static bool use32(const YLaunch& c) {
+    if constexpr (PL::f32) return c.y32 && c.io32 && c.explicit_visc == 1;
+    return false;
+  }
   static void substep(const YLaunch& c, unsigned nb, const YArgs& a, int rk) {
+    if constexpr (PL::f32) {
+      if (use32(c)) {
+        k_yf_explicit32<PL><<<nb, 64 * YF_LINES, Y32<PL>::lds_bytes, c.stream>>>(a, rk);
+        return;
+      }
+    }
     if (c.io32)
       substep_io<3>(c, nb, a, rk);
     else
@@ -49,6 +61,14 @@ struct YPlanImpl {
     }
   }
   static void table(const YLaunch& c, unsigned nb, const YArgs* t, int n, int rk, int dev) {
+    if constexpr (PL::f32) {
+      if (use32(c)) {
+        lds_attr((const void*)k_yf_explicit32_tab<PL>, Y32<PL>::lds_bytes, dev);
+        if (n == 0) return;
+        k_yf_explicit32_tab<PL><<<dim3(nb, 1, n), 64 * YF_LINES, Y32<PL>::lds_bytes, c.stream>>>(t, rk);
+        return;
+      }
+    }
     if (c.io32)
       table_io<3>(c, nb, t, n, rk, dev);
     else
@@ -60,6 +80,7 @@ struct YPlanImpl {
                           (const void*)k_yf_implicit<PL, 3>, (const void*)k_yf_first<PL, 3>,
                           (const void*)k_yf_filter<PL>})
       lds_attr(k, lds, dev);
+    if constexpr (PL::f32) lds_attr((const void*)k_yf_explicit32<PL>, Y32<PL>::lds_bytes, dev);
   }
   static const YPlanOps* ops() {
     static const YPlanOps o = {lds_bytes, substep, first, filter, table, lds_attrs};
