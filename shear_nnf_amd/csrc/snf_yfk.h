@@ -226,6 +226,35 @@ __device__ __forceinline__ void yf_flush_blk(const cplx* T, cplx* W, int N) {
   __syncthreads();
 }
 
+// the same for a chunk stored FP32 when io32 (cplxf element e at W + e)
+template <int Q>
+__device__ __forceinline__ void yf_issue_blk_io(cplx (&r)[Q], const cplx* W, int N, int io32) {
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = threadIdx.x + q * 64 * YF_LINES;
+    if (e < N * YF_LINES)
+      r[q] = io32 ? mkc(yf_raw64(W, e), 0.0) : gld(W + e);
+    else
+      r[q] = io32 ? mkc(__longlong_as_double(0LL), 0.0) : mkc(0.0, 0.0);
+  }
+}
+__device__ __forceinline__ void yf_flush_blk_io(const cplx* T, cplx* W, int N, int io32) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    st_io(W, e, T[TileCol{c}(j)], io32);
+  }
+  __syncthreads();
+}
+__device__ __forceinline__ void yf_fill_blk_io(cplx* T, const cplx* W, int N, int io32) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
+    const int j = e / YF_LINES, c = e - j * YF_LINES;
+    T[TileCol{c}(j)] = ld_io(W, e, io32);
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void yf_fill_blk(cplx* T, const cplx* W, int N) {
   __syncthreads();
   for (int e = threadIdx.x; e < N * YF_LINES; e += blockDim.x) {
@@ -250,6 +279,9 @@ template <int Q> struct YPf<Q, true> {
     yf_issue_io(r, F, P, L0, nv, N, io32);
   }
   __device__ __forceinline__ void issue_blk(const cplx* W, int N) { yf_issue_blk(r, W, N); }
+  __device__ __forceinline__ void issue_blk_io(const cplx* W, int N, int io32) {
+    yf_issue_blk_io(r, W, N, io32);
+  }
   __device__ __forceinline__ void commit(cplx* T, int N) { yf_commit(T, r, N); }
   __device__ __forceinline__ void commit_io(cplx* T, int N, int io32) { yf_commit_io(T, r, N, io32); }
 };
@@ -272,6 +304,7 @@ template <int Q> struct YPf<Q, false> {
     F = W;
     blk = true;
   }
+  __device__ __forceinline__ void issue_blk_io(const cplx* W, int N, int) { issue_blk(W, N); }
   __device__ __forceinline__ void commit(cplx* T, int N) {
     if (blk)
       yf_fill_blk(T, F, N);
@@ -279,7 +312,10 @@ template <int Q> struct YPf<Q, false> {
       yf_fill(T, F, P, L0, nv, N);
   }
   __device__ __forceinline__ void commit_io(cplx* T, int N, int io32) {
-    yf_fill_io(T, F, P, L0, nv, N, io32);
+    if (blk)
+      yf_fill_blk_io(T, F, N, io32);
+    else
+      yf_fill_io(T, F, P, L0, nv, N, io32);
   }
 };
 // whether a plan's kernels keep the second field transfer in flight

@@ -4,16 +4,17 @@
 // In the mixed mode the x-z stage runs in FP32 and the six inter-stage
 // fields (rhg, rhv, rdome in; v, dv, dome out) are stored FP32, so the
 // transforms that produce them, and the viscous terms added to the FP32
-// right-hand sides, need no more than FP32 either.  What stays FP64: the
-// state vor, phi, U, W and its update vn = v + dt (gamma h + acc), the RK
-// accumulators (stored FP64), the phase tables and the moving-frame maps
-// applied to the state.  The transforms run on FP32 columns against an FP32
+// right-hand sides, need no more than FP32 either; so do the RK
+// accumulators (right-hand sides kept within a step), stored FP32.  What
+// stays FP64: the state vor, phi, U, W and its update vn = v + dt (gamma h +
+// acc), the phase tables and the moving-frame maps applied to the state.  The transforms run on FP32 columns against an FP32
 // copy of the plan's table: half the LDS bytes of the FP64 transforms and
 // fewer registers.  The 00 lines (the mean flow U, W) keep FP64 transforms.
 //
 // LDS of a block: the FP64 transfer tile [N][YF_LINES] (the state passes
 // through it unrounded), two FP32 columns per wave, the FP32 table, the four
-// FP64 phase tables.  The multiplier rows are read from the global table.
+// FP64 phase tables, the FP64 multiplier rows where two blocks per CU still
+// fit (else they are read from the global table).
 // Radix plans only (PL::f32).
 #pragma once
 #include "snf_yfk.h"
@@ -24,11 +25,31 @@ template <class PL>
 struct Y32 {
   static constexpr int N = PL::NC;
   static constexpr int RTE = (PL::tw_len(PL::NC) + 1) & ~1;   // FP32 table entries (even)
-  static constexpr size_t lds_bytes = (size_t)N * YF_LINES * sizeof(cplx) +
-                                      (size_t)2 * YF_LINES * N * sizeof(cplxf) +
-                                      (size_t)RTE * sizeof(cplxf) +
-                                      (size_t)4 * YPH_N(PL::Q) * sizeof(cplx);
+  static constexpr size_t base = (size_t)N * YF_LINES * sizeof(cplx) +
+                                 (size_t)2 * YF_LINES * N * sizeof(cplxf) +
+                                 (size_t)RTE * sizeof(cplxf) +
+                                 (size_t)4 * YPH_N(PL::Q) * sizeof(cplx);
+  // the two multiplier rows (FP64, [2][N] each) staged in LDS where two
+  // blocks still fit a CU (the kernel runs at 2 waves per SIMD), else read
+  // from the global table
+  static constexpr size_t rows = (size_t)4 * N * sizeof(cplx);
+  static constexpr bool rows_lds = 2 * (base + rows) <= 160 * 1024;
+  static constexpr size_t lds_bytes = base + (rows_lds ? rows : 0);
 };
+// row of kx i of the multiplier table tab, staged into dst (2 N entries)
+// or read in place
+template <bool LDS>
+__device__ __forceinline__ YSymRow<!LDS> y32_row(cplx* dst, const cplx* tab, int i, int N) {
+  if constexpr (!LDS) {
+    return YSymRow<true>{tab + (long)i * N * YS, N};
+  } else {
+    for (int t = threadIdx.x; t < 2 * N; t += blockDim.x) {
+      const long src = ((long)i * N + (t >> 1)) * YS + (t & 1);
+      dst[(t & 1) * N + (t >> 1)] = gld(tab + src);
+    }
+    return YSymRow<false>{dst, N};
+  }
+}
 
 template <class PL>
 __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
@@ -122,19 +143,22 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
     return;
   }
 
-  cplx* vwk_blk = a.vorwk + L0;
-  cplx* pwk_blk = a.phiwk + L0;
+  // the RK accumulators hold FP32-accurate right-hand sides and live for
+  // one step: stored FP32, the block's chunk at cplxf offset L0
+  cplx* vwk_blk = reinterpret_cast<cplx*>(reinterpret_cast<cplxf*>(a.vorwk) + L0);
+  cplx* pwk_blk = reinterpret_cast<cplx*>(reinterpret_cast<cplxf*>(a.phiwk) + L0);
   const int k = min(YF_LINES * ch + w, g.mz - 1);
   const double kx = g.alp * (double)i, kzv = kz_of(k, g.nz, g.mz, g.gam);
   const double k2 = kx * kx + kzv * kzv;
   const bool is00 = (i == 0 && k == 0);
   const cplx so = gld(a.shp_old + i), sn = gld(a.shp_new + i);
   const double tho = atan2(so.y, so.x) * invN, thn = atan2(sn.y, sn.x) * invN;
-  const YSymRow<true> syo{a.ysym_old + (long)i * N * YS, N};
-  const YSymRow<true> syn{a.ysym_new + (long)i * N * YS, N};
   cplx* phn = pho + YPH_N(Q);
   cplx* pm1 = phn + YPH_N(Q);
   cplx* pm2 = pm1 + YPH_N(Q);
+  cplx* rows = pm2 + YPH_N(Q);
+  const auto syo = y32_row<Y32<PL>::rows_lds>(rows, a.ysym_old, i, N);
+  const auto syn = y32_row<Y32<PL>::rows_lds>(rows + 2 * N, a.ysym_new, i, N);
   {
     const double pa[4] = {tho, thn, kx * f1 * g.dy, kx * f2 * g.dy};
     const double pa0[4] = {0.0, 0.0, -0.5 * kx * f1 * g.Ly, -0.5 * kx * f2 * g.Ly};
@@ -146,7 +170,7 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
   YPf<Q, YPfLive<PL>::value> pfb;
   yf_commit(T, pv, N);                     // vor
   yf_issue_io(pfa, a.rhg, P, L0, nv, N, 1);
-  if (rk > 1) pfb.issue_blk(vwk_blk, N);
+  if (rk > 1) pfb.issue_blk_io(vwk_blk, N, 1);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -181,7 +205,7 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
       if (j < N) A[j] = y32_f(cadd(T[tc(j)], cmul(to_d(A[j]), e)));   // hg
     }
   }
-  if (rk > 1) pfb.commit(T, N);            // vorwk
+  if (rk > 1) pfb.commit_io(T, N, 1);      // vorwk
   pfb.issue(a.phi, P, L0, nv, N);
   {
     YPh<false> m1(pm1, lane);
@@ -201,7 +225,7 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
     }
   }
   // the third substep's accumulators are never read (a step starts afresh)
-  if (rk != 3) yf_flush_blk(T, vwk_blk, N);
+  if (rk != 3) yf_flush_blk_io(T, vwk_blk, N, 1);
   {
     YPh<true> ph(phn, lane);
 #pragma unroll
@@ -252,7 +276,7 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
     if (m < N) A[m] = wf_mul(A[m], y32_f(crmul(-invN, syo.d1(m))));
   }
   pfb.commit(T, N);                        // phi
-  if (rk > 1) pfb.issue_blk(pwk_blk, N);
+  if (rk > 1) pfb.issue_blk_io(pwk_blk, N, 1);
   {
     YPh<true> ph(pho, lane);
 #pragma unroll
@@ -286,7 +310,7 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
       if (j < N) A[j] = y32_f(cadd(T[tc(j)], cmul(to_d(A[j]), e)));   // hv
     }
   }
-  if (rk > 1) pfb.commit(T, N);            // phiwk
+  if (rk > 1) pfb.commit_io(T, N, 1);      // phiwk
   {
     YPh<false> m1(pm1, lane);
     YPh<false> m2(pm2, lane);
@@ -304,7 +328,7 @@ __device__ __forceinline__ void k_yf_explicit32_body(YArgs a, int rk) {
       }
     }
   }
-  if (rk != 3) yf_flush_blk(T, pwk_blk, N);
+  if (rk != 3) yf_flush_blk_io(T, pwk_blk, N, 1);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int j = lane + 64 * q;
