@@ -1152,10 +1152,10 @@ int snf_create(const snf_params* p, snf_handle** out) {
     // the mixed precision mode with a spectral y stage stores the six
     // inter-stage fields as FP32 (the banded sweeps read them as FP64)
     h->g.io32 = (h->mixed && h->yfft) ? 3 : 0;
-    // and transforms the lines of its explicit substeps in FP32 (radix
-    // plans; SNF_Y32=0 keeps them FP64)
+    // SNF_Y32=1: the explicit substeps of such a handle also transform
+    // their lines in FP32 (radix plans; faster, less accurate statistics)
     const char* e32 = getenv("SNF_Y32");
-    h->y32 = h->g.io32 && !(e32 && std::string(e32) == "0");
+    h->y32 = h->g.io32 && e32 && std::string(e32) == "1";
     // the direct DFT of line lengths without a radix plan: on the matrix
     // cores (YMfma) or as per-wave sums (YDirect); SNF_Y_DFT=mfma|direct
     const char* ed = getenv("SNF_Y_DFT");

@@ -81,3 +81,30 @@ def test_box_tables_equal_separate_advances(nsteps):
         for x, y in zip(b.get_state(), s.get_state()):
             assert np.array_equal(x, y)
         assert np.array_equal(np.array(b.history), np.array(s.history))
+
+
+def test_batch_fp32_y_transforms(monkeypatch):
+    """SNF_Y32=1: mixed boxes whose explicit y substeps transform in FP32, a
+    size class of two (box tables) and a box of its own, bitwise against
+    their own advances"""
+    import shear_nnf_amd as snf
+    monkeypatch.setenv("SNF_Y32", "1")
+    cases = [((48, 385, 48), 1500.0), ((64, 65, 64), 1200.0), ((64, 65, 64), 2200.0),
+             ((48, 513, 48), 1000.0)]
+
+    def boxes():
+        out = []
+        for q, (grid, Re) in enumerate(cases):
+            b = snf.ShearBox(*grid, Re, alp=ALP, gam=GAM, Ly=LY, nhist=1, precision="mixed")
+            b.set_state(*box_ic(*grid, seed=20260200 + q))
+            out.append(b)
+        return out
+
+    batch, single = boxes(), boxes()
+    st = snf.advance_batch(batch, 5)
+    for b, s, r in zip(batch, single, st):
+        r1 = s.cross(5)
+        assert r.steps_done == r1.steps_done == 5 and r.time == r1.time
+        for x, y in zip(b.get_state(), s.get_state()):
+            assert np.array_equal(x, y)
+        assert np.array_equal(np.array(b.history), np.array(s.history))

@@ -1,15 +1,17 @@
 """The mixed precision mode (SNF_PREC_MIXED: FP32 x-z transforms, hybrid
-planes and u x omega products, FP32 line transforms of the explicit y
-substeps on radix plans; FP64 state, KMM reconstruction, mean-flow lines, RK
-update and statistics sums) against the FP64 path and the oracle.
+planes and u x omega products; FP64 state, KMM reconstruction, y operators,
+RK update and statistics sums) against the FP64 path and the oracle.
 
-Bars: the FP32 stages perturb each substep's right-hand side at the FP32
-rounding level (~1e-7 relative), so fields after a few steps agree with FP64
-to <= 1e-5 relative L2, energy and dissipation to <= 1e-5 relative record by
-record, and the Reynolds stress <uv> (which changes sign, so that a record
-near zero has no relative accuracy) to <= 1e-5 of its largest magnitude over
-the run.  With SNF_Y32=0 (FP64 y transforms) the Reynolds stress also meets
-the record-by-record bar (test_mixed_fp64_y_transforms)."""
+Bars: the FP32 nonlinear stage perturbs each substep's right-hand side at the
+FP32 rounding level (~1e-7 relative), so fields after a few steps agree with
+FP64 to <= 1e-5 relative L2 and the UNER statistics (energy, Reynolds stress,
+dissipation) to <= 1e-5 relative, record by record.
+
+SNF_Y32=1 (FP32 line transforms in the explicit y substeps as well,
+test_mixed_fp32_y_transforms): the same field bar, energy and dissipation
+record by record, and the Reynolds stress <uv>, which changes sign (a record
+near zero has no relative accuracy), to <= 1e-5 of its largest magnitude over
+the run."""
 import math
 
 import numpy as np
@@ -59,20 +61,22 @@ def test_mixed_agrees_with_fp64(grid, explicit):
     for a, b in zip(fmx, f64):
         e = rel_l2(a, b)
         assert 0.0 < e < MIX_TOL        # FP32 rounding is there, and small
-    _check_stats(_stats(h64), _stats(hmx))
+    _check_stats(_stats(h64), _stats(hmx), pointwise=True)
     assert abs(stmx.time - st64.time) < 1e-6 * st64.time
 
 
-@pytest.mark.parametrize("grid", [(192, 33, 128), (128, 513, 128)])
-def test_mixed_fp64_y_transforms(grid, monkeypatch):
-    """SNF_Y32=0: the mixed mode with FP64 line transforms in the y stage
-    (the FP32 x-z stage alone), every statistic record by record"""
+@pytest.mark.parametrize("grid", [(128, 49, 128), (192, 33, 128), (128, 513, 128),
+                                  (64, 65, 64), (32, 33, 32)])
+def test_mixed_fp32_y_transforms(grid, monkeypatch):
+    """SNF_Y32=1: the explicit y substeps with FP32 line transforms (radix
+    plans 7*7, 11*3, 9*3*19, 13*5; FP64 state, update and mean-flow lines)"""
     f64, h64, _ = _run(grid, "fp64", 10)
-    monkeypatch.setenv("SNF_Y32", "0")
-    fmx, hmx, _ = _run(grid, "mixed", 10)
+    monkeypatch.setenv("SNF_Y32", "1")
+    fmx, hmx, st = _run(grid, "mixed", 10)
+    assert st.steps_done == 10
     for a, b in zip(fmx, f64):
         assert 0.0 < rel_l2(a, b) < MIX_TOL
-    _check_stats(_stats(h64), _stats(hmx), pointwise=True)
+    _check_stats(_stats(h64), _stats(hmx))
 
 
 def test_mixed_agrees_with_oracle():
@@ -118,7 +122,7 @@ def test_mixed_stockham_sizes_agree_with_fp64(grid, explicit):
     assert stmx.steps_done == st64.steps_done == 10
     for a, b in zip(fmx, f64):
         assert 0.0 < rel_l2(a, b) < MIX_TOL
-    _check_stats(_stats(h64), _stats(hmx))
+    _check_stats(_stats(h64), _stats(hmx), pointwise=True)
 
 
 def test_mixed_stockham_agrees_with_oracle():
