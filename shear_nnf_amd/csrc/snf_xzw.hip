@@ -26,6 +26,10 @@
 #define WC_WAVES 4
 #define UMAX_SLOTS 64   // umax accumulators of the x-row kernel, folded by k_scalar
 
+// kz modes of a plane of Nz = mgalz points: mz = 2 (Nz / 3) - 1, nz = (mz - 1) / 2
+// (compile-time constants in the kernels of this file, where Nz = 64 R)
+__host__ __device__ constexpr int wz_mz(int Nz) { return 2 * (Nz / 3) - 1; }
+__host__ __device__ constexpr int wz_nz(int Nz) { return Nz / 3 - 1; }
 // padded z position (0..Nz-1) of lane l, value m -> kz index, or -1 (gap)
 __device__ __forceinline__ int wz_k(int pos, int nz, int mz, int Nz) {
   if (pos <= nz) return pos;
@@ -68,7 +72,7 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
   extern __shared__ double4 xzw_smem[];
   H* lds = reinterpret_cast<H*>(xzw_smem);
   H* rt = lds;
-  const int Nz = 64 * RZ;
+  constexpr int Nz = 64 * RZ, NZ = wz_nz(Nz), MZ = wz_mz(Nz);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   H* buf = lds + Nz + wave * W::LDS;
   stage_roots(rt, twz, Nz);
@@ -81,12 +85,12 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
   H d[6][RZ];
 #pragma unroll
   for (int m = 0; m < RZ; ++m) {
-    const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
+    const int k = wz_k(lane + 64 * m, NZ, MZ, Nz);
     if (k >= 0) {
       const long idx = sidx(g, j, i, k);
       PlaneIn q;
-      if (i == 0 && k > g.nz) {   // set_conj (operators.f90:1382-1429)
-        const long cj = sidx(g, j, 0, g.mz - k);
+      if (i == 0 && k > NZ) {     // set_conj (operators.f90:1382-1429)
+        const long cj = sidx(g, j, 0, MZ - k);
         q.phic = cconj(gld(phi + cj));
         q.ome2c = cconj(gld(vor + cj));
       } else {
@@ -129,7 +133,11 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   if (sc->halted) return;
   extern __shared__ double4 xzw_smem[];
   H* lds = reinterpret_cast<H*>(xzw_smem);
-  const int Mx = 64 * RX, Nz = g.mgalz, kxn = g.kxn;
+  // kxn = mgalx / 3 on every plane of this path: a compile-time constant, so
+  // that the dealiasing gaps (bins kxn .. Mx - kxn, zero in and unused out)
+  // drop out of the transforms' first and last stages
+  constexpr int Mx = 64 * RX, kxn = Mx / 3;
+  const int Nz = g.mgalz;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // planes up to 384 points (WIDE) stage all 6 fields in one round and run
   // independent transforms in pairs; the LDS this takes (67.5 KB at 384,
@@ -150,7 +158,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   // every load of the block is issued up front (runs of 4 z values, 64 B),
   // field by field: element i of a field is (kx, z) = (i / WB_ROWS, i % WB_ROWS);
   // the tile then passes through LDS, NFR fields at a time
-  constexpr int KXM = 64 * RX / 3;                                  // kxn bound
+  constexpr int KXM = kxn;
   constexpr int NEF = (KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);   // per field
   const int tile = kxn * WB_ROWS;                                   // per field
   const int i0 = threadIdx.x, kx0 = i0 / WB_ROWS, rr0 = i0 % WB_ROWS;
@@ -184,7 +192,9 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
       for (int m = 0; m < RX; ++m) {
         const int b = lane + 64 * m;
         H s = wf_mk((R)0, (R)0);
-        if (b == 0) {
+        if (64 * m >= kxn && 64 * m + 63 <= Mx - kxn) {
+          // a row of the dealiasing gap: zero
+        } else if (b == 0) {
           const H Ga = Sa[r], Gb = Sb[r];
           s = wf_mk(Ga.x, Gb.x);   // c2r ignores the kx=0 imaginary part
         } else if (b < kxn) {
@@ -255,9 +265,11 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     wfft_t2l<RX, -1>(z3, a3, buf, rt);
   }
   // separate H1, H2 from Z12 = H1 + i H2: needs bin Mx - k from another lane
+  // (only the rows of bins below kxn or above Mx - kxn are read)
   wf_sync();
 #pragma unroll
-  for (int m = 0; m < RX; ++m) buf[lane + 64 * m] = a12[m];
+  for (int m = 0; m < RX; ++m)
+    if (64 * m < kxn || 64 * m + 63 > Mx - kxn) buf[lane + 64 * m] = a12[m];
   wf_sync();
   __syncthreads();   // every wave done with its FFT buffer before the output tile
   H* O = work;       // [3][kxn][WB_ROWS + 1]
@@ -342,7 +354,7 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
   extern __shared__ double4 xzw_smem[];
   H* lds = reinterpret_cast<H*>(xzw_smem);
   H* rt = lds;
-  const int Nz = 64 * RZ;
+  constexpr int Nz = 64 * RZ, NZ = wz_nz(Nz), MZ = wz_mz(Nz);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   H* buf = lds + Nz + wave * W::LDS;
   stage_roots(rt, twz, Nz);
@@ -367,11 +379,11 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
   const double kx2 = kx * kx;
 #pragma unroll
   for (int m = 0; m < RZ; ++m) {
-    const int k = wz_k(lane + 64 * m, g.nz, g.mz, Nz);
+    const int k = wz_k(lane + 64 * m, NZ, MZ, Nz);
     if (k >= 0) {
       const cplx h1 = crmul(norm, to_d(h[0][m])), h2 = crmul(norm, to_d(h[1][m])),
                  h3 = crmul(norm, to_d(h[2][m]));
-      const double kz = kz_of(k, g.nz, g.mz, g.gam);
+      const double kz = kz_of(k, NZ, MZ, g.gam);
       const long idx = sidx(g, j, i, k);
       const cplx v = ld_io(vel, idx, g.io32 & 1);
       st_io(dvordy, idx, cadd(mul_i(kx, h1), mul_i(kz, h3)), g.io32 & 2);
