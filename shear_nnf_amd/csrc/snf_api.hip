@@ -325,8 +325,9 @@ void launch_xzw_A(snf_handle* h) {
 template <int R, class H>
 void launch_xzw_B(snf_handle* h, int rk) {
   const Geom& g = h->g;
-  dim3 gB(g.mgalz / WB_ROWS, g.my);
-  k_xzw_B<R, H><<<gB, 64 * WB_ROWS, h->ldsWB, h->stream>>>(g, reinterpret_cast<const H*>(h->hyb_in),
+  constexpr int rows = wb_rows<H>();
+  dim3 gB(g.mgalz / rows, g.my);
+  k_xzw_B<R, H><<<gB, 64 * rows, h->ldsWB, h->stream>>>(g, reinterpret_cast<const H*>(h->hyb_in),
       reinterpret_cast<H*>(h->hyb_out), h->twx, rk, h->upart, h->phist, h->sc, h->p.nhist);
 }
 template <int R, class H>
@@ -1246,7 +1247,7 @@ int snf_create(const snf_params* p, snf_handle** out) {
       const size_t es = h->mixed ? sizeof(cplxf) : sizeof(cplx);
       h->ldsWA = xzw_lds_A(h->rz, es);
       h->ldsWC = xzw_lds_C(h->rz, es);
-      h->ldsWB = xzw_lds_B(h->rx, g.kxn, es);
+      h->ldsWB = xzw_lds_B(h->rx, g.kxn, es, h->mixed ? wb_rows<cplxf>() : wb_rows<cplx>());
       const void* ka[10] = {(const void*)k_xzw_A<2, cplx>, (const void*)k_xzw_A<3, cplx>,
                            (const void*)k_xzw_A<4, cplx>, (const void*)k_xzw_A<6, cplx>,
                            (const void*)k_xzw_A<2, cplxf>, (const void*)k_xzw_A<3, cplxf>,

@@ -23,6 +23,10 @@
 #ifndef WB_ROWS
 #define WB_ROWS 4
 #endif
+// z-rows of an x-row block: 4 in FP64 (the 4-row tile's 64-byte runs), 8 in
+// FP32 (64-byte runs again; 4 rows of 32 bytes took 0.437 ms against 0.430 at
+// 384x385x384)
+template <class H> __host__ __device__ constexpr int wb_rows() { return sizeof(H) == 8 ? 8 : WB_ROWS; }
 #define WC_WAVES 4
 #define UMAX_SLOTS 64   // umax accumulators of the x-row kernel, folded by k_scalar
 
@@ -123,8 +127,8 @@ k_xzw_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
 }
 
 // ---------------------------------------------------------------------------
-template <int RX, class H>
-__global__ void __launch_bounds__(64 * WB_ROWS)
+template <int RX, class H, int ROWS = wb_rows<H>()>
+__global__ void __launch_bounds__(64 * ROWS)
 k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
         const cplx* __restrict__ twx, int rk, double* __restrict__ upart,
         double* __restrict__ phist, const DevScalars* sc, int nhist) {
@@ -151,17 +155,17 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
   stage_roots(rt, twx, Mx);
   const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
   const int zblk = (int)(lb % gridDim.x), j = (int)(lb / gridDim.x);
-  const int z0 = WB_ROWS * zblk, r = wave;
+  const int z0 = ROWS * zblk, r = wave;
   const long fstride = (long)g.my * kxn * Nz;
   const long rowbase = (long)j * kxn * Nz + z0;   // + kx*Nz + r
   // --- load the 3 field pairs of row r in lane layout (bins l + 64 m) ---
   // every load of the block is issued up front (runs of 4 z values, 64 B),
-  // field by field: element i of a field is (kx, z) = (i / WB_ROWS, i % WB_ROWS);
+  // field by field: element i of a field is (kx, z) = (i / ROWS, i % ROWS);
   // the tile then passes through LDS, NFR fields at a time
   constexpr int KXM = kxn;
-  constexpr int NEF = (KXM * WB_ROWS + 64 * WB_ROWS - 1) / (64 * WB_ROWS);   // per field
-  const int tile = kxn * WB_ROWS;                                   // per field
-  const int i0 = threadIdx.x, kx0 = i0 / WB_ROWS, rr0 = i0 % WB_ROWS;
+  constexpr int NEF = (KXM * ROWS + 64 * ROWS - 1) / (64 * ROWS);   // per field
+  const int tile = kxn * ROWS;                                   // per field
+  const int i0 = threadIdx.x, kx0 = i0 / ROWS, rr0 = i0 % ROWS;
   constexpr int KXS = 64;                                           // kx step per q
   H pre[6][NEF];
 #pragma unroll
@@ -169,11 +173,11 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     const H* src = hin + f * fstride + rowbase + rr0;
 #pragma unroll
     for (int q = 0; q < NEF; ++q)
-      if (i0 + q * 64 * WB_ROWS < tile) pre[f][q] = gld(src + (long)(kx0 + q * KXS) * Nz);
+      if (i0 + q * 64 * ROWS < tile) pre[f][q] = gld(src + (long)(kx0 + q * KXS) * Nz);
   }
   H zin[3][RX];
-  H* S = work;                           // [NFR][kxn][WB_ROWS + 1]
-  const int SP = WB_ROWS + 1;
+  H* S = work;                           // [NFR][kxn][ROWS + 1]
+  const int SP = ROWS + 1;
 #pragma unroll
   for (int f0 = 0; f0 < 6; f0 += NFR) {
     __syncthreads();
@@ -181,7 +185,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     for (int h = 0; h < NFR; ++h)
 #pragma unroll
       for (int q = 0; q < NEF; ++q)
-        if (i0 + q * 64 * WB_ROWS < tile) S[(h * kxn + kx0 + q * KXS) * SP + rr0] = pre[f0 + h][q];
+        if (i0 + q * 64 * ROWS < tile) S[(h * kxn + kx0 + q * KXS) * SP + rr0] = pre[f0 + h][q];
     __syncthreads();
 #pragma unroll
     for (int pp = 0; pp < NFR / 2; ++pp) {
@@ -272,7 +276,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     if (64 * m < kxn || 64 * m + 63 > Mx - kxn) buf[lane + 64 * m] = a12[m];
   wf_sync();
   __syncthreads();   // every wave done with its FFT buffer before the output tile
-  H* O = work;       // [3][kxn][WB_ROWS + 1]
+  H* O = work;       // [3][kxn][ROWS + 1]
   H h1[RX], h2[RX];
 #pragma unroll
   for (int m = 0; m < RX; ++m) {
@@ -299,7 +303,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     H* dst = hout + f * fstride + rowbase + rr0;
 #pragma unroll
     for (int q = 0; q < NEF; ++q)
-      if (i0 + q * 64 * WB_ROWS < tile)
+      if (i0 + q * 64 * ROWS < tile)
         gst(dst + (long)(kx0 + q * KXS) * Nz, O[(f * kxn + kx0 + q * KXS) * SP + rr0]);
   }
   // --- reductions: one atomic per wave for umax, one partial per row ---
@@ -310,7 +314,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
       m2 = fmax(m2, __shfl_xor(m2, o, 64));
       m3 = fmax(m3, __shfl_xor(m3, o, 64));
     }
-    __shared__ double um[WB_ROWS][4];
+    __shared__ double um[ROWS][4];
     if (lane == 0) {
       um[wave][0] = (double)m0;
       um[wave][1] = (double)m1;
@@ -321,7 +325,7 @@ k_xzw_B(Geom g, const H* __restrict__ hin, H* __restrict__ hout,
     if (threadIdx.x < 4) {   // one partial per block
       double m = 0.0;
 #pragma unroll
-      for (int w = 0; w < WB_ROWS; ++w) m = fmax(m, um[w][threadIdx.x]);
+      for (int w = 0; w < ROWS; ++w) m = fmax(m, um[w][threadIdx.x]);
       // maxima of non-negative doubles order like their bit patterns
       atomicMax(reinterpret_cast<unsigned long long*>(upart) + (lb % UMAX_SLOTS) * 4 + threadIdx.x,
                 (unsigned long long)__double_as_longlong(m));
@@ -403,11 +407,11 @@ k_xzw_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
 // elements of es bytes (16: FP64, 8: FP32)
 inline size_t xzw_lds_A(int R, size_t es) { return (size_t)(64 * R + WA_WAVES * (R * WF_EP > 8 * R * WF_FP ? R * WF_EP : 8 * R * WF_FP)) * es; }
 inline size_t xzw_lds_C(int R, size_t es) { return xzw_lds_A(R, es); }
-inline size_t xzw_lds_B(int Rx, int kxn, size_t es) {
+inline size_t xzw_lds_B(int Rx, int kxn, size_t es, int rows = WB_ROWS) {
   const bool wide = Rx <= 6;   // k_xzw_B's WIDE
-  const size_t fft = (size_t)WB_ROWS * (wide ? 2 : 1) * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
-  const size_t stage = (size_t)(wide ? 6 : 2) * kxn * (WB_ROWS + 1);
-  const size_t out = (size_t)3 * kxn * (WB_ROWS + 1);
+  const size_t fft = (size_t)rows * (wide ? 2 : 1) * (Rx * WF_EP > 8 * Rx * WF_FP ? Rx * WF_EP : 8 * Rx * WF_FP);
+  const size_t stage = (size_t)(wide ? 6 : 2) * kxn * (rows + 1);
+  const size_t out = (size_t)3 * kxn * (rows + 1);
   size_t w = fft > stage ? fft : stage;
   w = w > out ? w : out;
   return (size_t)(64 * Rx + w) * es;
