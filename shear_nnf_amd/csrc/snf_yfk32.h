@@ -1,15 +1,19 @@
-// The explicit y-stage kernel of the mixed precision mode with its line
-// transforms in FP32 (cross.f90:565-796, the same substep as k_yf_explicit).
+// The explicit y-stage kernel with its line transforms in FP32 (cross.f90:
+// 565-796, the same substep as k_yf_explicit), an option of the mixed
+// precision mode (SNF_Y32=1; the default mixed mode keeps FP64 transforms).
 //
 // In the mixed mode the x-z stage runs in FP32 and the six inter-stage
-// fields (rhg, rhv, rdome in; v, dv, dome out) are stored FP32, so the
-// transforms that produce them, and the viscous terms added to the FP32
-// right-hand sides, need no more than FP32 either; so do the RK
-// accumulators (right-hand sides kept within a step), stored FP32.  What
-// stays FP64: the state vor, phi, U, W and its update vn = v + dt (gamma h +
-// acc), the phase tables and the moving-frame maps applied to the state.  The transforms run on FP32 columns against an FP32
-// copy of the plan's table: half the LDS bytes of the FP64 transforms and
-// fewer registers.  The 00 lines (the mean flow U, W) keep FP64 transforms.
+// fields (rhg, rhv, rdome in; v, dv, dome out) are stored FP32; this kernel
+// also runs the transforms that produce them, and the viscous terms added to
+// the FP32 right-hand sides, in FP32, and stores the RK accumulators
+// (right-hand sides kept within a step) as FP32.  What stays FP64: the state
+// vor, phi, U, W and its update vn = v + dt (gamma h + acc), the phase
+// tables and the moving-frame maps applied to the state, the transforms of
+// the 00 lines (the mean flow U, W).  The transforms run on FP32 columns
+// against an FP32 copy of the plan's table: half the LDS bytes of the FP64
+// transforms and fewer registers.  The price is accuracy: the rounding of a
+// transform scales with the norm of its line, not with each coefficient, so
+// small correlations such as the Reynolds stress move further from FP64.
 //
 // LDS of a block: the FP64 transfer tile [N][YF_LINES] (the state passes
 // through it unrounded), two FP32 columns per wave, the FP32 table, the four
