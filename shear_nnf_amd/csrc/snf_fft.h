@@ -127,10 +127,122 @@ __device__ __forceinline__ void stockham_pass_ip(C* buf, int n, int nseq, int Ns
   __syncthreads();
 }
 
+// ---- compile-time plans for the small sizes of the ensemble boxes ----
+// The same radix order as the host's plan_of (4s, then 2s, then 3s); with n
+// and the stride Ns constants, the index arithmetic of a pass (t / nb,
+// j % Ns, j / Ns) compiles to multiplies and shifts instead of the integer
+// divisions that dominate the runtime-plan passes at these sizes.  The
+// arithmetic on the data is the same, so the results are the same bits.
+__host__ __device__ constexpr int ct_nr(int n) {
+  int c = 0, m = n;
+  while (m % 4 == 0 && m > 1) { m /= 4; ++c; }
+  while (m % 2 == 0 && m > 1) { m /= 2; ++c; }
+  while (m % 3 == 0 && m > 1) { m /= 3; ++c; }
+  return c;
+}
+__host__ __device__ constexpr int ct_r(int n, int p) {
+  int c = 0, m = n;
+  while (m % 4 == 0 && m > 1) { if (c == p) return 4; m /= 4; ++c; }
+  while (m % 2 == 0 && m > 1) { if (c == p) return 2; m /= 2; ++c; }
+  while (m % 3 == 0 && m > 1) { if (c == p) return 3; m /= 3; ++c; }
+  return 1;
+}
+template <int R, int MB, int N, int NS, class C>
+__device__ __forceinline__ void stockham_pass_ip_ct(C* buf, int nseq, const cplx* tw, int sign) {
+  constexpr int nb = N / R, stw = N / (NS * R);
+  const int total = nseq * nb;
+  C v[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      const int s = t / nb;
+      const int j = t - s * nb;
+      const int k = j % NS;
+      const C* sp = buf + s * N;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        C x = sp[j + r * nb];
+        if (r > 0) x = cmul_sign(x, tw[r * k * stw], sign);
+        v[q][r] = x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * blockDim.x;
+    if (t < total) {
+      const int s = t / nb;
+      const int j = t - s * nb;
+      const int k = j % NS;
+      dft_small<R>(v[q], sign);
+      C* dp = buf + s * N + (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dp[r * NS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+template <int MAXV, int N, int P, int NS, class C>
+__device__ __forceinline__ void fft_ip_ct(C* buf, int nseq, const cplx* tw, int sign) {
+  if constexpr (P < ct_nr(N)) {
+    constexpr int R = ct_r(N, P);
+    stockham_pass_ip_ct<R, MAXV / R, N, NS>(buf, nseq, tw, sign);
+    fft_ip_ct<MAXV, N, P + 1, NS * R>(buf, nseq, tw, sign);
+  }
+}
+template <int R, int N, int NS, class C>
+__device__ __forceinline__ void stockham_pass_ct(const C* src, C* dst, int nseq, const cplx* tw,
+                                                 int sign, int tid, int nthr) {
+  constexpr int nb = N / R, stw = N / (NS * R);
+  const int total = nseq * nb;
+  for (int t = tid; t < total; t += nthr) {
+    const int s = t / nb;
+    const int j = t - s * nb;
+    const int k = j % NS;
+    const C* sp = src + s * N;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      v[r] = sp[j + r * nb];
+      if (r > 0) v[r] = cmul_sign(v[r], tw[r * k * stw], sign);
+    }
+    dft_small<R>(v, sign);
+    C* dp = dst + s * N + (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dp[r * NS] = v[r];
+  }
+}
+// ping-pong passes P.. of size N: returns the buffer that holds the result
+template <int N, int P, int NS, class C>
+__device__ __forceinline__ C* fft_pp_ct(C* src, C* dst, int nseq, const cplx* tw, int sign) {
+  if constexpr (P < ct_nr(N)) {
+    constexpr int R = ct_r(N, P);
+    stockham_pass_ct<R, N, NS>(src, dst, nseq, tw, sign, threadIdx.x, blockDim.x);
+    __syncthreads();
+    return fft_pp_ct<N, P + 1, NS * R>(dst, src, nseq, tw, sign);
+  } else {
+    return src;
+  }
+}
+// sizes with a compile-time plan: the ensemble boxes' 32, 48, 64 and 96
+#define SNF_FFT_CT_CASES(F) F(32) F(48) F(64) F(96)
+
 // MAXV: complex values a thread may hold per pass (R * butterflies)
 template <int MAXV, class C>
 __device__ __forceinline__ void fft_lds_ip(C* buf, int nseq, const FFTPlan& pl,
                                            const cplx* tw, int sign) {
+  switch (pl.n) {
+#define SNF_FFT_CT_IP(n)                                       \
+    case n:                                                     \
+      fft_ip_ct<MAXV, n, 0, 1>(buf, nseq, tw, sign);            \
+      return;
+    SNF_FFT_CT_CASES(SNF_FFT_CT_IP)
+#undef SNF_FFT_CT_IP
+    default:
+      break;
+  }
   int Ns = 1;
   for (int p = 0; p < pl.nr; ++p) {
     const int R = pl.r[p];
@@ -158,9 +270,21 @@ inline int fft_max_values(const FFTPlan& pl, int nseq, int nthreads) {
 
 // Transforms nseq sequences in a (result in the returned buffer, a or b).
 // sign = -1: forward exp(-i...), sign = +1: inverse exp(+i...), unnormalised.
-template <class C>
+// CT = false: the runtime plan at every size (the spectral-to-hybrid kernel
+// keeps it: its compile-time passes raise its registers from 6 waves per
+// SIMD to 4)
+template <bool CT = true, class C>
 __device__ __forceinline__ C* fft_lds(C* a, C* b, int nseq, const FFTPlan& pl,
                                       const cplx* tw, int sign) {
+  if (CT) switch (pl.n) {
+#define SNF_FFT_CT_PP(n) \
+    case n:               \
+      return fft_pp_ct<n, 0, 1>(a, b, nseq, tw, sign);
+    SNF_FFT_CT_CASES(SNF_FFT_CT_PP)
+#undef SNF_FFT_CT_PP
+    default:
+      break;
+  }
   const int tid = threadIdx.x, nthr = blockDim.x;
   C* src = a;
   C* dst = b;

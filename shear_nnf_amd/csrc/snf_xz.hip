@@ -145,7 +145,7 @@ __device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor
       for (int f = 0; f < 3; ++f) a[(3 * r + f) * Nz + pos] = to_h<H>(d[grp * 3 + f]);
     }
     __syncthreads();
-    H* res = fft_lds(a, b, 3 * nr, pz, tw, +1);
+    H* res = fft_lds<false>(a, b, 3 * nr, pz, tw, +1);
     for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) {
       const int rf = t / Nz, z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
       hyb[(grp * 3 + f) * fstride + ((long)(j0 + r) * g.kxn + i) * Nz + z] = res[t];
