@@ -664,10 +664,11 @@ int launch_twist(snf_handle* h, int which, bool tails = true) {
 unsigned ylines_blocks(const snf_handle* h) { return (unsigned)((h->P + 2 + 127) / 128); }
 
 // the Stockham x-row kernel with nt threads and zb z-rows per block fits
-// its LDS and per-thread plans
+// its LDS and per-thread plans (zb a power of two: the kernel indexes rows by
+// mask and shift)
 bool xz_b_fits(const snf_handle* h, int nt, int zb) {
   const int mgalx = h->g.mgalx, mgalz = h->g.mgalz;
-  return zb >= 2 && mgalz % zb == 0 && xz_lds_B(zb, mgalx, sizeof(cplx)) <= 156 * 1024 &&
+  return zb >= 2 && (zb & (zb - 1)) == 0 && mgalz % zb == 0 && xz_lds_B(zb, mgalx, sizeof(cplx)) <= 156 * 1024 &&
          (zb / 2) * mgalx <= XZ_B_MAXW * nt && fft_max_values(h->px, 3 * zb, nt) <= XZ_MAXV;
 }
 

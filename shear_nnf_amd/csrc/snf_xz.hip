@@ -196,15 +196,20 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   const long fstride = (long)g.my * kxn * Nz;
   const int ihist = ((sc->steps_local - 1) % nhist == 0) && rk == 1;
   const int nseq = 3 * ZB;    // sequence s = p*ZB + r (pair p, row r)
+  // ZB is a power of two (2..16): row and group of an index by mask and shift
+  const int zm = ZB - 1, zs = 31 - __clz(ZB);
+  // t / Mx for t < 2^16 through the FP32 reciprocal: (t + 1/2) / Mx lies at
+  // least 1/(2 Mx) from an integer, far beyond the rounding of the product
+  const float rmx = 1.0f / (float)Mx;
   cplx* tw = reinterpret_cast<cplx*>(lds + nseq * Mx);   // the x roots, staged once
   for (int t = threadIdx.x; t < Mx; t += blockDim.x) tw[t] = gld(twx + t);
   for (int t = threadIdx.x; t < nseq * Mx; t += blockDim.x) lds[t] = wf_mk((R)0, (R)0);
   __syncthreads();
   // Z_pair = G_a + i G_b with the Hermitian extension in kx
   for (int t = threadIdx.x; t < 3 * kxn * ZB; t += blockDim.x) {
-    const int r = t % ZB;
-    const int pi_ = t / ZB;
-    const int i = pi_ % kxn, p = pi_ / kxn;
+    const int r = t & zm;
+    const int pi_ = t >> zs;
+    const int p = (pi_ >= kxn) + (pi_ >= 2 * kxn), i = pi_ - p * kxn;
     const long off = ((long)j * kxn + i) * Nz + z0 + r;
     const H Ga = hin[(2 * p) * fstride + off];
     const H Gb = hin[(2 * p + 1) * fstride + off];
@@ -233,7 +238,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   for (int q = 0; q < MAXW; ++q) {
     const int t = threadIdx.x + q * blockDim.x;
     if (t < nwork) {
-      const int rp = t / Mx, x = t - rp * Mx;
+      const int rp = (int)(((float)t + 0.5f) * rmx), x = t - rp * Mx;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int r = 2 * rp + h;
@@ -263,7 +268,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   for (int q = 0; q < MAXW; ++q) {
     const int t = threadIdx.x + q * blockDim.x;
     if (t < nwork) {
-      const int rp = t / Mx, x = t - rp * Mx;
+      const int rp = (int)(((float)t + 0.5f) * rmx), x = t - rp * Mx;
       lds[(2 * rp) * Mx + x] = wf_mk(h1[q][0], h2[q][0]);
       lds[(2 * rp + 1) * Mx + x] = wf_mk(h1[q][1], h2[q][1]);
       lds[(ZB + rp) * Mx + x] = wf_mk(h3[q][0], h3[q][1]);
@@ -315,10 +320,10 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   fft_lds_ip<XZ_MAXV>(lds, ZB + npair, px, tw, -1);
   // unpack the pairs, z fastest among consecutive threads
   for (int t = threadIdx.x; t < kxn * ZB; t += blockDim.x) {
-    const int r = t % ZB, k = t / ZB;
+    const int r = t & zm, k = t >> zs;
     const int km = (k == 0) ? 0 : Mx - k;
     const H A = lds[r * Mx + k], Am = lds[r * Mx + km];
-    const H* Bs = lds + (ZB + r / 2) * Mx;
+    const H* Bs = lds + (ZB + (r >> 1)) * Mx;
     const H B = Bs[k], Bm = Bs[km];
     const R hf = (R)0.5;
     const H H1 = wf_mk(hf * (A.x + Am.x), hf * (A.y - Am.y));
