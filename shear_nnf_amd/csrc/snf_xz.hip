@@ -101,6 +101,11 @@ __device__ __forceinline__ void derived6(const Geom& g, const PlaneIn& q, int i,
 // planes), at most 8.  The host sizes the grid (my / rows blocks per kx)
 // and the LDS (2 * 3 * rows * Nz elements and Nz FP64 roots) with the same
 // function.
+// t / d for 0 <= t < 2^16 and 1 <= d <= 1024 from rd = 1/d in FP32: (t + 1/2) / d
+// lies at least 1/(2d) from an integer, and the two roundings move it by at
+// most (t/d) 2^-22, so the truncation is exact (no integer division)
+__device__ __forceinline__ int xz_div(int t, float rd) { return (int)(((float)t + 0.5f) * rd); }
+
 __host__ __device__ __forceinline__ int xz_rows_per_block(int Nz) {
   const int r = 1536 / (3 * Nz);
   return r < 1 ? 1 : (r > 8 ? 8 : r);
@@ -130,11 +135,13 @@ __device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor
   const int nr = min(RB, g.my - j0);                 // rows of this block
   const long fstride = (long)g.my * g.kxn * Nz;
   const int nload = g.nkzb * nr * SNF_CW;
+  const float rnr = 1.0f / (float)nr, rnz = 1.0f / (float)Nz;
   for (int grp = 0; grp < 2; ++grp) {
     for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) a[t] = wf_mk((R)0, (R)0);
     __syncthreads();
     for (int t = threadIdx.x; t < nload; t += blockDim.x) {
-      const int c = t & (SNF_CW - 1), r = (t / SNF_CW) % nr, k = (t / (SNF_CW * nr)) * SNF_CW + c;
+      const int c = t & (SNF_CW - 1), rc = t / SNF_CW, kc = xz_div(rc, rnr), r = rc - kc * nr;
+      const int k = kc * SNF_CW + c;
       if (k >= g.mz) continue;
       const int j = j0 + r;
       PlaneIn q = load_in(g, vor, phi, hg, hv, dvordy, j, i, k);
@@ -147,7 +154,7 @@ __device__ __forceinline__ void k_xz_A_body(Geom g, const cplx* __restrict__ vor
     __syncthreads();
     H* res = fft_lds<false>(a, b, 3 * nr, pz, tw, +1);
     for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) {
-      const int rf = t / Nz, z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
+      const int rf = xz_div(t, rnz), z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
       hyb[(grp * 3 + f) * fstride + ((long)(j0 + r) * g.kxn + i) * Nz + z] = res[t];
     }
     __syncthreads();
@@ -198,9 +205,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   const int nseq = 3 * ZB;    // sequence s = p*ZB + r (pair p, row r)
   // ZB is a power of two (2..16): row and group of an index by mask and shift
   const int zm = ZB - 1, zs = 31 - __clz(ZB);
-  // t / Mx for t < 2^16 through the FP32 reciprocal: (t + 1/2) / Mx lies at
-  // least 1/(2 Mx) from an integer, far beyond the rounding of the product
-  const float rmx = 1.0f / (float)Mx;
+  const float rmx = 1.0f / (float)Mx;   // t / Mx by xz_div
   cplx* tw = reinterpret_cast<cplx*>(lds + nseq * Mx);   // the x roots, staged once
   for (int t = threadIdx.x; t < Mx; t += blockDim.x) tw[t] = gld(twx + t);
   for (int t = threadIdx.x; t < nseq * Mx; t += blockDim.x) lds[t] = wf_mk((R)0, (R)0);
@@ -238,7 +243,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   for (int q = 0; q < MAXW; ++q) {
     const int t = threadIdx.x + q * blockDim.x;
     if (t < nwork) {
-      const int rp = (int)(((float)t + 0.5f) * rmx), x = t - rp * Mx;
+      const int rp = xz_div(t, rmx), x = t - rp * Mx;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int r = 2 * rp + h;
@@ -268,7 +273,7 @@ __device__ __forceinline__ void k_xz_B_body(Geom g, const H* __restrict__ hin, H
   for (int q = 0; q < MAXW; ++q) {
     const int t = threadIdx.x + q * blockDim.x;
     if (t < nwork) {
-      const int rp = (int)(((float)t + 0.5f) * rmx), x = t - rp * Mx;
+      const int rp = xz_div(t, rmx), x = t - rp * Mx;
       lds[(2 * rp) * Mx + x] = wf_mk(h1[q][0], h2[q][0]);
       lds[(2 * rp + 1) * Mx + x] = wf_mk(h1[q][1], h2[q][1]);
       lds[(ZB + rp) * Mx + x] = wf_mk(h3[q][0], h3[q][1]);
@@ -364,8 +369,9 @@ __device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, 
   const int j0 = blockIdx.x * RB, i = blockIdx.y;   // consecutive blocks: consecutive y of one kx
   const int nr = min(RB, g.my - j0);
   const long fstride = (long)g.my * g.kxn * Nz;
+  const float rnr = 1.0f / (float)nr, rnz = 1.0f / (float)Nz;
   for (int t = threadIdx.x; t < 3 * nr * Nz; t += blockDim.x) {
-    const int rf = t / Nz, z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
+    const int rf = xz_div(t, rnz), z = t - rf * Nz, r = rf / 3, f = rf - 3 * r;
     a[t] = hout[f * fstride + ((long)(j0 + r) * g.kxn + i) * Nz + z];
   }
   __syncthreads();
@@ -375,7 +381,8 @@ __device__ __forceinline__ void k_xz_C_body(Geom g, const H* __restrict__ hout, 
   const double kx2 = kx * kx;
   const int nst = g.nkzb * nr * SNF_CW;
   for (int t = threadIdx.x; t < nst; t += blockDim.x) {
-    const int c = t & (SNF_CW - 1), r = (t / SNF_CW) % nr, k = (t / (SNF_CW * nr)) * SNF_CW + c;
+    const int c = t & (SNF_CW - 1), rc = t / SNF_CW, kc = xz_div(rc, rnr), r = rc - kc * nr;
+    const int k = kc * SNF_CW + c;
     if (k >= g.mz) continue;
     const int j = j0 + r;
     const H* rr = res + 3 * r * Nz;
