@@ -20,6 +20,7 @@
 #include "snf_setup.h"
 #include "snf_xz.hip"
 #include "snf_xzw.hip"
+#include "snf_xzs.hip"
 #include "snf_scalar.hip"
 #include "snf_ykern.hip"
 #include "snf_yplan.h"
@@ -378,6 +379,15 @@ void launch_xz_generic(snf_handle* h, int which, int rk) {
     k_xz_A<H><<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                                h->w00r, h->dudy, h->dwdy, h->p.vbulk, hin, h->pz,
                                                h->twz, h->sc);
+  } else if (which == 1 && h->bthreads == 0) {
+    // the half-wave x-row kernel (shape "0 threads, 8 rows")
+    dim3 gB(g.mgalz / XZS_ROWS, g.my);
+    if (g.mgalx == 96)
+      k_xzs_B<3, H><<<gB, 256, h->ldsB, h->stream>>>(g, hin, hout, h->twx, rk, h->umax_acc,
+                                                    h->phist, h->sc, h->p.nhist);
+    else
+      k_xzs_B<2, H><<<gB, 256, h->ldsB, h->stream>>>(g, hin, hout, h->twx, rk, h->umax_acc,
+                                                    h->phist, h->sc, h->p.nhist);
   } else if (which == 1) {
     dim3 gB(g.mgalz / h->zb, g.my);
     auto kb = h->bthreads == 256 ? k_xz_B<256, H>
@@ -668,8 +678,23 @@ unsigned ylines_blocks(const snf_handle* h) { return (unsigned)((h->P + 2 + 127)
 // mask and shift)
 bool xz_b_fits(const snf_handle* h, int nt, int zb) {
   const int mgalx = h->g.mgalx, mgalz = h->g.mgalz;
+  if (nt == 0) return zb == XZS_ROWS && xzs_size(mgalx, mgalz);   // the half-wave kernel
   return zb >= 2 && (zb & (zb - 1)) == 0 && mgalz % zb == 0 && xz_lds_B(zb, mgalx, sizeof(cplx)) <= 156 * 1024 &&
          (zb / 2) * mgalx <= XZ_B_MAXW * nt && fft_max_values(h->px, 3 * zb, nt) <= XZ_MAXV;
+}
+
+// LDS of the handle's x-row kernel shape, and its kernels' LDS limits
+size_t xz_b_lds(const snf_handle* h) {
+  const size_t es = h->mixed ? sizeof(cplxf) : sizeof(cplx);
+  return h->bthreads == 0 ? xzs_lds_B(h->g.mgalx / 32, es) : xz_lds_B(h->zb, h->g.mgalx, es);
+}
+void xz_b_attrs(const snf_handle* h) {
+  for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
+                         (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
+                         (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>,
+                         (const void*)k_xzs_B<2, cplx>, (const void*)k_xzs_B<3, cplx>,
+                         (const void*)k_xzs_B<2, cplxf>, (const void*)k_xzs_B<3, cplxf>})
+    lds_attr(kb, h->ldsB, h->dev);
 }
 
 // ---- box tables of the batched ensemble (kernels in snf_batch.hip)
@@ -710,6 +735,17 @@ void tab_B_nt(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s)
 }
 template <class H>
 void tab_B(const snf_handle* h, const BxXzB* t, int n, int rk, hipStream_t s) {
+  if (h->bthreads == 0) {
+    const dim3 gB(h->g.mgalz / XZS_ROWS, h->g.my, n);
+    if (h->g.mgalx == 96) {
+      lds_attr((const void*)k_xzs_B_tab<3, H>, h->ldsB, h->dev);
+      if (n) k_xzs_B_tab<3, H><<<gB, 256, h->ldsB, s>>>(t, rk);
+    } else {
+      lds_attr((const void*)k_xzs_B_tab<2, H>, h->ldsB, h->dev);
+      if (n) k_xzs_B_tab<2, H><<<gB, 256, h->ldsB, s>>>(t, rk);
+    }
+    return;
+  }
   if (h->bthreads == 256)
     tab_B_nt<256, H>(h, t, n, rk, s);
   else if (h->bthreads == 512)
@@ -880,11 +916,8 @@ int set_b_shape(const std::vector<snf_handle*>& hs, int nt, int zb) {
     if (h->bthreads == nt && h->zb == zb) continue;
     h->bthreads = nt;
     h->zb = zb;
-    h->ldsB = xz_lds_B(zb, h->g.mgalx, h->mixed ? sizeof(cplxf) : sizeof(cplx));
-    for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
-                           (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
-                           (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>})
-      lds_attr(kb, h->ldsB, h->dev);
+    h->ldsB = xz_b_lds(h);
+    xz_b_attrs(h);
     // the step graphs of the single advance carry the old shape
     for (hipGraphExec_t& ge : h->gexec)
       if (ge) {
@@ -910,7 +943,7 @@ int tune_class(const std::vector<snf_handle*>& hs, hipStream_t s) {
   float best = 1e30f;
   std::pair<int, int> pick(h0->bthreads, h0->zb);
   int rc = SNF_OK;
-  for (int nt : {256, 512, 1024})
+  for (int nt : {256, 512, 1024, 0})
     for (int zb : {2, 4, 8, 16}) {
       if (!xz_b_fits(h0, nt, zb)) continue;
       set_b_shape(hs, nt, zb);
@@ -1212,7 +1245,8 @@ int snf_create(const snf_params* p, snf_handle** out) {
     }
   }
   // LDS plans: A holds 6 z-sequences, C 3, B 3*zb x-sequences (in place).
-  // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning).
+  // B's block shape can be overridden with SNF_XZ_B="threads,zb" (tuning;
+  // threads 0: the half-wave kernel).
   auto b_fits = [&](int nt, int zb) { return xz_b_fits(h, nt, zb); };
   // 256 threads x 2 rows measured fastest at 384x385x384 (8 LDS-resident
   // rows per CU at 4 blocks/CU beat 1-2 larger blocks)
@@ -1223,9 +1257,15 @@ int snf_create(const snf_params* p, snf_handle** out) {
     h->zb = 8;
     while (h->zb > 2 && !b_fits(h->bthreads, h->zb)) h->zb /= 2;
   }
+  // planes of 64 and 96 points: the half-wave kernel (threads 0, 8 rows)
+  if (b_fits(0, XZS_ROWS)) {
+    h->bthreads = 0;
+    h->zb = XZS_ROWS;
+  }
   if (const char* e = getenv("SNF_XZ_B")) {
     int nt = 0, zb = 0;
-    if (sscanf(e, "%d,%d", &nt, &zb) == 2 && (nt == 256 || nt == 512 || nt == 1024) && b_fits(nt, zb)) {
+    if (sscanf(e, "%d,%d", &nt, &zb) == 2 && (nt == 0 || nt == 256 || nt == 512 || nt == 1024) &&
+        b_fits(nt, zb)) {
       h->bthreads = nt;
       h->zb = zb;
     }
@@ -1276,15 +1316,12 @@ int snf_create(const snf_params* p, snf_handle** out) {
   // ping-pong buffers of 3 sequences per row, xz_rows_per_block rows, and
   // the FP64 z roots
   h->ldsA = 2 * 3 * (size_t)xz_rows_per_block(mgalz) * mgalz * es + (size_t)mgalz * sizeof(cplx);
-  h->ldsB = xz_lds_B(h->zb, mgalx, es);
+  h->ldsB = xz_b_lds(h);
   h->ldsC = h->ldsA;
   lds_attr((const void*)k_ydealias, 3 * (size_t)my * sizeof(cplx), h->dev);
   for (const void* k : {(const void*)k_xz_A<cplx>, (const void*)k_xz_A<cplxf>})
     lds_attr(k, h->ldsA, h->dev);
-  for (const void* kb : {(const void*)k_xz_B<256, cplx>, (const void*)k_xz_B<512, cplx>,
-                         (const void*)k_xz_B<1024, cplx>, (const void*)k_xz_B<256, cplxf>,
-                         (const void*)k_xz_B<512, cplxf>, (const void*)k_xz_B<1024, cplxf>})
-    lds_attr(kb, h->ldsB, h->dev);
+  xz_b_attrs(h);
   for (const void* k : {(const void*)k_xz_C<cplx>, (const void*)k_xz_C<cplxf>})
     lds_attr(k, h->ldsC, h->dev);
 #undef TRY

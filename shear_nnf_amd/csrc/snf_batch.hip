@@ -120,6 +120,14 @@ __global__ void __launch_bounds__(NT) k_xz_B_tab(const BxXzB* __restrict__ t, in
   k_xz_B_body<NT, H>(a.g, reinterpret_cast<const H*>(a.hin), reinterpret_cast<H*>(a.hout), a.px,
                      a.twx, rk, a.umax_acc, a.phist, a.sc, a.nhist, a.zb);
 }
+template <int RX, class H>
+__global__ void __launch_bounds__(256) k_xzs_B_tab(const BxXzB* __restrict__ t, int rk) {
+  const BxXzB& a = t[blockIdx.z];
+  const long lb = xcd_block((long)blockIdx.y * gridDim.x + blockIdx.x, (long)gridDim.x * gridDim.y);
+  k_xzs_B_body<RX, H>(a.g, reinterpret_cast<const H*>(a.hin), reinterpret_cast<H*>(a.hout), a.twx,
+                      rk, a.umax_acc, a.phist, a.sc, a.nhist, (int)(lb % gridDim.x),
+                      (int)(lb / gridDim.x));
+}
 template <class H>
 __global__ void __launch_bounds__(256) k_xz_C_tab(const BxXzC* __restrict__ t) {
   const BxXzC& a = t[blockIdx.z];
