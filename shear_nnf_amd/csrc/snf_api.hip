@@ -187,6 +187,7 @@ struct snf_handle {
   bool ysym_on = false;   // multiplier tables maintained (spectral y or filter)
   bool ymfma = false;     // line lengths without a radix plan: DFT on the matrix cores
   bool y32 = false;       // mixed mode: explicit y substeps with FP32 line transforms
+  bool zhalf = false;     // generic x-z path: half-wave z kernels (64, 96 points)
   cplx* twy = nullptr;
   cplx* tpy = nullptr;                  // pass-major twiddles of the y plan
   cplx* ysym[2] = {nullptr, nullptr};   // per-twist multiplier tables [kxn][my][2]
@@ -375,7 +376,15 @@ void launch_xz_generic(snf_handle* h, int which, int rk) {
   dim3 gA((g.my + rb - 1) / rb, g.kxn);
   H* hin = reinterpret_cast<H*>(h->hyb_in);
   H* hout = reinterpret_cast<H*>(h->hyb_out);
-  if (which == 0) {
+  if (which == 0 && h->zhalf) {
+    // the half-wave z kernels of 64- and 96-point planes
+    if (g.mgalz == 96)
+      k_xzs_A<3, H><<<xzs_rows_blocks(g), 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv,
+          h->dome, h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk, hin, h->twz, h->sc);
+    else
+      k_xzs_A<2, H><<<xzs_rows_blocks(g), 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv,
+          h->dome, h->u00r, h->w00r, h->dudy, h->dwdy, h->p.vbulk, hin, h->twz, h->sc);
+  } else if (which == 0) {
     k_xz_A<H><<<gA, 256, h->ldsA, h->stream>>>(g, h->vor, h->phi, h->v, h->dv, h->dome, h->u00r,
                                                h->w00r, h->dudy, h->dwdy, h->p.vbulk, hin, h->pz,
                                                h->twz, h->sc);
@@ -394,6 +403,13 @@ void launch_xz_generic(snf_handle* h, int which, int rk) {
                                  : (h->bthreads == 512 ? k_xz_B<512, H> : k_xz_B<1024, H>);
     kb<<<gB, h->bthreads, h->ldsB, h->stream>>>(g, hin, hout, h->px, h->twx, rk, h->umax_acc,
                                                  h->phist, h->sc, h->p.nhist, h->zb);
+  } else if (h->zhalf) {
+    if (g.mgalz == 96)
+      k_xzs_C<3, H><<<xzs_rows_blocks(g), 256, h->ldsC, h->stream>>>(g, hout, h->v, h->rdome, h->rhg,
+          h->rhv, h->rf0u, h->rf0w, h->twz, h->sc);
+    else
+      k_xzs_C<2, H><<<xzs_rows_blocks(g), 256, h->ldsC, h->stream>>>(g, hout, h->v, h->rdome, h->rhg,
+          h->rhv, h->rf0u, h->rf0w, h->twz, h->sc);
   } else {
     k_xz_C<H><<<gA, 256, h->ldsC, h->stream>>>(g, hout, h->v, h->rdome, h->rhg, h->rhv, h->rf0u,
                                                h->rf0w, h->pz, h->twz, h->sc);
@@ -717,12 +733,34 @@ struct BoxTables {
 
 template <class H>
 void tab_A(const snf_handle* h, const BxXzA* t, int n, hipStream_t s) {
+  if (h->zhalf) {
+    const dim3 gA(xzs_rows_blocks(h->g), 1, n);
+    if (h->g.mgalz == 96) {
+      lds_attr((const void*)k_xzs_A_tab<3, H>, h->ldsA, h->dev);
+      if (n) k_xzs_A_tab<3, H><<<gA, 256, h->ldsA, s>>>(t);
+    } else {
+      lds_attr((const void*)k_xzs_A_tab<2, H>, h->ldsA, h->dev);
+      if (n) k_xzs_A_tab<2, H><<<gA, 256, h->ldsA, s>>>(t);
+    }
+    return;
+  }
   lds_attr((const void*)k_xz_A_tab<H>, h->ldsA, h->dev);
   if (n == 0) return;   // the LDS limit only (set before a graph capture)
   k_xz_A_tab<H><<<dim3((h->g.my + xz_rows_per_block(h->g.mgalz) - 1) / xz_rows_per_block(h->g.mgalz), h->g.kxn, n), 256, h->ldsA, s>>>(t);
 }
 template <class H>
 void tab_C(const snf_handle* h, const BxXzC* t, int n, hipStream_t s) {
+  if (h->zhalf) {
+    const dim3 gC(xzs_rows_blocks(h->g), 1, n);
+    if (h->g.mgalz == 96) {
+      lds_attr((const void*)k_xzs_C_tab<3, H>, h->ldsC, h->dev);
+      if (n) k_xzs_C_tab<3, H><<<gC, 256, h->ldsC, s>>>(t);
+    } else {
+      lds_attr((const void*)k_xzs_C_tab<2, H>, h->ldsC, h->dev);
+      if (n) k_xzs_C_tab<2, H><<<gC, 256, h->ldsC, s>>>(t);
+    }
+    return;
+  }
   lds_attr((const void*)k_xz_C_tab<H>, h->ldsC, h->dev);
   if (n == 0) return;
   k_xz_C_tab<H><<<dim3((h->g.my + xz_rows_per_block(h->g.mgalz) - 1) / xz_rows_per_block(h->g.mgalz), h->g.kxn, n), 256, h->ldsC, s>>>(t);
@@ -1318,6 +1356,18 @@ int snf_create(const snf_params* p, snf_handle** out) {
   h->ldsA = 2 * 3 * (size_t)xz_rows_per_block(mgalz) * mgalz * es + (size_t)mgalz * sizeof(cplx);
   h->ldsB = xz_b_lds(h);
   h->ldsC = h->ldsA;
+  // planes of 64 and 96 z points: the half-wave z kernels (SNF_XZ_ZHALF=0:
+  // the Stockham ones)
+  {
+    const char* ez = getenv("SNF_XZ_ZHALF");
+    h->zhalf = !h->wave_xz && xzs_z_size(mgalz) && !(ez && std::string(ez) == "0");
+    if (h->zhalf) h->ldsA = h->ldsC = xzs_lds_AC(mgalz / 32, es);
+    for (const void* k : {(const void*)k_xzs_A<2, cplx>, (const void*)k_xzs_A<3, cplx>,
+                          (const void*)k_xzs_A<2, cplxf>, (const void*)k_xzs_A<3, cplxf>,
+                          (const void*)k_xzs_C<2, cplx>, (const void*)k_xzs_C<3, cplx>,
+                          (const void*)k_xzs_C<2, cplxf>, (const void*)k_xzs_C<3, cplxf>})
+      lds_attr(k, h->ldsA, h->dev);
+  }
   lds_attr((const void*)k_ydealias, 3 * (size_t)my * sizeof(cplx), h->dev);
   for (const void* k : {(const void*)k_xz_A<cplx>, (const void*)k_xz_A<cplxf>})
     lds_attr(k, h->ldsA, h->dev);

@@ -120,6 +120,18 @@ __global__ void __launch_bounds__(NT) k_xz_B_tab(const BxXzB* __restrict__ t, in
   k_xz_B_body<NT, H>(a.g, reinterpret_cast<const H*>(a.hin), reinterpret_cast<H*>(a.hout), a.px,
                      a.twx, rk, a.umax_acc, a.phist, a.sc, a.nhist, a.zb);
 }
+template <int RZ, class H>
+__global__ void __launch_bounds__(256) k_xzs_A_tab(const BxXzA* __restrict__ t) {
+  const BxXzA& a = t[blockIdx.z];
+  k_xzs_A_body<RZ, H>(a.g, a.vor, a.phi, a.v, a.dv, a.dome, a.u00r, a.w00r, a.dudy, a.dwdy, a.vbulk,
+                      reinterpret_cast<H*>(a.hyb), a.twz, a.sc, (int)xcd_block(blockIdx.x, gridDim.x));
+}
+template <int RZ, class H>
+__global__ void __launch_bounds__(256) k_xzs_C_tab(const BxXzC* __restrict__ t) {
+  const BxXzC& a = t[blockIdx.z];
+  k_xzs_C_body<RZ, H>(a.g, reinterpret_cast<const H*>(a.hout), a.vel, a.dvordy, a.hg, a.hv, a.rf0u,
+                      a.rf0w, a.twz, a.sc, (int)xcd_block(blockIdx.x, gridDim.x));
+}
 template <int RX, class H>
 __global__ void __launch_bounds__(256) k_xzs_B_tab(const BxXzB* __restrict__ t, int rk) {
   const BxXzB& a = t[blockIdx.z];

@@ -231,3 +231,168 @@ inline size_t xzs_lds_B(int RX, size_t es) {
 inline bool xzs_size(int mgalx, int mgalz) {
   return (mgalx == 64 || mgalx == 96) && mgalz % XZS_ROWS == 0;
 }
+
+// ---------------------------------------------------------------------------
+// The z-row stages for planes of Nz = 32 RZ points (RZ = 2, 3), as the wave
+// kernels A and C of snf_xzw.hip with a (kx, y) row per 32-lane half: a block
+// of 4 waves takes 8 consecutive y of one kx (the kz chunks' runs of 8 x 4
+// values), blocks in order of y then kx.  A half whose y lies beyond my
+// computes on the last row and stores nothing.
+__device__ __forceinline__ void xzs_row(const Geom& g, int blk, int rr, int& j, int& i, bool& on) {
+  const int nyg = (g.my + XZS_ROWS - 1) / XZS_ROWS;
+  i = blk / nyg;
+  j = (blk - i * nyg) * XZS_ROWS + rr;
+  on = j < g.my;
+  if (!on) j = g.my - 1;
+}
+inline unsigned xzs_rows_blocks(const Geom& g) {
+  return (unsigned)(g.kxn * ((g.my + XZS_ROWS - 1) / XZS_ROWS));
+}
+
+template <int RZ, class H>
+__device__ __forceinline__ void k_xzs_A_body(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+       const cplx* __restrict__ hgf, const cplx* __restrict__ hvf, const cplx* __restrict__ dvordy,
+       const double* __restrict__ u00, const double* __restrict__ w00, const double* __restrict__ dudy,
+       const double* __restrict__ dwdy, double vbulk, H* __restrict__ hyb,
+       const cplx* __restrict__ twz, const DevScalars* sc, int blk) {
+  typedef WFFT32<RZ, 1, H> W;
+  typedef typename wf_real<H>::T R;
+  if (sc->halted) return;
+  extern __shared__ double4 xzs_smem[];
+  H* lds = reinterpret_cast<H*>(xzs_smem);
+  constexpr int Nz = 32 * RZ, NZ = wz_nz(Nz), MZ = wz_mz(Nz);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, l = lane & 31;
+  const int rr = 2 * wave + (lane >> 5);
+  H* rt = lds;
+  H* buf = lds + Nz + rr * W::LDS;
+  stage_roots(rt, twz, Nz);
+  __syncthreads();
+  int j, i;
+  bool on;
+  xzs_row(g, blk, rr, j, i, on);
+  const double u00j = gldd(u00 + j), w00j = gldd(w00 + j);
+  const double dudyj = gldd(dudy + j), dwdyj = gldd(dwdy + j);
+  H d[6][RZ];
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) {
+    const int k = wz_k(l + 32 * m, NZ, MZ, Nz);
+    if (k >= 0) {
+      const long idx = sidx(g, j, i, k);
+      PlaneIn q;
+      if (i == 0 && k > NZ) {     // set_conj (operators.f90:1382-1429)
+        const long cj = sidx(g, j, 0, MZ - k);
+        q.phic = cconj(gld(phi + cj));
+        q.ome2c = cconj(gld(vor + cj));
+      } else {
+        q.phic = gld(phi + idx);
+        q.ome2c = gld(vor + idx);
+      }
+      q.v = ld_io(hgf, idx, g.io32 & 1);
+      q.dv = ld_io(hvf, idx, g.io32 & 1);
+      q.dome = ld_io(dvordy, idx, g.io32 & 1);
+      cplx o[6];
+      derived6(g, q, i, k, u00j, w00j, dudyj, dwdyj, vbulk, o);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) d[f][m] = to_h<H>(o[f]);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) d[f][m] = wf_mk((R)0, (R)0);
+    }
+  }
+  const long fstride = (long)g.my * g.kxn * Nz;
+  H* dst = hyb + ((long)j * g.kxn + i) * Nz;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    H out[8];
+    wfft32_l2t<RZ, 1>(d[f], out, buf, rt);
+    if (on && l < W::NT) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) gst(dst + f * fstride + l + 4 * RZ * q, out[q]);
+    }
+  }
+}
+
+template <int RZ, class H>
+__device__ __forceinline__ void k_xzs_C_body(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel,
+       cplx* __restrict__ dvordy, cplx* __restrict__ hgf, cplx* __restrict__ hvf,
+       double* __restrict__ rf0u, double* __restrict__ rf0w, const cplx* __restrict__ twz,
+       const DevScalars* sc, int blk) {
+  typedef WFFT32<RZ, -1, H> W;
+  if (sc->halted) return;
+  extern __shared__ double4 xzs_smem[];
+  H* lds = reinterpret_cast<H*>(xzs_smem);
+  constexpr int Nz = 32 * RZ, NZ = wz_nz(Nz), MZ = wz_mz(Nz);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, l = lane & 31;
+  const int rr = 2 * wave + (lane >> 5);
+  H* rt = lds;
+  H* buf = lds + Nz + rr * W::LDS;
+  stage_roots(rt, twz, Nz);
+  __syncthreads();
+  int j, i;
+  bool on;
+  xzs_row(g, blk, rr, j, i, on);
+  const long fstride = (long)g.my * g.kxn * Nz;
+  const H* src = hout + ((long)j * g.kxn + i) * Nz;
+  H h[3][RZ];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    H v[8];
+    if (l < W::NT) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = gld(src + f * fstride + l + 4 * RZ * q);
+    }
+    wfft32_t2l<RZ, -1>(v, h[f], buf, rt);
+  }
+  if (!on) return;
+  const double norm = 1.0 / ((double)g.mgalx * (double)Nz);
+  const double kx = g.alp * (double)i;
+  const double kx2 = kx * kx;
+#pragma unroll
+  for (int m = 0; m < RZ; ++m) {
+    const int k = wz_k(l + 32 * m, NZ, MZ, Nz);
+    if (k >= 0) {
+      const cplx h1 = crmul(norm, to_d(h[0][m])), h2 = crmul(norm, to_d(h[1][m])),
+                 h3 = crmul(norm, to_d(h[2][m]));
+      const double kz = kz_of(k, NZ, MZ, g.gam);
+      const long idx = sidx(g, j, i, k);
+      const cplx v = ld_io(vel, idx, g.io32 & 1);
+      st_io(dvordy, idx, cadd(mul_i(kx, h1), mul_i(kz, h3)), g.io32 & 2);
+      const cplx dd = mkc(h1.x - g.s * v.x, h1.y - g.s * v.y);
+      st_io(hgf, idx, cadd(mul_i(-kx, h3), mul_i(kz, dd)), g.io32 & 2);
+      const double kk = kx2 + kz * kz;
+      st_io(hvf, idx, mkc(-h2.x * kk, -h2.y * kk), g.io32 & 2);
+      if (i == 0 && k == 0) {
+        gstd(rf0u + j, h1.x);
+        gstd(rf0w + j, h3.x);
+      }
+    }
+  }
+}
+
+template <int RZ, class H>
+__global__ void __launch_bounds__(256)
+k_xzs_A(Geom g, const cplx* __restrict__ vor, const cplx* __restrict__ phi,
+        const cplx* __restrict__ hgf, const cplx* __restrict__ hvf, const cplx* __restrict__ dvordy,
+        const double* __restrict__ u00, const double* __restrict__ w00, const double* __restrict__ dudy,
+        const double* __restrict__ dwdy, double vbulk, H* __restrict__ hyb,
+        const cplx* __restrict__ twz, const DevScalars* sc) {
+  k_xzs_A_body<RZ, H>(g, vor, phi, hgf, hvf, dvordy, u00, w00, dudy, dwdy, vbulk, hyb, twz, sc,
+                      (int)xcd_block(blockIdx.x, gridDim.x));
+}
+template <int RZ, class H>
+__global__ void __launch_bounds__(256)
+k_xzs_C(Geom g, const H* __restrict__ hout, const cplx* __restrict__ vel, cplx* __restrict__ dvordy,
+        cplx* __restrict__ hgf, cplx* __restrict__ hvf, double* __restrict__ rf0u,
+        double* __restrict__ rf0w, const cplx* __restrict__ twz, const DevScalars* sc) {
+  k_xzs_C_body<RZ, H>(g, hout, vel, dvordy, hgf, hvf, rf0u, rf0w, twz, sc,
+                      (int)xcd_block(blockIdx.x, gridDim.x));
+}
+// LDS of k_xzs_A and k_xzs_C: the z roots and the 8 halves' FFT buffers
+inline size_t xzs_lds_AC(int RZ, size_t es) {
+  const int R0 = RZ, lw = R0 * WF32_EP > 4 * R0 * WF32_FP
+                              ? (R0 * WF32_EP > 8 * R0 * WF32_GP ? R0 * WF32_EP : 8 * R0 * WF32_GP)
+                              : (4 * R0 * WF32_FP > 8 * R0 * WF32_GP ? 4 * R0 * WF32_FP : 8 * R0 * WF32_GP);
+  return (size_t)(32 * RZ + XZS_ROWS * lw) * es;
+}
+// the z-plane sizes k_xzs_A and k_xzs_C serve
+inline bool xzs_z_size(int mgalz) { return mgalz == 64 || mgalz == 96; }
