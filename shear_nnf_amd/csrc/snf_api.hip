@@ -5,6 +5,7 @@
 // included here so the library is a single hipcc invocation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1718,12 +1719,16 @@ struct ClassRun {
   BoxTables d;
   hipGraphExec_t ge = nullptr;
   std::vector<hipEvent_t> ev;
+  std::vector<hipEvent_t> ready;   // each box's start of cross done
+  hipStream_t run = nullptr;       // the stream of the class's steps
+  double cost = 0.0;               // its share of the device, for the stream choice
   void release() {
     if (ge) (void)hipGraphExecDestroy(ge);
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     if (d.blob) (void)hipFree(d.blob);
     ge = nullptr;
     ev.clear();
+    ready.clear();
     d.blob = nullptr;
   }
 };
@@ -1734,10 +1739,45 @@ static int class_event(ClassRun& r, hipEvent_t* e) {
   return SNF_OK;
 }
 
-// every box's start of cross on its own stream, then nstep steps of the
-// whole class on the stream of its first box: pairs of steps replayed from
-// one captured graph, an odd last step launched directly
-static int class_enqueue(ClassRun& r, const std::vector<int>& resume, int nstep) {
+// Streams of the class runs, per device, created together on first use so
+// that they take the device's hardware queues (GPU_MAX_HW_QUEUES, HIP's
+// default 4) one each.  Work of streams that share a hardware queue runs one
+// after another; the classes are spread over these streams by their cost so
+// that the small classes' latency-bound kernels run beside the large ones.
+// Non-blocking: the blocking copies of other handles do not wait for them.
+// They live until the process ends.
+static constexpr int kRunStreamsMax = 8;
+// Returns SNF_OK and the streams' count in *ns.
+static int run_streams(int dev, hipStream_t* out, int* ns) {
+  static std::mutex mu;
+  static std::map<int, std::vector<hipStream_t>> pool;
+  std::lock_guard<std::mutex> lk(mu);
+  std::vector<hipStream_t>& v = pool[dev];
+  if (v.empty()) {
+    int nq = 4;
+    if (const char* e = getenv("GPU_MAX_HW_QUEUES")) nq = atoi(e);
+    nq = std::max(1, std::min(nq, kRunStreamsMax));
+    for (int q = 0; q < nq; ++q) {
+      hipStream_t st = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        for (hipStream_t x : v) (void)hipStreamDestroy(x);
+        v.clear();
+        return fail(SNF_DEVICE_ERROR, "stream");
+      }
+      v.push_back(st);
+    }
+  }
+  for (size_t q = 0; q < v.size(); ++q) out[q] = v[q];
+  *ns = (int)v.size();
+  return SNF_OK;
+}
+
+// every box's start of cross on its own stream, the class's box tables and
+// the captured graph of a pair of steps; nothing of the class's steps is
+// enqueued yet.  The table upload is a blocking copy, which waits for the
+// work of every (blocking) stream of the device: all classes are prepared
+// before any class's steps are launched, so that the classes run side by side.
+static int class_prepare(ClassRun& r, const std::vector<int>& resume, int nstep) {
   const int n = (int)r.hs.size();
   snf_handle* h0 = r.hs[0];
   const hipStream_t s = h0->stream;
@@ -1754,8 +1794,15 @@ static int class_enqueue(ClassRun& r, const std::vector<int>& resume, int nstep)
   if ((rc = tune_class(r.hs, s))) return rc;
   if ((rc = build_tables(r.hs, r.d))) return rc;
   table_attrs(h0, s);
-  int left = nstep;
-  if (left >= 2) {
+  {
+    hipEvent_t e;   // every box's start of cross is ordered before it
+    if ((rc = class_event(r, &e))) return rc;
+    HIPCHK(hipEventRecord(e, s));
+    r.ready.push_back(e);
+  }
+  const Geom& g = h0->g;
+  r.cost = (double)n * g.mgalx * g.mgalz * g.my;
+  if (nstep >= 2) {
     hipGraph_t gr = nullptr;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     enqueue_table_step(h0, r.d, n, 0, s);
@@ -1768,25 +1815,67 @@ static int class_enqueue(ClassRun& r, const std::vector<int>& resume, int nstep)
     } else {
       (void)hipGraphDestroy(gr);
     }
-    for (; left >= 2; left -= 2) {
-      if (r.ge) {
-        HIPCHK(hipGraphLaunch(r.ge, s));
-      } else {
-        enqueue_table_step(h0, r.d, n, 0, s);
-        enqueue_table_step(h0, r.d, n, 1, s);
-      }
+  }
+  return SNF_OK;
+}
+
+// the class's nstep steps (after class_prepare) on the stream of its first
+// box: pairs of steps replayed from the captured graph, an odd last step
+// launched directly; the other boxes' streams then wait for them
+static int class_launch(ClassRun& r, int nstep) {
+  const int n = (int)r.hs.size();
+  snf_handle* h0 = r.hs[0];
+  const hipStream_t s = r.run ? r.run : h0->stream;
+  int rc;
+  HIPCHK(hipSetDevice(h0->dev));
+  for (hipEvent_t e : r.ready) HIPCHK(hipStreamWaitEvent(s, e, 0));
+  int left = nstep;
+  for (; left >= 2; left -= 2) {
+    if (r.ge) {
+      HIPCHK(hipGraphLaunch(r.ge, s));
+    } else {
+      enqueue_table_step(h0, r.d, n, 0, s);
+      enqueue_table_step(h0, r.d, n, 1, s);
     }
   }
   if (left == 1) enqueue_table_step(h0, r.d, n, 0, s);
   HIPCHK(hipGetLastError());
   // three substeps a step flip each box's twist parity
   for (snf_handle* h : r.hs) h->cur ^= (nstep & 1);
-  // the other boxes' streams wait for the class's steps (advance_finish
-  // synchronises each box's own stream)
+  // advance_finish synchronises each box's own stream
   hipEvent_t done;
   if ((rc = class_event(r, &done))) return rc;
   HIPCHK(hipEventRecord(done, s));
-  for (int q = 1; q < n; ++q) HIPCHK(hipStreamWaitEvent(r.hs[q]->stream, done, 0));
+  for (int q = 0; q < n; ++q)
+    if (r.hs[q]->stream != s) HIPCHK(hipStreamWaitEvent(r.hs[q]->stream, done, 0));
+  return SNF_OK;
+}
+
+// the class runs' streams: largest cost first, each to the least loaded
+// run stream of its device (SNF_CLASS_STREAMS=0: each class on the stream
+// of its first box)
+static int assign_run_streams(std::vector<ClassRun>& runs) {
+  const char* e = getenv("SNF_CLASS_STREAMS");
+  if (e && std::string(e) == "0") return SNF_OK;
+  std::vector<int> order(runs.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::sort(order.begin(), order.end(),
+            [&](int a, int b) { return runs[a].cost > runs[b].cost; });
+  std::map<int, std::vector<double>> load;
+  for (int i : order) {
+    ClassRun& r = runs[i];
+    const int dev = r.hs[0]->dev;
+    hipStream_t st[kRunStreamsMax];
+    int ns = 0, rc;
+    if ((rc = run_streams(dev, st, &ns))) return rc;
+    std::vector<double>& l = load[dev];
+    if (l.empty()) l.assign(ns, 0.0);
+    int best = 0;
+    for (int q = 1; q < ns; ++q)
+      if (l[q] < l[best]) best = q;
+    l[best] += r.cost;
+    r.run = st[best];
+  }
   return SNF_OK;
 }
 
@@ -1838,22 +1927,42 @@ static int batch_enqueue(snf_handle* const* hs, int n, const AdvanceMark* m, int
   auto cleanup = [&]() {
     for (ClassRun& r : runs) r.release();
   };
-  for (int i : single)
-    if ((rc = advance_enqueue(hs[i], nstep, m[i].resume))) {
-      cleanup();
-      return rc;
+  auto drain = [&]() {
+    for (ClassRun& r : runs) {
+      if (r.run) (void)hipStreamSynchronize(r.run);
+      for (snf_handle* h : r.hs) (void)hipStreamSynchronize(h->stream);
     }
+  };
+  // every class is prepared (its blocking table upload done) before any
+  // steps are enqueued; then the single boxes and the classes' steps
   for (ClassRun& r : runs) {
     std::vector<int> resume;
     for (snf_handle* h : r.hs)
       for (int i = 0; i < n; ++i)
         if (hs[i] == h) resume.push_back(m[i].resume);
-    if ((rc = class_enqueue(r, resume, nstep))) {
-      for (snf_handle* h : r.hs) (void)hipStreamSynchronize(h->stream);
+    if ((rc = class_prepare(r, resume, nstep))) {
+      drain();
       cleanup();
       return rc;
     }
   }
+  if ((rc = assign_run_streams(runs))) {
+    drain();
+    cleanup();
+    return rc;
+  }
+  for (int i : single)
+    if ((rc = advance_enqueue(hs[i], nstep, m[i].resume))) {
+      drain();
+      cleanup();
+      return rc;
+    }
+  for (ClassRun& r : runs)
+    if ((rc = class_launch(r, nstep))) {
+      drain();
+      cleanup();
+      return rc;
+    }
   // the class runs' device tables are freed once their steps are done
   for (ClassRun& r : runs)
     for (snf_handle* h : r.hs) (void)hipStreamSynchronize(h->stream);
