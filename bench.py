@@ -33,6 +33,9 @@ def parse():
     ap.add_argument("--batch", type=int, default=128,
                     help="boxes of the ensemble leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the ensemble's box-table legs (FP64 and mixed), for a kernel "
+                         "trace of the ensemble; prints the ensemble object alone")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="precision of the main timed leg (mixed: no separate mixed leg)")
     ap.add_argument("--grid2", type=str, default="512,513,512",
@@ -243,7 +246,8 @@ def grid2_leg(snf, make_ic, grid, Re, explicit, steps, warmup, cmp_steps, seed):
     return res
 
 
-def batch_leg(snf, make_ic, nbox, steps, seed):
+def batch_leg(snf, make_ic, nbox, steps, seed,
+              modes=("sequential", "batched", "batched_mixed", "batched_streams")):
     """Ensemble of independent boxes (grids n^3 and n x (n+1) x n, n cycling
     over 32, 48, 64, 96; Re_z uniform in [500, 3000]; seeded fields): box-steps
     per second of snf_advance_batch (each size class advanced by one launch
@@ -270,6 +274,8 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
     rates = {}
     for mode, prec in (("sequential", "fp64"), ("batched", "fp64"), ("batched_mixed", "mixed"),
                        ("batched_streams", "fp64")):
+        if mode not in modes:
+            continue
         # batched: one launch per kernel for each size class (box tables);
         # batched_streams: every box on its own stream, for comparison
         os.environ["SNF_BATCH_TABLES"] = "0" if mode == "batched_streams" else "1"
@@ -284,6 +290,10 @@ def batch_leg(snf, make_ic, nbox, steps, seed):
         rates[mode] = sum(s.steps_done for s in st) / (t1 - t0)
         for b in bs:
             b.close()
+    if "sequential" not in rates:
+        return {"boxes": nbox, "steps": steps,
+                "grids": sorted({"%dx%dx%d" % g for g, _, _ in specs}), "unit": "box-steps/s",
+                **rates}
     return {"boxes": nbox, "steps": steps, "grids": sorted({"%dx%dx%d" % g for g, _, _ in specs}),
             "unit": "box-steps/s", "dtype": "f64 (batched_mixed: the mixed precision mode)",
             "batched": rates["batched"], "sequential": rates["sequential"],
@@ -307,7 +317,7 @@ def main():
     mgalx, my, mgalz = (int(x) for x in a.grid.split(","))
     seed = 20260001
     cpu = None
-    if not a.no_cpu and a.copy_probe <= 0:
+    if not a.no_cpu and a.copy_probe <= 0 and not a.batch_only:
         # before anything touches the GPU: the workers are fresh processes
         cg = tuple(int(x) for x in a.cpu_grid.split(",")) if a.cpu_grid else (mgalx, my, mgalz)
         procs = max(1, min(a.cpu_procs, os.cpu_count() or 1))
@@ -323,6 +333,10 @@ def main():
         ms = snf.copy_probe(nbytes, iters=20)
         print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms,
                           "GBs_read_plus_write": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    if a.batch_only:
+        print(json.dumps({"ensemble": batch_leg(snf, make_ic, a.batch, a.batch_steps, seed,
+                                                ("batched", "batched_mixed"))}), flush=True)
         return
     ic = make_ic(mgalx, my, mgalz, math.pi, 2 * math.pi, 1.0, seed=seed)
     grid = (mgalx, my, mgalz)
