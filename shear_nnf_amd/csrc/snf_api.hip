@@ -1757,6 +1757,7 @@ static int run_streams(int dev, hipStream_t* out, int* ns) {
     int nq = 4;
     if (const char* e = getenv("GPU_MAX_HW_QUEUES")) nq = atoi(e);
     nq = std::max(1, std::min(nq, kRunStreamsMax));
+    if (hipSetDevice(dev) != hipSuccess) return fail(SNF_DEVICE_ERROR, "hipSetDevice");
     for (int q = 0; q < nq; ++q) {
       hipStream_t st = nullptr;
       if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
